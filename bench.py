@@ -1,0 +1,140 @@
+"""Benchmark: PISO solver steps per second on one MI355X, reported as MLUPS.
+
+One "step" is one FvmSolverGpu.Step() (LES, momentum predictor with V
+Jacobi sweeps, Rhie-Chow pressure correction with P sweeps, C PISO correctors)
+over the whole grid, inputs resident in HBM.  Default workload: the urban
+case, 40 seeded buildings on a 256 (x) x 96 (height) x 128 (span) grid, V = P = 50,
+C = 2.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MLUPS per PISO step + %HBM peak; batched vs plain cases/s at ≤1e-3 rel L2"
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_cell_step(V: int, P: int, C: int) -> int:
+    """Algorithmic HBM bytes per cell per step: vectors 12 B, scalars 4 B,
+    flag 1 B, each array counted once per read and per write per kernel."""
+    return 263 + 77 * V + 29 * P + C * (123 + 33 * P)
+
+
+def cpu_baseline(budget_s: float = 15.0) -> dict:
+    """The CPU oracle (a scalar restatement of the reference kernels) on the
+    small isolated-building case, steps until ~budget_s of CPU time."""
+    from oracle.oracle import Oracle
+    from unitywindsimulation_amd import workloads
+    from unitywindsimulation_amd.geometry import noise_velocity
+
+    w = workloads.isolated_building()
+    o = Oracle(w.cf)
+    o.init_like_solver()
+    o.init_flags(w.flags)
+    o.set_velocity(noise_velocity(w.flags, w.noise_seed, w.noise_amp))
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        o.step(1)
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    X, Y, Z = w.grid
+    return {"value": round(X * Y * Z * n / dt / 1e6, 4), "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"{n} PISO steps of {w.name} (V=P=50, C=2), single-thread FP32 oracle"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="urban")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the solver step runs on a single MI355X and this build has no "
+              f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from unitywindsimulation_amd import FvmSolverHip, workloads
+    from unitywindsimulation_amd import _native as N
+
+    torch.cuda.set_device(0)
+    w = workloads.WORKLOADS[args.workload]()
+    cf = w.cf
+    X, Y, Z = w.grid
+    ncell = X * Y * Z
+    V, P, C = cf.velMaxNumIter, cf.presMaxNumIter, cf.PISONumCorrectors
+    s = FvmSolverHip(cf)
+    s.InitFlags(w.flags)
+    s.init_noise(w.noise_seed, w.noise_amp)
+    s.step(args.warmup)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1e3
+    mlups = ncell / (ms_step * 1e-3) / 1e6
+    b_step = bytes_per_cell_step(V, P, C)
+    hbm_frac = b_step * ncell / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+
+    # dominant kernel: the Jacobi sweep with the larger share of step time
+    t_vel = s.time_kernel(N.KERNEL_VEL_SWEEP, 20)
+    t_pres = s.time_kernel(N.KERNEL_PRES_SWEEP, 20)
+    share_vel, share_pres = V * t_vel, (1 + C) * P * t_pres
+    if share_vel >= share_pres:
+        kname, kms, kbytes = "k_vel_sweep", t_vel, 77 * ncell
+    else:
+        kname, kms, kbytes = "k_pres_sweep", t_pres, 29 * ncell
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    u = s.velocity(3)
+    finite = bool(torch.isfinite(torch.from_numpy(u)).all())
+    s.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "grid_xyz": [X, Y, Z], "cells": ncell, "vel_sweeps": V,
+                   "pres_sweeps": P, "piso_correctors": C, "les": "smagorinsky", "wall": "rough log law",
+                   "bytes_per_cell_step": b_step},
+        "hbm_frac": round(hbm_frac, 4),
+        "algorithmic_GBps": round(b_step * ncell / (ms_step * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": {"k_vel_sweep": round(t_vel, 5), "k_pres_sweep": round(t_pres, 5)}},
+        "finite": finite,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

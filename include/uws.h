@@ -1,0 +1,230 @@
+/*
+ * uws.h -- C ABI of the MI355X (gfx950) finite-volume incompressible wind
+ * solver step: the drop-in replacement for UnityWindSimulation's GPU solver
+ * (`IFvmSolver` implemented by `FvmSolverGpu`).
+ *
+ * Conventions
+ *  - Plain C, blittable types only; every entry point returns int
+ *    (UWS_OK == 0, negative = error class, see uws_strerror).
+ *  - Axes follow the reference: x streamwise (inlet X0, outlet Xn),
+ *    y up (ground Y0), z spanwise.  Cell (x, y, z) has the flat index
+ *    z*X*Y + y*X + x  (Assets/Code/FvmSolver/FvmSolverGpu.cs:947-950,
+ *    Assets/Resources/Shaders/SimShaders/FvmColl.compute:122-126).
+ *  - Host arrays are owned by the caller; the library copies in/out.
+ *    Vector fields cross the boundary as [cell][ncomp] float (ncomp 3 or 4,
+ *    the 4th lane written as 0), the reference's netCDF export layout
+ *    velField(time, z, y, x, 4) (Assets/Code/IO/NcFlowFieldExporter.cs:244-257).
+ *  - One handle = one HIP stream.  Calls on one handle must be serialised
+ *    by the caller; different handles may be driven from different threads.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *  - IFvmSolver                 Assets/Code/FvmSolver/IFvmSolver.cs:3-24
+ *  - FvmSolverGpu ctor          Assets/Code/FvmSolver/FvmSolverGpu.cs:173-222
+ *  - FluidSimConfig / enums     Assets/Code/FvmSolver/FluidSimConfig.cs:9-370
+ *  - RuntimeConfig.windSpeed    Assets/Code/RuntimeConfig/RuntimeConfig.cs:9
+ */
+#ifndef UWS_H
+#define UWS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UWS_ABI_VERSION 1
+
+/* ---- status codes (netCDF-style int status, CSnetCDF.cs:273-280) ---- */
+enum {
+    UWS_OK = 0,
+    UWS_ERR_INVALID_ARG = -1,   /* null pointer, bad enum, size mismatch */
+    UWS_ERR_CONFIG = -2,        /* inconsistent configuration */
+    UWS_ERR_ALLOC = -3,         /* device allocation failed */
+    UWS_ERR_DEVICE = -4,        /* HIP runtime error / no device */
+    UWS_ERR_UNSUPPORTED = -5,   /* combination the reference also rejects */
+    UWS_ERR_NONFINITE = -6      /* NaN/Inf detected by a norm check */
+};
+
+/* ---- enums: identical integer values to the reference ---- */
+/* VelBndCond (FluidSimConfig.cs:16-21, Constants.hlsl:8-10) */
+enum { UWS_VEL_FIXED_VALUE = 0, UWS_VEL_ZERO_GRAD = 1, UWS_VEL_SYMMETRY = 2 };
+/* PresBndCond (FluidSimConfig.cs:23-28, Constants.hlsl:12-14) */
+enum { UWS_PRES_FIXED_VALUE = 0, UWS_PRES_ZERO_GRAD = 1, UWS_PRES_SYMMETRY = 2 };
+/* FvmSolverType (FluidSimConfig.cs:30-34) */
+enum { UWS_SIMPLE = 0, UWS_PISO = 1 };
+/* TurbulenceModel (FluidSimConfig.cs:73-77) */
+enum { UWS_TURB_NONE = 0, UWS_TURB_SMAGORINSKY = 1 };
+/* InflowType (FluidSimConfig.cs:86-90); POWER_LAW is an extension */
+enum { UWS_INFLOW_CONSTANT = 0, UWS_INFLOW_LOG = 1, UWS_INFLOW_POWER_LAW = 2 };
+/* cell flags (Constants.hlsl:4-6) */
+enum { UWS_CELL_FLUID = 0, UWS_CELL_SOLID = 1 };
+/* wall model: 0 = reference smooth log law solved by Newton
+ * (WallFunc.compute:77-123); 1 = rough-wall log law with roughness z0
+ * (extension). */
+enum { UWS_WALL_SMOOTH = 0, UWS_WALL_ROUGH = 1 };
+/* boundary faces, the order of the reference's six boundary textures
+ * (FvmSolverGpu.cs:295-300) */
+enum { UWS_X0 = 0, UWS_XN = 1, UWS_Y0 = 2, UWS_YN = 3, UWS_Z0 = 4, UWS_ZN = 5 };
+
+/*
+ * Solver configuration: the FluidSimConfig fields the step reads, already
+ * derived by FluidSimConfig.Init() (FluidSimConfig.cs:322-370): grid_res is
+ * gridRes, dx is physDomainSize.x / gridResX.  ds = dx*dx, dv = dx*dx*dx and
+ * nu = mu/den are derived inside the library in float, as Init() does.
+ */
+typedef struct uws_config {
+    int32_t grid_res[3];          /* gridRes (x, y up, z span) */
+    float   dx;                   /* cell size [m] */
+    float   dt;                   /* time step [s] */
+    float   mu;                   /* dynamic viscosity */
+    float   den;                  /* density */
+    float   external_force[3];    /* externalForce */
+    int32_t vel_bnd[6];           /* velBndCondX0..Zn  (UWS_VEL_*) */
+    int32_t pres_bnd[6];          /* presBndCondX0..Zn (UWS_PRES_*) */
+    int32_t fvm_solver_type;      /* fvmSolverType (UWS_SIMPLE / UWS_PISO) */
+    int32_t piso_num_correctors;  /* PISONumCorrectors */
+    int32_t turbulence_model;     /* turbulenceModel */
+    float   smagorinsky_constant; /* smagorinskyConstant */
+    int32_t vel_max_num_iter;     /* velMaxNumIter (momentum Jacobi sweeps) */
+    int32_t pres_max_num_iter;    /* presMaxNumIter (pressure Jacobi sweeps) */
+    int32_t cal_residual;         /* calResidual: residual-based early exit */
+    int32_t vel_residual_check_interval;
+    float   vel_tolerance;
+    int32_t pres_residual_check_interval;
+    float   pres_tolerance;
+    /* extensions (no reference field) */
+    int32_t wall_slots;           /* 1: only the x-normal wall-viscosity slot
+                                     survives, as the reference runs with its
+                                     single-channel texture (FvmSolverGpu.cs:304);
+                                     3: all three slots, as the shaders are written */
+    int32_t wall_model;           /* UWS_WALL_SMOOTH / UWS_WALL_ROUGH */
+    float   wall_z0;              /* roughness length for UWS_WALL_ROUGH [m] */
+    int32_t reserved[8];          /* must be zero */
+} uws_config;
+
+typedef struct uws_solver* uws_handle;
+
+/* Fill *cfg with the FluidSimConfig field defaults (FluidSimConfig.cs:113-178)
+ * for a grid of the given resolution and dx = 10/gridResX (physDomainSize.x
+ * default 10), plus wall_slots = 1, wall_model = smooth. */
+int uws_config_default(uws_config* cfg);
+
+const char* uws_strerror(int status);
+int uws_abi_version(void);
+
+/* FvmSolverGpu(FluidSimConfig, RuntimeConfig) (FvmSolverGpu.cs:173-222):
+ * allocates every field on the current HIP device, initialises the flag
+ * field to all-fluid (InitFlagField, :935-939), the velocity/pressure fields
+ * to zero and all boundary textures to zero (SetZeroVelBndCond, :989-994).
+ * The caller then applies InitVelPresFields / SetFixedValueVelBndCond with
+ * the inflow it wants (uws_init_vel_pres_fields, uws_set_fixed_value_vel_bnd_cond). */
+int uws_create(const uws_config* cfg, uws_handle* out);
+int uws_destroy(uws_handle h);
+int uws_get_config(uws_handle h, uws_config* out);
+int uws_num_cells(uws_handle h, int64_t* n);
+
+/* IFvmSolver.InitFlags(int[] flags) (FvmSolverGpu.cs:972-987,
+ * InitShaderColl.compute:345-364): flags[z*X*Y + y*X + x] > 0.5 marks a solid
+ * cell; solid cells get zero velocity in every velocity buffer.  n must be X*Y*Z. */
+int uws_init_flags(uws_handle h, const int32_t* flags, int64_t n);
+
+/* SolidType.Box (FvmSolverGpu.cs:963-970, InitShaderColl.compute:302-343):
+ * the reference's hard-coded box x in [20,24), y in [0,8), z in [18,22). */
+int uws_init_box(uws_handle h);
+
+/* IFvmSolver.InitVelPresFields() (FvmSolverGpu.cs:790-896): inflow_type
+ * UWS_INFLOW_CONSTANT sets u = (wind_speed, 0, 0) everywhere
+ * (InitShaderColl.compute:94-110); UWS_INFLOW_LOG sets the log profile with
+ * y = j*dx (InitShaderColl.compute:112-132).  p, p', u' are zeroed.  Like the
+ * reference this does not look at the flag field. */
+int uws_init_vel_pres_fields(uws_handle h, int32_t inflow_type, float wind_speed);
+
+/* IFvmSolver.SetFixedValueVelBndCond() (FvmSolverGpu.cs:996-1022):
+ * CONSTANT fills X0 with (wind_speed,0,0) and Zn with vel_zn
+ * (InitShaderColl.compute:252-265); LOG fills X0 with the log profile at
+ * y = (j+1/2)dx (InitShaderColl.compute:272-285).  POWER_LAW (extension) fills
+ * X0 with u = wind_speed * ((j+1/2)dx / ref_height)^alpha. vel_zn may be NULL. */
+int uws_set_fixed_value_vel_bnd_cond(uws_handle h, int32_t inflow_type, float wind_speed,
+                                     const float vel_zn[3], float ref_height, float alpha);
+
+/* Upload one boundary texture (face UWS_X0..UWS_ZN) as [j][i][3] float where
+ * (i, j) are the reference texture coordinates: X faces (y, z), Y faces (x, z),
+ * Z faces (x, y) (FvmSolverGpu.cs:295-300).  Download mirrors it. */
+int uws_set_bnd_field(uws_handle h, int32_t face, const float* xyz, int64_t n);
+int uws_get_bnd_field(uws_handle h, int32_t face, float* xyz, int64_t n);
+
+/* Velocity state upload/download, [cell][ncomp], ncomp in {3, 4}.  Upload
+ * writes every velocity buffer (as the reference init kernels do). */
+int uws_set_velocity(uws_handle h, const float* src, int32_t ncomp, int64_t n);
+int uws_get_velocity(uws_handle h, float* dst, int32_t ncomp, int64_t n);
+int uws_set_pressure(uws_handle h, const float* src, int64_t n);
+int uws_get_pressure(uws_handle h, float* dst, int64_t n);
+int uws_get_flags(uws_handle h, int32_t* dst, int64_t n);
+
+/* Seeded initial condition (extension): u = U(-amp, amp) per component in
+ * fluid cells (SplitMix64 stream from seed), 0 in solid cells; p = 0. */
+int uws_init_noise(uws_handle h, uint64_t seed, float amp);
+
+/* FvmSolverGpu.Step() (FvmSolverGpu.cs:1050-1121) run nsteps times: LES
+ * (wall function, eddy viscosity, transposed-stress term), momentum
+ * predictor with velMaxNumIter Jacobi sweeps, Rhie-Chow pressure correction
+ * with presMaxNumIter sweeps, velocity correction, PISO correctors.  Whole
+ * steps only (no 30 ms time slicing).  Enqueued on the handle's stream. */
+int uws_step(uws_handle h, int32_t nsteps);
+int uws_synchronize(uws_handle h);
+/* currSimStep / currPhyTime bookkeeping (FvmSolverGpu.cs:1108-1109) */
+int uws_get_time(uws_handle h, int64_t* step, double* phys_time);
+
+/* ---- intermediate fields, for per-kernel parity and diagnostics ---- */
+enum {
+    UWS_FIELD_VEL = 0,        /* u^n, velFieldLastTime (3) */
+    UWS_FIELD_PRES = 1,       /* p^n, presFieldLastTime (1) */
+    UWS_FIELD_VEL_WORK = 2,   /* velField within a step: u*, then corrected u (3) */
+    UWS_FIELD_VEL_CORR = 3,   /* u', velCorrectField (3) */
+    UWS_FIELD_PRES_CORR = 4,  /* p', presCorrectField (1) */
+    UWS_FIELD_EDDY_VIS = 5,   /* nu_t, eddyVisField (1) */
+    UWS_FIELD_WALL_VIS = 6,   /* nu_w slots (W, S, D faces), faceEddyVisField (3) */
+    UWS_FIELD_B = 7,          /* bField (3) */
+    UWS_FIELD_D = 8,          /* DField (3) */
+    UWS_FIELD_DP = 9,         /* DFieldPresCorrect (1) */
+    UWS_FIELD_BP = 10,        /* bFieldPresCorrect (1) */
+    UWS_FIELD_AODU = 11,      /* AodUCorrectField (3) */
+    UWS_FIELD_COUNT = 12
+};
+int uws_field_ncomp(int32_t field);
+int uws_get_field(uws_handle h, int32_t field, float* dst, int64_t n_floats);
+int uws_set_field(uws_handle h, int32_t field, const float* src, int64_t n_floats);
+
+/* The step split at the reference's kernel boundaries (FvmSolverGpu.cs
+ * 1050-1121).  Running UWS_STAGE_WALL_FUNC .. UWS_STAGE_FINISH in order is
+ * one uws_step (stages whose feature is disabled are no-ops). */
+enum {
+    UWS_STAGE_WALL_FUNC = 0,     /* CSCalLogLawWallFunc */
+    UWS_STAGE_EDDY_VIS = 1,      /* CSCalEddyVis */
+    UWS_STAGE_LES_TERM = 2,      /* CSCalLesDeferCorrectTerm */
+    UWS_STAGE_VEL_PRECOMP = 3,   /* CSVelPredictPreCompute */
+    UWS_STAGE_VEL_SWEEPS = 4,    /* velMaxNumIter x CSVelPredict */
+    UWS_STAGE_PRES_PRECOMP = 5,  /* CSPresCorrectRhieChowPreCompute */
+    UWS_STAGE_PRES_SWEEPS = 6,   /* presMaxNumIter x CSPresCorrectRhieChow */
+    UWS_STAGE_APPLY_PRES = 7,    /* CSApplyPresCorrection (+ normalisation) */
+    UWS_STAGE_VEL_CORRECT = 8,   /* CSApplyVelCorrectionDivergenceTheorem, zero p' */
+    UWS_STAGE_PISO = 9,          /* one PISO corrector (PisoColl.compute) */
+    UWS_STAGE_FINISH = 10,       /* u^{n+1} <- u, time bookkeeping */
+    UWS_STAGE_COUNT = 11
+};
+int uws_run_stage(uws_handle h, int32_t stage);
+
+/* Diagnostics (no reference counterpart): average duration in milliseconds of
+ * one launch of a hot-loop kernel, timed with HIP events on the handle's
+ * stream over `reps` back-to-back launches on the current step data.
+ * UWS_KERNEL_VEL_SWEEP reads u^n and writes a scratch velocity buffer;
+ * UWS_KERNEL_PRES_SWEEP reads and writes the p' scratch pair.  The solver
+ * state (u^n, p^n) is not changed.  Call after at least one uws_step. */
+enum { UWS_KERNEL_VEL_SWEEP = 0, UWS_KERNEL_PRES_SWEEP = 1 };
+int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UWS_H */

@@ -1,0 +1,140 @@
+"""ctypes binding of the C ABI declared in ``include/uws.h``.
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libuws_hip.so"
+
+# ---- constants mirrored from uws.h ----
+UWS_OK = 0
+VEL_FIXED_VALUE, VEL_ZERO_GRAD, VEL_SYMMETRY = 0, 1, 2
+PRES_FIXED_VALUE, PRES_ZERO_GRAD, PRES_SYMMETRY = 0, 1, 2
+SIMPLE, PISO = 0, 1
+TURB_NONE, TURB_SMAGORINSKY = 0, 1
+INFLOW_CONSTANT, INFLOW_LOG, INFLOW_POWER_LAW = 0, 1, 2
+CELL_FLUID, CELL_SOLID = 0, 1
+WALL_SMOOTH, WALL_ROUGH = 0, 1
+X0, XN, Y0, YN, Z0, ZN = range(6)
+
+FIELD_VEL, FIELD_PRES, FIELD_VEL_WORK, FIELD_VEL_CORR, FIELD_PRES_CORR = 0, 1, 2, 3, 4
+FIELD_EDDY_VIS, FIELD_WALL_VIS, FIELD_B, FIELD_D, FIELD_DP, FIELD_BP, FIELD_AODU = 5, 6, 7, 8, 9, 10, 11
+FIELD_NCOMP = {0: 3, 1: 1, 2: 3, 3: 3, 4: 1, 5: 1, 6: 3, 7: 3, 8: 3, 9: 1, 10: 1, 11: 3}
+
+KERNEL_VEL_SWEEP, KERNEL_PRES_SWEEP = 0, 1
+
+(STAGE_WALL_FUNC, STAGE_EDDY_VIS, STAGE_LES_TERM, STAGE_VEL_PRECOMP, STAGE_VEL_SWEEPS, STAGE_PRES_PRECOMP,
+ STAGE_PRES_SWEEPS, STAGE_APPLY_PRES, STAGE_VEL_CORRECT, STAGE_PISO, STAGE_FINISH) = range(11)
+
+
+class uws_config(ctypes.Structure):
+    """Mirror of ``struct uws_config`` (include/uws.h)."""
+
+    _fields_ = [
+        ("grid_res", c_int32 * 3),
+        ("dx", c_float),
+        ("dt", c_float),
+        ("mu", c_float),
+        ("den", c_float),
+        ("external_force", c_float * 3),
+        ("vel_bnd", c_int32 * 6),
+        ("pres_bnd", c_int32 * 6),
+        ("fvm_solver_type", c_int32),
+        ("piso_num_correctors", c_int32),
+        ("turbulence_model", c_int32),
+        ("smagorinsky_constant", c_float),
+        ("vel_max_num_iter", c_int32),
+        ("pres_max_num_iter", c_int32),
+        ("cal_residual", c_int32),
+        ("vel_residual_check_interval", c_int32),
+        ("vel_tolerance", c_float),
+        ("pres_residual_check_interval", c_int32),
+        ("pres_tolerance", c_float),
+        ("wall_slots", c_int32),
+        ("wall_model", c_int32),
+        ("wall_z0", c_float),
+        ("reserved", c_int32 * 8),
+    ]
+
+
+class UwsError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {strerror(status)} ({status})")
+
+
+# every exported symbol with (restype, argtypes); kept in the order of uws.h
+SIGNATURES = {
+    "uws_config_default": (c_int, [POINTER(uws_config)]),
+    "uws_strerror": (c_char_p, [c_int]),
+    "uws_abi_version": (c_int, []),
+    "uws_create": (c_int, [POINTER(uws_config), POINTER(c_void_p)]),
+    "uws_destroy": (c_int, [c_void_p]),
+    "uws_get_config": (c_int, [c_void_p, POINTER(uws_config)]),
+    "uws_num_cells": (c_int, [c_void_p, POINTER(c_int64)]),
+    "uws_init_flags": (c_int, [c_void_p, c_void_p, c_int64]),
+    "uws_init_box": (c_int, [c_void_p]),
+    "uws_init_vel_pres_fields": (c_int, [c_void_p, c_int32, c_float]),
+    "uws_set_fixed_value_vel_bnd_cond": (c_int, [c_void_p, c_int32, c_float, c_void_p, c_float, c_float]),
+    "uws_set_bnd_field": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
+    "uws_get_bnd_field": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
+    "uws_set_velocity": (c_int, [c_void_p, c_void_p, c_int32, c_int64]),
+    "uws_get_velocity": (c_int, [c_void_p, c_void_p, c_int32, c_int64]),
+    "uws_set_pressure": (c_int, [c_void_p, c_void_p, c_int64]),
+    "uws_get_pressure": (c_int, [c_void_p, c_void_p, c_int64]),
+    "uws_get_flags": (c_int, [c_void_p, c_void_p, c_int64]),
+    "uws_init_noise": (c_int, [c_void_p, c_uint64, c_float]),
+    "uws_step": (c_int, [c_void_p, c_int32]),
+    "uws_synchronize": (c_int, [c_void_p]),
+    "uws_get_time": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_double)]),
+    "uws_field_ncomp": (c_int, [c_int32]),
+    "uws_get_field": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
+    "uws_set_field": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
+    "uws_run_stage": (c_int, [c_void_p, c_int32]),
+    "uws_time_kernel": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_double)]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load ``libuws_hip.so`` (built by ``unitywindsimulation_amd.build``)."""
+    global _lib
+    if _lib is None:
+        path = Path(os.environ.get("UWS_LIB", LIB_PATH))
+        if not path.exists():
+            raise ImportError(
+                f"HIP library {path} is missing: build it with "
+                "`python -m unitywindsimulation_amd.build` (there is no CPU fallback)"
+            )
+        dll = ctypes.CDLL(str(path))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(dll, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = dll
+    return _lib
+
+
+def strerror(status: int) -> str:
+    try:
+        return lib().uws_strerror(status).decode()
+    except ImportError:
+        return "library not loaded"
+
+
+def check(status: int, what: str) -> None:
+    if status != UWS_OK:
+        raise UwsError(status, what)
+
+
+def ptr(a: np.ndarray) -> c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(c_void_p)

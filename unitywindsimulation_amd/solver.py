@@ -1,0 +1,204 @@
+"""Host-side mirror of the reference's solver plugin interface.
+
+``IFvmSolver`` restates ``Assets/Code/FvmSolver/IFvmSolver.cs:3-24``;
+``FvmSolverHip`` implements it over the C ABI of ``libuws_hip.so`` the way
+``FvmSolverGpu`` (``Assets/Code/FvmSolver/FvmSolverGpu.cs``) implements it over
+Unity compute shaders.  Fields are returned as numpy arrays: velocity
+``[z][y][x][4]`` (the reference's float4 texel, w = 0), pressure ``[z][y][x]``,
+flags ``[z][y][x]`` int32.
+"""
+from __future__ import annotations
+
+import abc
+import ctypes
+from ctypes import byref, c_double, c_int64, c_void_p
+
+import numpy as np
+
+from . import _native as N
+from .config import FluidSimConfig, InflowType, RuntimeConfig
+
+
+class IFvmSolver(abc.ABC):
+    """IFvmSolver (IFvmSolver.cs:3-24)."""
+
+    @abc.abstractmethod
+    def Step(self) -> bool: ...
+
+    @abc.abstractmethod
+    def InitFlags(self, flags) -> None: ...
+
+    @abc.abstractmethod
+    def InitVelPresFields(self) -> None: ...
+
+    @abc.abstractmethod
+    def InitVelPresFieldsFromBackGroundFlow(self, cfBackGround, velTexBackGround, presTexBackGround) -> None: ...
+
+    @abc.abstractmethod
+    def SetFixedValueVelBndCond(self) -> None: ...
+
+    @abc.abstractmethod
+    def ChangePhysFieldPos(self) -> None: ...
+
+    @abc.abstractmethod
+    def ChangePhysDomainSize(self) -> None: ...
+
+    @abc.abstractmethod
+    def GetVelField(self): ...
+
+    @abc.abstractmethod
+    def GetPresField(self): ...
+
+    @abc.abstractmethod
+    def GetFlagField(self): ...
+
+
+class FvmSolverHip(IFvmSolver):
+    """FvmSolverGpu's behaviour on MI355X through ``libuws_hip.so``.
+
+    Constructor = ``FvmSolverGpu(FluidSimConfig, RuntimeConfig)`` (:173-222):
+    fields allocated, flags all fluid, fields initialised by
+    ``InitVelPresFields``, boundary textures zeroed then filled by
+    ``SetFixedValueVelBndCond``; ``SolidType.Box`` applies the hard-coded box.
+    """
+
+    def __init__(self, cf: FluidSimConfig, rcf: RuntimeConfig | None = None):
+        self.cf = cf
+        cf.Init()
+        self.rcf = rcf or RuntimeConfig.from_config(cf)
+        self._lib = N.lib()
+        self._cfg = cf.to_c()
+        h = c_void_p()
+        N.check(self._lib.uws_create(byref(self._cfg), byref(h)), "uws_create")
+        self._h = h
+        X, Y, Z = cf.gridRes
+        self.shape = (Z, Y, X)
+        self.n = X * Y * Z
+        self.InitVelPresFields()
+        if int(cf.solidType) == 1:  # SolidType.Box (FvmSolverGpu.cs:216-217)
+            N.check(self._lib.uws_init_box(self._h), "uws_init_box")
+        self.SetFixedValueVelBndCond()
+
+    # ---- lifetime ----
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.uws_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- IFvmSolver ----
+    def Step(self) -> bool:
+        """One whole step; returns readyForRender (always True: no time slicing)."""
+        self.step(1)
+        return True
+
+    def InitFlags(self, flags) -> None:
+        a = np.ascontiguousarray(np.asarray(flags, dtype=np.int32).reshape(-1))
+        N.check(self._lib.uws_init_flags(self._h, N.ptr(a), a.size), "uws_init_flags")
+
+    def InitVelPresFields(self) -> None:
+        it = int(self.cf.inflowType)
+        # power-law inflow has no reference init kernel; the field starts at rest
+        if it == int(InflowType.PowerLaw):
+            N.check(self._lib.uws_init_vel_pres_fields(self._h, N.INFLOW_CONSTANT, 0.0), "uws_init_vel_pres_fields")
+        else:
+            N.check(self._lib.uws_init_vel_pres_fields(self._h, it, float(self.rcf.windSpeed)),
+                    "uws_init_vel_pres_fields")
+
+    def InitVelPresFieldsFromBackGroundFlow(self, cfBackGround, velTexBackGround, presTexBackGround) -> None:
+        raise NotImplementedError("nested background-flow initialisation is outside the solver-step path")
+
+    def SetFixedValueVelBndCond(self) -> None:
+        vz = (ctypes.c_float * 3)(*[float(v) for v in self.cf.velZn])
+        N.check(self._lib.uws_set_fixed_value_vel_bnd_cond(
+            self._h, int(self.cf.inflowType), float(self.rcf.windSpeed), ctypes.cast(vz, c_void_p),
+            float(self.cf.inflowRefHeight), float(self.cf.inflowAlpha)), "uws_set_fixed_value_vel_bnd_cond")
+
+    def ChangePhysFieldPos(self) -> None:
+        self.InitVelPresFields()
+
+    def ChangePhysDomainSize(self) -> None:
+        raise NotImplementedError("the grid of a handle is fixed; create a new FvmSolverHip")
+
+    def GetVelField(self) -> np.ndarray:
+        return self.velocity(4)
+
+    def GetPresField(self) -> np.ndarray:
+        return self.pressure()
+
+    def GetFlagField(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        N.check(self._lib.uws_get_flags(self._h, N.ptr(out), self.n), "uws_get_flags")
+        return out.reshape(self.shape)
+
+    # ---- extensions ----
+    def step(self, nsteps: int) -> None:
+        N.check(self._lib.uws_step(self._h, int(nsteps)), "uws_step")
+        s, t = c_int64(), c_double()
+        self._lib.uws_get_time(self._h, byref(s), byref(t))
+        self.cf._advance(int(s.value), float(t.value))
+
+    def synchronize(self) -> None:
+        N.check(self._lib.uws_synchronize(self._h), "uws_synchronize")
+
+    def init_noise(self, seed: int, amp: float) -> None:
+        N.check(self._lib.uws_init_noise(self._h, int(seed), float(amp)), "uws_init_noise")
+
+    def velocity(self, ncomp: int = 3) -> np.ndarray:
+        out = np.empty((self.n, ncomp), dtype=np.float32)
+        N.check(self._lib.uws_get_velocity(self._h, N.ptr(out), ncomp, self.n), "uws_get_velocity")
+        return out.reshape(*self.shape, ncomp)
+
+    def set_velocity(self, u) -> None:
+        a = np.ascontiguousarray(np.asarray(u, dtype=np.float32).reshape(self.n, -1))
+        N.check(self._lib.uws_set_velocity(self._h, N.ptr(a), a.shape[1], self.n), "uws_set_velocity")
+
+    def pressure(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float32)
+        N.check(self._lib.uws_get_pressure(self._h, N.ptr(out), self.n), "uws_get_pressure")
+        return out.reshape(self.shape)
+
+    def set_pressure(self, p) -> None:
+        a = np.ascontiguousarray(np.asarray(p, dtype=np.float32).reshape(-1))
+        N.check(self._lib.uws_set_pressure(self._h, N.ptr(a), self.n), "uws_set_pressure")
+
+    def field(self, fid: int) -> np.ndarray:
+        nc = N.FIELD_NCOMP[fid]
+        out = np.empty(self.n * nc, dtype=np.float32)
+        N.check(self._lib.uws_get_field(self._h, fid, N.ptr(out), out.size), f"uws_get_field({fid})")
+        return out.reshape(self.n, nc) if nc == 3 else out
+
+    def set_field(self, fid: int, a) -> None:
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+        N.check(self._lib.uws_set_field(self._h, fid, N.ptr(a), a.size), f"uws_set_field({fid})")
+
+    def bnd_field(self, face: int) -> np.ndarray:
+        X, Y, Z = self.cf.gridRes
+        n = (Y if face < 2 else X) * (Z if face < 4 else Y)
+        out = np.empty((n, 3), dtype=np.float32)
+        N.check(self._lib.uws_get_bnd_field(self._h, face, N.ptr(out), n), "uws_get_bnd_field")
+        return out
+
+    def set_bnd_field(self, face: int, xyz) -> None:
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float32).reshape(-1, 3))
+        N.check(self._lib.uws_set_bnd_field(self._h, face, N.ptr(a), a.shape[0]), "uws_set_bnd_field")
+
+    def run_stage(self, stage: int) -> None:
+        N.check(self._lib.uws_run_stage(self._h, int(stage)), f"uws_run_stage({stage})")
+
+    def time_kernel(self, kernel: int, reps: int = 20) -> float:
+        """Average milliseconds per launch of a hot-loop kernel (HIP events on the solver's stream)."""
+        ms = c_double()
+        N.check(self._lib.uws_time_kernel(self._h, int(kernel), int(reps), byref(ms)), "uws_time_kernel")
+        return float(ms.value)

@@ -1,0 +1,117 @@
+"""Synthetic wind-engineering workloads (grids, buildings, inflow, physics).
+
+Axes are the reference's: x streamwise, y up, z spanwise; a grid written
+``length x span x height`` in the workload descriptions is ``gridRes =
+(length, height, span)`` here.
+
+Common physics (all workloads): dx = 1 m, air (mu = 1.5e-5, den = 1), inlet
+X0 fixed-value power law u = 5 (y/16)^0.25 m/s at the cell-centre height
+y = (j + 1/2) dx, zero-gradient outlet Xn with fixed (zero) pressure,
+ground Y0 a no-slip wall, free-slip (symmetry) top and sides, pressure zero
+gradient elsewhere (the boundary set of the reference's AijBuilding0 case,
+Code/Config/AijBuilding0.asset:35-46).  Smagorinsky LES with Cs = 0.1, the
+rough-wall log law (z0 = 0.01 m) on the ground and all building faces (all
+three wall-viscosity slots), PISO with 2 correctors, 50 momentum and 50
+pressure Jacobi sweeps, dt from CFL 0.5 on the largest inlet speed.  The
+initial velocity is U(-0.05, 0.05) m/s noise per component in fluid cells.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .config import (FluidSimConfig, FvmSolverType, InflowType, PresBndCond, TurbulenceModel, VelBndCond,
+                     WallModel)
+from .geometry import Box, lattice_layout, rasterise
+
+U_REF, H_REF, ALPHA = 5.0, 16.0, 0.25
+
+
+@dataclass
+class Workload:
+    name: str
+    cf: FluidSimConfig
+    flags: np.ndarray            # int32 [X*Y*Z]
+    noise_seed: int
+    noise_amp: float = 0.05
+    steps: int = 200
+    mean_last: int = 0
+    boxes: list = field(default_factory=list)
+
+    @property
+    def grid(self):
+        return self.cf.gridRes
+
+    @property
+    def ncells(self) -> int:
+        X, Y, Z = self.grid
+        return X * Y * Z
+
+
+def power_law_umax(ny: int, dx: float = 1.0) -> float:
+    return U_REF * ((ny - 0.5) * dx / H_REF) ** ALPHA
+
+
+def base_config(grid, vel_iters=50, pres_iters=50, correctors=2) -> FluidSimConfig:
+    X, Y, Z = grid
+    dt = 0.5 * 1.0 / power_law_umax(Y)
+    cf = FluidSimConfig(
+        physDomainSize=(float(X), float(Y), float(Z)), gridResX=X, dt=dt, mu=1.5e-5, den=1.0,
+        velBndCondX0=VelBndCond.FixedValue, velBndCondXn=VelBndCond.ZeroGrad, velBndCondY0=VelBndCond.FixedValue,
+        velBndCondYn=VelBndCond.Symmetry, velBndCondZ0=VelBndCond.Symmetry, velBndCondZn=VelBndCond.Symmetry,
+        presBndCondX0=PresBndCond.ZeroGrad, presBndCondXn=PresBndCond.FixedValue,
+        presBndCondY0=PresBndCond.ZeroGrad, presBndCondYn=PresBndCond.Symmetry,
+        presBndCondZ0=PresBndCond.Symmetry, presBndCondZn=PresBndCond.Symmetry,
+        velX0=(U_REF, 0.0, 0.0), fvmSolverType=FvmSolverType.PISO, PISONumCorrectors=correctors,
+        turbulenceModel=TurbulenceModel.Smagorinsky, smagorinskyConstant=0.1, inflowType=InflowType.PowerLaw,
+        velMaxNumIter=vel_iters, presMaxNumIter=pres_iters, wallSlots=3, wallModel=WallModel.RoughLogLaw,
+        wallZ0=0.01, inflowRefHeight=H_REF, inflowAlpha=ALPHA,
+    )
+    cf.Init()
+    assert cf.gridRes == tuple(grid), (cf.gridRes, grid)
+    return cf
+
+
+def isolated_building(grid=(128, 64, 64), b=8) -> Workload:
+    """A b x b x 2b square prism in a 128 x 64 x 64 grid (the small plain case):
+    x in [32, 40), y in [0, 16), z in [28, 36) for b = 8 (a quarter of the
+    length downstream of the inlet, centred spanwise)."""
+    X, Y, Z = grid
+    x0 = X // 4
+    z0 = Z // 2 - b // 2
+    boxes = [Box(x0, x0 + b, z0, z0 + b, 2 * b)]
+    cf = base_config(grid)
+    return Workload(f"isolated_building_{X}x{Y}x{Z}", cf, rasterise(grid, boxes), noise_seed=0, steps=200,
+                    boxes=boxes)
+
+
+def urban(grid=(256, 96, 128), n=40, lattice=(8, 5), seed=1, height=(4, 24), steps=1000, mean_last=500) -> Workload:
+    """Urban case: n seeded rectangular buildings on a jittered lattice
+    (footprints U{4..12} per side, heights U{height}), 256 x 128 span x 96 high."""
+    boxes = lattice_layout(grid, n, lattice, seed, foot=(4, 12), height=height)
+    cf = base_config(grid)
+    X, Y, Z = grid
+    return Workload(f"urban_{n}bldg_{X}x{Y}x{Z}", cf, rasterise(grid, boxes), noise_seed=seed, steps=steps,
+                    mean_last=mean_last, boxes=boxes)
+
+
+def tiny(grid=(16, 8, 12), seed=7, les=True, piso=True, vel_iters=6, pres_iters=8) -> Workload:
+    """A few-thousand-cell case with two buildings, for oracle-sized checks."""
+    X, Y, Z = grid
+    boxes = [Box(4, 7, 3, 6, 4), Box(9, 11, 7, 10, 6)]
+    cf = base_config(grid, vel_iters=vel_iters, pres_iters=pres_iters)
+    if not les:
+        cf.turbulenceModel = TurbulenceModel.None_
+    if not piso:
+        cf.fvmSolverType = FvmSolverType.SIMPLE
+    cf.Init()
+    return Workload(f"tiny_{X}x{Y}x{Z}", cf, rasterise(grid, boxes), noise_seed=seed, steps=4, boxes=boxes)
+
+
+WORKLOADS = {
+    "isolated_building": isolated_building,
+    "urban": urban,
+    "tiny": tiny,
+}
