@@ -1,0 +1,85 @@
+"""The C-ABI library loads and exports every function include/uws.h declares;
+argument validation and error reporting work without a GPU."""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from unitywindsimulation_amd import _native as N
+from unitywindsimulation_amd import workloads
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "uws.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(uws_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "uws_create" in names and "uws_step" in names and len(names) >= 25
+
+
+def test_every_declared_symbol_exported():
+    lib = N.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert set(declared_functions()) == set(N.SIGNATURES)
+
+
+def test_config_struct_size():
+    # 3 ints + 4 floats + 3 floats + 12 ints + 3 ints + 1 float + 3 ints + int float int float + 2 int 1 float + 8
+    assert ctypes.sizeof(N.uws_config) == 4 * (3 + 4 + 3 + 12 + 3 + 1 + 2 + 1 + 4 + 3 + 8)
+
+
+def test_strerror_and_version():
+    lib = N.lib()
+    assert lib.uws_abi_version() == 1
+    assert lib.uws_strerror(0) == b"no error"
+    assert b"configuration" in lib.uws_strerror(-2)
+
+
+def test_defaults_match_fluidsimconfig():
+    c = N.uws_config()
+    assert N.lib().uws_config_default(ctypes.byref(c)) == 0
+    assert c.vel_max_num_iter == 50 and c.pres_max_num_iter == 400 and c.piso_num_correctors == 2
+    assert abs(c.smagorinsky_constant - 0.15) < 1e-7 and abs(c.mu - 0.005) < 1e-9
+    assert list(c.pres_bnd) == [1] * 6 and list(c.vel_bnd) == [0] * 6
+
+
+@pytest.mark.parametrize("field,value", [("dx", 0.0), ("dt", -1.0), ("den", 0.0), ("wall_slots", 2),
+                                         ("fvm_solver_type", 7), ("turbulence_model", 3)])
+def test_invalid_config_rejected_before_device(field, value):
+    c = workloads.tiny().cf.to_c()
+    setattr(c, field, value)
+    h = ctypes.c_void_p()
+    rc = N.lib().uws_create(ctypes.byref(c), ctypes.byref(h))
+    assert rc == -2 and not h.value
+
+
+def test_invalid_bnd_and_grid_rejected():
+    c = workloads.tiny().cf.to_c()
+    c.vel_bnd[3] = 5
+    h = ctypes.c_void_p()
+    assert N.lib().uws_create(ctypes.byref(c), ctypes.byref(h)) == -2
+    c = workloads.tiny().cf.to_c()
+    c.grid_res[1] = 0
+    assert N.lib().uws_create(ctypes.byref(c), ctypes.byref(h)) == -2
+    c = workloads.tiny().cf.to_c()
+    c.reserved[0] = 1
+    assert N.lib().uws_create(ctypes.byref(c), ctypes.byref(h)) == -2
+
+
+def test_null_handle_calls_fail_cleanly():
+    lib = N.lib()
+    assert lib.uws_step(None, 1) == -1
+    assert lib.uws_destroy(None) == -1
+    assert lib.uws_run_stage(None, 0) == -1
+    assert lib.uws_field_ncomp(99) == -1 and lib.uws_field_ncomp(N.FIELD_D) == 3
