@@ -20,6 +20,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "MLUPS per PISO step + %HBM peak; batched vs plain cases/s at ≤1e-3 rel L2"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per cell per launch read by rocprofv3 FETCH_SIZE (x2, calibrated on a
+# 256 MiB copy) + WRITE_SIZE on the urban case: profiles/urban_256x96x128_hbm_traffic.md
+MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 75.4, "k_pres_sweep": 29.7}}
 
 
 def bytes_per_cell_step(V: int, P: int, C: int) -> int:
@@ -61,6 +64,8 @@ def main() -> int:
     ap.add_argument("--workload", default="urban")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (reference point for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solver step runs on a single MI355X and this build has no "
@@ -103,9 +108,17 @@ def main() -> int:
     else:
         kname, kms, kbytes = "k_pres_sweep", t_pres, 29 * ncell
     achieved = kbytes / (kms * 1e-3) / 1e9
+    tr = MEASURED_TRAFFIC_B_PER_CELL.get(args.workload, {}).get(kname)
+    traffic = int(tr * ncell) if tr is not None else None
     u = s.velocity(3)
     finite = bool(torch.isfinite(torch.from_numpy(u)).all())
     s.close()
+    if args.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
 
     out = {
         "metric": METRIC,
@@ -126,7 +139,8 @@ def main() -> int:
         "hbm_frac": round(hbm_frac, 4),
         "algorithmic_GBps": round(b_step * ncell / (ms_step * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": kbytes,
                      "kernel_ms": {"k_vel_sweep": round(t_vel, 5), "k_pres_sweep": round(t_pres, 5)}},
         "finite": finite,
     }

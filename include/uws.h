@@ -100,8 +100,19 @@ typedef struct uws_config {
                                      3: all three slots, as the shaders are written */
     int32_t wall_model;           /* UWS_WALL_SMOOTH / UWS_WALL_ROUGH */
     float   wall_z0;              /* roughness length for UWS_WALL_ROUGH [m] */
-    int32_t reserved[8];          /* must be zero */
+    int32_t precision;            /* UWS_PREC_*: arithmetic of the run */
+    int32_t refine_rounds;        /* low-precision modes: defect-correction rounds per pressure solve */
+    int32_t refine_sweeps;        /* low-precision modes: Jacobi sweeps per round */
+    int32_t compute_norms;        /* 1: every step also computes the FP64 norms (uws_get_norms) */
+    int32_t reserved[4];          /* must be zero */
 } uws_config;
+
+/* precision modes (extension).  FP32 is the reference's arithmetic; FP64 is
+ * the same step in double on the GPU; FP32_FP16 / FP32_BF16 solve every
+ * pressure equation as FP32 defect correction (refine_rounds rounds of
+ * refine_sweeps Jacobi sweeps) with the correction iterate stored in 2 bytes,
+ * instead of pres_max_num_iter plain FP32 sweeps. */
+enum { UWS_PREC_FP32 = 0, UWS_PREC_FP64 = 1, UWS_PREC_FP32_FP16 = 2, UWS_PREC_FP32_BF16 = 3 };
 
 typedef struct uws_solver* uws_handle;
 
@@ -214,6 +225,25 @@ enum {
     UWS_STAGE_COUNT = 11
 };
 int uws_run_stage(uws_handle h, int32_t stage);
+
+/* Full-precision field download: as uws_get_field, in double (exact for FP64
+ * runs, widened for FP32 runs). */
+int uws_get_field_f64(uws_handle h, int32_t field, double* dst, int64_t n_floats);
+
+/* FP64 norms of the last step (needs compute_norms = 1), host-normalised:
+ *  [0] momentum residual of the last Jacobi sweep, sqrt(sum (D du)^2 / 3N)
+ *      (the reference's definition, FvmSolverGpu.cs:1216-1236)
+ *  [1] pressure residual of the last SIMPLE sweep, sqrt(sum (Dp dp')^2 / N)
+ *      (FvmSolverGpu.cs:1345-1365)
+ *  [2] divergence: rms over cells of the Rhie-Chow net outflow of u^{n+1}
+ *      (bFieldPresCorrect of the final state, FvmColl.compute:2480-2787)
+ *  [3] number of cells whose u or p is not finite
+ *  [4 + 8*s + r] low-precision modes: rms of the defect of pressure solve s
+ *      (0 = SIMPLE, 1.. = PISO correctors) at the start of round r.
+ * Accumulation is in FP64 with a fixed summation order (deterministic). */
+enum { UWS_NORM_VEL_RESIDUAL = 0, UWS_NORM_PRES_RESIDUAL = 1, UWS_NORM_DIVERGENCE = 2, UWS_NORM_NONFINITE = 3,
+       UWS_NORM_REFINE0 = 4, UWS_NORM_COUNT = 64 };
+int uws_get_norms(uws_handle h, double* out, int32_t n);
 
 /* Diagnostics (no reference counterpart): average duration in milliseconds of
  * one launch of a hot-loop kernel, timed with HIP events on the handle's

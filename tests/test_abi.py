@@ -35,8 +35,9 @@ def test_binding_covers_header():
 
 
 def test_config_struct_size():
-    # 3 ints + 4 floats + 3 floats + 12 ints + 3 ints + 1 float + 3 ints + int float int float + 2 int 1 float + 8
-    assert ctypes.sizeof(N.uws_config) == 4 * (3 + 4 + 3 + 12 + 3 + 1 + 2 + 1 + 4 + 3 + 8)
+    # grid 3, dx dt mu den 4, force 3, bnd 12, solver/piso/turb 3, cs 1, iters 2, residual 5,
+    # wall 3, precision/refine/norms 4, reserved 4
+    assert ctypes.sizeof(N.uws_config) == 4 * (3 + 4 + 3 + 12 + 3 + 1 + 2 + 5 + 3 + 4 + 4)
 
 
 def test_strerror_and_version():

@@ -29,6 +29,8 @@ FIELD_EDDY_VIS, FIELD_WALL_VIS, FIELD_B, FIELD_D, FIELD_DP, FIELD_BP, FIELD_AODU
 FIELD_NCOMP = {0: 3, 1: 1, 2: 3, 3: 3, 4: 1, 5: 1, 6: 3, 7: 3, 8: 3, 9: 1, 10: 1, 11: 3}
 
 KERNEL_VEL_SWEEP, KERNEL_PRES_SWEEP = 0, 1
+PREC_FP32, PREC_FP64, PREC_FP32_FP16, PREC_FP32_BF16 = 0, 1, 2, 3
+NORM_VEL_RESIDUAL, NORM_PRES_RESIDUAL, NORM_DIVERGENCE, NORM_NONFINITE, NORM_REFINE0, NORM_COUNT = 0, 1, 2, 3, 4, 64
 
 (STAGE_WALL_FUNC, STAGE_EDDY_VIS, STAGE_LES_TERM, STAGE_VEL_PRECOMP, STAGE_VEL_SWEEPS, STAGE_PRES_PRECOMP,
  STAGE_PRES_SWEEPS, STAGE_APPLY_PRES, STAGE_VEL_CORRECT, STAGE_PISO, STAGE_FINISH) = range(11)
@@ -60,7 +62,11 @@ class uws_config(ctypes.Structure):
         ("wall_slots", c_int32),
         ("wall_model", c_int32),
         ("wall_z0", c_float),
-        ("reserved", c_int32 * 8),
+        ("precision", c_int32),
+        ("refine_rounds", c_int32),
+        ("refine_sweeps", c_int32),
+        ("compute_norms", c_int32),
+        ("reserved", c_int32 * 4),
     ]
 
 
@@ -98,6 +104,8 @@ SIGNATURES = {
     "uws_get_field": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
     "uws_set_field": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
     "uws_run_stage": (c_int, [c_void_p, c_int32]),
+    "uws_get_field_f64": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
+    "uws_get_norms": (c_int, [c_void_p, c_void_p, c_int32]),
     "uws_time_kernel": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_double)]),
 }
 

@@ -115,6 +115,10 @@ class FluidSimConfig:
     wallZ0: float = 0.01
     inflowRefHeight: float = 16.0    # power-law inlet: u = U_ref (y/h_ref)^alpha
     inflowAlpha: float = 0.25
+    precision: int = 0               # 0 FP32, 1 FP64, 2 FP32 + fp16 corrections, 3 FP32 + bf16 corrections
+    refineRounds: int = 3            # low-precision modes: defect-correction rounds per pressure solve
+    refineSweeps: int = 20           # low-precision modes: sweeps per round
+    computeNorms: bool = False       # FP64 residual / divergence norms every step
     # derived by Init()
     _derived: dict = field(default_factory=dict, repr=False)
 
@@ -184,6 +188,10 @@ class FluidSimConfig:
         c.wall_slots = int(self.wallSlots)
         c.wall_model = int(self.wallModel)
         c.wall_z0 = float(self.wallZ0)
+        c.precision = int(self.precision)
+        c.refine_rounds = int(self.refineRounds)
+        c.refine_sweeps = int(self.refineSweeps)
+        c.compute_norms = int(bool(self.computeNorms))
         return c
 
 

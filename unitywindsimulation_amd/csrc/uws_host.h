@@ -1,0 +1,309 @@
+// Host-side solver objects and the step driver shared by the single-case
+// handle (uws_api.cpp) and the ensemble (uws_batch.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "uws_internal.h"
+
+namespace uws {
+
+// what a step does, derived from uws_config
+struct StepCfg {
+    int V = 0, P = 0, C = 0;  // momentum sweeps, pressure sweeps, PISO correctors (0 for SIMPLE)
+    bool les = false, norms = false, lowp = false, bf16 = false, normalise = false, piso = false;
+    int R = 0, K = 0;         // defect-correction rounds / sweeps per round
+    bool cal_residual = false;
+    int vci = 1, pci = 1;
+    float vtol = 0.f, ptol = 0.f;
+};
+
+// Buffer roles: which of the three velocity buffers holds u^n (state), which
+// holds velField within a step (cur), and which p' buffer holds the latest
+// pressure correction (-1: the correction is zero).
+struct RoleState {
+    int state = 0;
+    int cur = -1;
+    int pc_latest = -1;
+    int piso_done = 0;
+};
+
+// host<->device plane selectors: p, u' (comp 0..2), boundary textures, velocity buffer b comp c
+enum { PLANE_P = 0, PLANE_UCORR = 1, PLANE_BND0 = 2, PLANE_VEL0 = 8 };
+
+struct SolverBase {
+    uws_config cfg{};
+    StepCfg sc{};
+    RoleState rs{};
+    hipStream_t stream = nullptr;
+    bool owns_stream = true;
+    int device = 0;
+    void* slab = nullptr;
+    size_t slab_bytes = 0;
+    double* h_norms = nullptr;
+    int64_t step = 0;
+    double phys_time = 0.0;
+
+    virtual ~SolverBase();
+    int sync();
+    virtual int ncells() const = 0;
+    virtual int dims(int i) const = 0;
+    virtual void* device_fields() = 0;   // FieldsT<T>*
+    virtual void* device_params() = 0;   // ParamsT<T>*
+    virtual int set_flags_u8(const uint8_t* fl) = 0;
+    virtual int get_flags_u8(uint8_t* fl) = 0;
+    virtual int set_velocity_all(const float* src, int ncomp) = 0;
+    virtual int get_velocity(float* dst, int ncomp) = 0;
+    virtual int set_plane(int which, int comp, const float* src, size_t n) = 0;
+    virtual int set_plane_d(int which, int comp, const double* src, size_t n) = 0;
+    virtual int get_plane(int which, int comp, float* dst, size_t n) = 0;
+    virtual int get_field(int field, float* dst, int64_t n) = 0;
+    virtual int get_field64(int field, double* dst, int64_t n) = 0;
+    virtual int set_field(int field, const float* src, int64_t n) = 0;
+    virtual int get_norms(double* raw) = 0;
+    virtual int run_stage(int stage) = 0;
+    virtual int step_n(int n) = 0;
+    virtual int time_kernel(int kernel, int reps, double* avg_ms) = 0;
+};
+
+int validate(const uws_config* c, bool allow_residual_break);
+StepCfg step_cfg(const uws_config& c);
+template <class T>
+void fill_params(const uws_config& c, ParamsT<T>& p);
+template <class T>
+size_t carve(const ParamsT<T>& p, FieldsT<T>& f, char* base, bool lowp);
+int bnd_w(int X, int Y, int face);
+int bnd_h(int Y, int Z, int face);
+int create_solver(const uws_config& cfg, bool allow_residual_break, hipStream_t shared_stream,
+                  std::unique_ptr<SolverBase>* out);
+int apply_fixed_value_bnd(SolverBase& s, int32_t inflow_type, float wind_speed, const float vel_zn[3],
+                          float ref_height, float alpha);
+int apply_noise(SolverBase& s, uint64_t seed, float amp);
+int init_vel_pres(SolverBase& s, int32_t inflow_type, float wind_speed);
+int flags_from_int(SolverBase& s, const int32_t* flags, int64_t n);
+int normalise_norms(const double* raw, double ncells, double* out, int n);
+
+// ---------------------------------------------------------------------------
+// Step driver.  L is a launcher: l(kernel, roles, slot) enqueues one kernel
+// (for one case, or for every case of an ensemble), l.copy_vel(dst, src)
+// copies a velocity buffer, l.read_norm(slot, &v) synchronises and reads a
+// norm (used only by the residual-based early exit), l.ncells() is N.
+// ---------------------------------------------------------------------------
+
+// VelPredict() (FvmSolverGpu.cs:1238-1282): V Jacobi sweeps from u^n
+template <class Lr>
+int stage_vel_sweeps(Lr& l, const StepCfg& c, RoleState& rs) {
+    const int s = rs.state, w0 = (s + 1) % 3, w1 = (s + 2) % 3;
+    int in = s, last_in = s, result = -1;
+    bool broke = false;
+    for (int k = 0; k < c.V; ++k) {
+        const int out = (k & 1) ? w1 : w0;
+        Roles r{};
+        r.state = s; r.vin = in; r.vout = out;
+        l(KN_VEL_SWEEP, r);
+        if (c.cal_residual && k % c.vci == 0) {
+            l(KN_RES_VEL, r);
+            l(KN_SUM_PARTIALS, r, UWS_NORM_VEL_RESIDUAL);
+            double sum = 0.0;
+            if (int rc = l.read_norm(UWS_NORM_VEL_RESIDUAL, &sum)) return rc;
+            const float res = (float)std::sqrt(sum / (3.0 * l.ncells()));
+            if (res < c.vtol) {
+                // the reference keeps the previous iterate after a break (the swap is skipped, :1256-1279)
+                result = in;
+                broke = true;
+                break;
+            }
+        }
+        last_in = in;
+        in = out;
+    }
+    if (result < 0) result = in;
+    if (c.norms && c.V >= 1 && !broke) {
+        Roles r{};
+        r.vin = last_in; r.vout = result;
+        l(KN_RES_VEL, r);
+        l(KN_SUM_PARTIALS, r, UWS_NORM_VEL_RESIDUAL);
+    }
+    if (result == s) {
+        // no sweep result survived: velField is the copy of u^n held in velFieldLastIter
+        if (int rc = l.copy_vel(w0, s)) return rc;
+        result = w0;
+    }
+    rs.cur = result;
+    return UWS_OK;
+}
+
+// One pressure-equation solve: the SIMPLE correction (coef 0,
+// FvmSolverGpu.cs:1367-1415) or a PISO corrector (coef 1, :1493-1514).
+// Returns the buffer of the latest iterate in *latest (-1: zero).
+template <class Lr>
+int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest) {
+    if (c.lowp) {
+        // FP32 defect correction with 2-byte correction storage
+        int x = -1;
+        for (int rd = 0; rd < c.R; ++rd) {
+            Roles r{};
+            r.coef = coef; r.pin = x;
+            l(KN_DEFECT_RESIDUAL, r);
+            l(KN_DEFECT_SCALE, r, UWS_NORM_REFINE0 + 8 * solve + rd);
+            int ein = -1;
+            for (int k = 0; k < c.K; ++k) {
+                Roles s{};
+                s.coef = coef; s.pin = x; s.pout = 0; s.ein = ein; s.eout = k & 1;
+                s.flags = (k == 0 ? K_LOWP_FIRST : 0) | (k == c.K - 1 ? K_LOWP_LAST : 0) | (c.bf16 ? K_BF16 : 0);
+                l(KN_LOWP_SWEEP, s);
+                ein = k & 1;
+            }
+            if (c.K > 0) x = 0;
+        }
+        *latest = x;
+        return UWS_OK;
+    }
+    int in = -1, last_in = -1;
+    bool broke = false;
+    for (int k = 0; k < c.P; ++k) {
+        const int out = k & 1;
+        Roles r{};
+        r.coef = coef; r.pin = in; r.pout = out;
+        l(KN_PRES_SWEEP, r);
+        if (coef == 0 && c.cal_residual && k % c.pci == 0) {
+            l(KN_RES_PRES, r);
+            l(KN_SUM_PARTIALS, r, UWS_NORM_PRES_RESIDUAL);
+            double sum = 0.0;
+            if (int rc = l.read_norm(UWS_NORM_PRES_RESIDUAL, &sum)) return rc;
+            if ((float)std::sqrt(sum / l.ncells()) < c.ptol) { broke = true; break; }
+        }
+        last_in = in;
+        in = out;
+    }
+    if (coef == 0 && c.norms && c.P >= 1 && !broke) {
+        Roles r{};
+        r.pin = last_in; r.pout = in;
+        l(KN_RES_PRES, r);
+        l(KN_SUM_PARTIALS, r, UWS_NORM_PRES_RESIDUAL);
+    }
+    *latest = in;
+    return UWS_OK;
+}
+
+template <class Lr>
+int ensure_pc(Lr& l, RoleState& rs) {
+    if (rs.pc_latest < 0) {
+        Roles r{};
+        r.pout = 0;
+        l(KN_FILL_PC, r);
+        rs.pc_latest = 0;
+    }
+    return UWS_OK;
+}
+
+template <class Lr>
+int run_stage(Lr& l, const StepCfg& c, RoleState& rs, int stage) {
+    Roles r{};
+    r.state = rs.state;
+    switch (stage) {
+    case UWS_STAGE_WALL_FUNC:
+        if (c.les) { r.flags = K_WALL; l(KN_WALL_EDDY, r); }
+        return UWS_OK;
+    case UWS_STAGE_EDDY_VIS:
+        if (c.les) { r.flags = K_EDDY; l(KN_WALL_EDDY, r); }
+        return UWS_OK;
+    case UWS_STAGE_LES_TERM:
+        if (c.les) l(KN_LES_TERM, r);
+        return UWS_OK;
+    case UWS_STAGE_VEL_PRECOMP:
+        l(KN_VEL_PRECOMP, r);
+        return UWS_OK;
+    case UWS_STAGE_VEL_SWEEPS: return stage_vel_sweeps(l, c, rs);
+    case UWS_STAGE_PRES_PRECOMP:
+        if (rs.cur < 0) return UWS_ERR_INVALID_ARG;
+        r.vout = rs.cur;
+        l(KN_PRES_PRECOMP, r);
+        return UWS_OK;
+    case UWS_STAGE_PRES_SWEEPS: return pres_solve(l, c, 0, 0, &rs.pc_latest);
+    case UWS_STAGE_APPLY_PRES:
+        ensure_pc(l, rs);
+        r.pin = rs.pc_latest;
+        l(KN_APPLY_PRES, r);
+        if (c.normalise) l(KN_PRES_NORMALISE, r);
+        return UWS_OK;
+    case UWS_STAGE_VEL_CORRECT:
+        if (rs.cur < 0) return UWS_ERR_INVALID_ARG;
+        ensure_pc(l, rs);
+        r.pin = rs.pc_latest;
+        r.vout = rs.cur;
+        l(KN_VEL_CORRECT, r);
+        rs.pc_latest = -1;  // SetPresCorrectFieldtoZero (FvmSolverGpu.cs:1088)
+        return UWS_OK;
+    case UWS_STAGE_PISO: {
+        if (!c.piso) return UWS_OK;
+        if (rs.cur < 0) return UWS_ERR_INVALID_ARG;
+        l(KN_PISO_AODU, r);
+        l(KN_PISO_PRECOMP, r);
+        const int solve = 1 + (rs.piso_done < 6 ? rs.piso_done : 6);
+        if (int rc = pres_solve(l, c, 1, solve, &rs.pc_latest)) return rc;
+        ensure_pc(l, rs);
+        Roles q{};
+        q.state = rs.state; q.pin = rs.pc_latest; q.vout = rs.cur;
+        l(KN_PISO_CORRECT, q);
+        rs.pc_latest = -1;
+        rs.piso_done += 1;
+        return UWS_OK;
+    }
+    case UWS_STAGE_FINISH:
+        if (rs.cur < 0) return UWS_ERR_INVALID_ARG;
+        if (c.norms) {
+            Roles q{};
+            q.vout = rs.cur; q.state = rs.cur;
+            l(KN_PRES_PRECOMP, q);
+            l(KN_DIVERGENCE, q);
+            l(KN_SUM_PARTIALS, q, UWS_NORM_DIVERGENCE);
+            l(KN_NONFINITE, q);
+            l(KN_SUM_PARTIALS, q, UWS_NORM_NONFINITE);
+        }
+        rs.state = rs.cur;
+        rs.cur = -1;
+        rs.piso_done = 0;
+        return UWS_OK;
+    default: return UWS_ERR_INVALID_ARG;
+    }
+}
+
+// FvmSolverGpu.Step() (FvmSolverGpu.cs:1050-1121) with the wall function,
+// eddy viscosity, LES term and momentum precompute fused into two kernels
+template <class Lr>
+int enqueue_step(Lr& l, const StepCfg& c, RoleState& rs) {
+    Roles r{};
+    r.state = rs.state;
+    if (c.les) {
+        r.flags = K_WALL | K_EDDY;
+        l(KN_WALL_EDDY, r);
+        r.flags = K_FUSE_LES;
+    }
+    l(KN_VEL_PRECOMP, r);
+    for (int s = UWS_STAGE_VEL_SWEEPS; s <= UWS_STAGE_VEL_CORRECT; ++s)
+        if (int rc = run_stage(l, c, rs, s)) return rc;
+    for (int n = 0; n < c.C; ++n)
+        if (int rc = run_stage(l, c, rs, UWS_STAGE_PISO)) return rc;
+    return run_stage(l, c, rs, UWS_STAGE_FINISH);
+}
+
+// the role change of one enqueue_step without a residual-based early exit
+inline void advance_roles(const StepCfg& c, RoleState& rs) {
+    const int w0 = (rs.state + 1) % 3, w1 = (rs.state + 2) % 3;
+    rs.state = c.V == 0 ? w0 : (((c.V - 1) & 1) ? w1 : w0);
+    rs.cur = -1;
+    rs.pc_latest = -1;
+    rs.piso_done = 0;
+}
+
+}  // namespace uws
+
+struct uws_solver {
+    std::unique_ptr<uws::SolverBase> impl;
+};

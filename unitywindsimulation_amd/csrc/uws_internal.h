@@ -9,67 +9,108 @@
 namespace uws {
 
 // Uniform parameters of one solver: the reference's per-shader uniforms
-// (FvmSolverGpu.cs:370-396, 499-507, 529-547, 561-580) gathered in one block.
-struct Params {
+// (FvmSolverGpu.cs:370-396, 499-507, 529-547, 561-580) gathered in one block,
+// in the arithmetic type T of the run.
+template <class T>
+struct ParamsT {
     int X, Y, Z;          // gridRes
-    int N;                // X*Y*Z (< 2^31, checked at creation)
-    float dx, dt, ds, dv, nu, den, cs;
-    float fext[3];
+    int N;                // X*Y*Z (< 2^29, checked at creation)
+    int tiles;            // workgroup tiles of this grid
+    T dx, dt, ds, dv, nu, den, cs;
+    T fext[3];
     int velBnd[6];        // X0 Xn Y0 Yn Z0 Zn
     int presBnd[6];
     int useLES;
     int wallSlots;        // 1 or 3
     int wallModel;        // UWS_WALL_SMOOTH / UWS_WALL_ROUGH
-    float wallZ0;
+    T wallZ0;
+    int normRef;          // flat index of the pressure normalisation cell, -1 when it is inactive
 };
 
 // Device fields, structure-of-arrays: every vector field is three planes of
-// N floats; flags are one byte per cell.  Face coefficients are stored once
+// N values; flags are one byte per cell.  Face coefficients are stored once
 // per face on the cell that owns the face's +x/+y/+z side (E, N, U faces);
 // the W/S/D coefficients of a cell are the E/N/U coefficients of its
 // W/S/D neighbour.  A face between a fluid cell and anything else has a
 // zero coefficient, so the Jacobi sweeps need no neighbour flags.
-struct Fields {
+template <class T>
+struct FieldsT {
     uint8_t* flag;
-    float* vel[3][3];     // three velocity buffers x 3 components (rotated, see Solver)
-    float* ucorr[3];      // u'  (velCorrectField)
-    float* p;             // p^n (presFieldLastTime)
-    float* pc[2];         // p' ping-pong (presCorrectField / LastIter)
-    float* nut;           // eddyVisField
-    float* nuw[3];        // faceEddyVisField slots W(.x) S(.y) D(.z)
-    float* D[3];          // DField
-    float* invD[3];       // 1/DField (fluid cells; 0 elsewhere)
-    float* b[3];          // bField
-    float* S[3];          // momentum: u^n_P,n + u^n_nb,n on the owned E/N/U face
-    float* G[3];          // momentum: nu_f*ds/dx on the owned E/N/U face
-    float* cp[3];         // pressure: (dv/den)*DInv_f/dx on the owned E/N/U face
-    float* cpisoE;        // PISO E/W face: (dv/den)*(DInv_f/dx)
-    float* Dp;            // DFieldPresCorrect
-    float* invDp;         // 1/Dp (fluid) or 0
-    float* bp;            // bFieldPresCorrect
-    float* aodu[3];       // AodUCorrectField
-    float* invDpiso;      // 1/presCoeff of the PISO corrector equation
-    float* bpiso;         // velTerm of the PISO corrector equation
-    float* bnd[6][3];     // boundary textures, [face][comp][j*W + i]
-    double* partial;      // reduction partials
-    float* scalar;        // small scalars (normalisation reference value)
+    T* vel[3][3];         // three velocity buffers x 3 components (roles rotate, see the host)
+    T* ucorr[3];          // u'  (velCorrectField)
+    T* p;                 // p^n (presFieldLastTime)
+    T* pc[2];             // p' ping-pong (presCorrectField / LastIter)
+    T* nut;               // eddyVisField
+    T* nuw[3];            // faceEddyVisField slots W(.x) S(.y) D(.z)
+    T* D[3];              // DField
+    T* invD[3];           // 1/DField (fluid cells; 0 elsewhere)
+    T* b[3];              // bField
+    T* S[3];              // momentum: u^n_P,n + u^n_nb,n on the owned E/N/U face
+    T* G[3];              // momentum: nu_f*ds/dx on the owned E/N/U face
+    T* cp[3];             // pressure: (dv/den)*DInv_f/dx on the owned E/N/U face
+    T* cpisoE;            // PISO E/W face: (dv/den)*(DInv_f/dx)
+    T* Dp;                // DFieldPresCorrect
+    T* invDp;             // 1/Dp (fluid) or 0
+    T* bp;                // bFieldPresCorrect
+    T* aodu[3];           // AodUCorrectField
+    T* invDpiso;          // 1/presCoeff of the PISO corrector equation
+    T* bpiso;             // velTerm of the PISO corrector equation
+    T* bnd[6][3];         // boundary textures, [face][comp][j*W + i]
+    // defect correction with low-precision correction storage
+    T* rho;               // defect right-hand side
+    void* e16[2];         // correction iterates, 2-byte storage
+    double* partial;      // reduction partials (one per workgroup)
+    double* norms;        // FP64 norms written by the step (see UWS_NORM_*)
+    T* scalar;            // small scalars: [0] normalisation value, [1] defect scale
 };
 
-// kernel launchers (uws_kernels.hip); all enqueue on `s`
-void launch_wall_eddy(const Params& p, const Fields& f, const float* const un[3], bool wall, bool eddy, hipStream_t s);
-void launch_les_term(const Params& p, const Fields& f, const float* const un[3], hipStream_t s);
-void launch_vel_precompute(const Params& p, const Fields& f, const float* const un[3], bool fuse_les, hipStream_t s);
-void launch_vel_sweep(const Params& p, const Fields& f, const float* const uk[3], float* const unew[3], hipStream_t s);
-void launch_pres_precompute(const Params& p, const Fields& f, const float* const ustar[3], hipStream_t s);
-void launch_pres_sweep(const Params& p, const Fields& f, const float* cE, const float* cN, const float* cU,
-                       const float* invDp, const float* bp, const float* pk, float* pnew, hipStream_t s);
-void launch_apply_pres(const Params& p, const Fields& f, const float* pc, bool normalise, hipStream_t s);
-void launch_vel_correct(const Params& p, const Fields& f, const float* pc, float* const u[3], hipStream_t s);
-void launch_piso_aodu(const Params& p, const Fields& f, const float* const un[3], hipStream_t s);
-void launch_piso_precompute(const Params& p, const Fields& f, hipStream_t s);
-void launch_piso_correct(const Params& p, const Fields& f, const float* pc, float* const u[3], hipStream_t s);
-// FP64 sum of squares of d[i]*(a[i]-b[i]) over ncomp planes; b may be null (zero)
-void launch_residual(const Params& p, const Fields& f, int ncomp, const float* const d[3], const float* const a[3],
-                     const float* const b[3], double* out, hipStream_t s);
+// Multi-case tile table for the ensemble launch: workgroup tile t belongs to
+// the case c with first_tile[c] <= t < first_tile[c+1].
+template <class T>
+struct BatchTable {
+    int ncases;
+    int total_tiles;
+    const int* first_tile;      // [ncases + 1]
+    const ParamsT<T>* params;   // [ncases]
+    const FieldsT<T>* fields;   // [ncases]
+};
+
+// Buffer roles of one launch: the same small integers for every case of an
+// ensemble, which share V, P and C and so rotate their buffers identically.
+struct Roles {
+    int state;      // velocity buffer holding u^n
+    int vin;        // sweep input velocity buffer
+    int vout;       // sweep output / corrected velocity buffer
+    int pin;        // p' input buffer (-1: zero)
+    int pout;       // p' output buffer
+    int coef;       // pressure coefficients: 0 SIMPLE, 1 PISO
+    int ein;        // low-precision correction input buffer (-1: zero)
+    int eout;       // low-precision correction output buffer
+    int flags;      // kernel options (K_* bits)
+};
+enum {
+    K_WALL = 1, K_EDDY = 2, K_FUSE_LES = 4, K_LOWP_FIRST = 16, K_LOWP_LAST = 32,
+    K_BF16 = 64
+};
+
+// kernel identifiers understood by launch()
+enum Kernel {
+    KN_WALL_EDDY, KN_LES_TERM, KN_VEL_PRECOMP, KN_VEL_SWEEP, KN_PRES_PRECOMP, KN_PRES_SWEEP,
+    KN_APPLY_PRES, KN_PRES_NORMALISE, KN_VEL_CORRECT, KN_PISO_AODU, KN_PISO_PRECOMP, KN_PISO_CORRECT,
+    KN_ZERO_SOLID_VEL, KN_FILL_PC,
+    KN_DEFECT_RESIDUAL, KN_LOWP_SWEEP,
+    KN_RES_VEL, KN_RES_PRES, KN_DIVERGENCE, KN_NONFINITE, KN_SUM_PARTIALS, KN_DEFECT_SCALE
+};
+
+// Single-case launch of kernel k on stream s.  `slot` selects the norms
+// entry for reduction kernels.
+template <class T>
+void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s, int slot = 0);
+
+// Ensemble launch: one grid over every case's tiles.
+template <class T>
+void launch_batch(Kernel k, const BatchTable<T>& tab, int total_tiles, const Roles& r, hipStream_t s, int slot = 0);
+
+inline int tiles_of(int X, int Y, int Z) { return ((X + 63) / 64) * ((Y + 3) / 4) * Z; }
 
 }  // namespace uws

@@ -202,3 +202,15 @@ class FvmSolverHip(IFvmSolver):
         ms = c_double()
         N.check(self._lib.uws_time_kernel(self._h, int(kernel), int(reps), byref(ms)), "uws_time_kernel")
         return float(ms.value)
+
+    def norms(self) -> np.ndarray:
+        """FP64 norms of the last step (needs ``computeNorms``): see ``uws_get_norms``."""
+        out = np.zeros(N.NORM_COUNT, dtype=np.float64)
+        N.check(self._lib.uws_get_norms(self._h, N.ptr(out), N.NORM_COUNT), "uws_get_norms")
+        return out
+
+    def field64(self, fid: int) -> np.ndarray:
+        nc = N.FIELD_NCOMP[fid]
+        out = np.empty(self.n * nc, dtype=np.float64)
+        N.check(self._lib.uws_get_field_f64(self._h, fid, N.ptr(out), out.size), f"uws_get_field_f64({fid})")
+        return out.reshape(self.n, nc) if nc == 3 else out
