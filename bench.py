@@ -56,6 +56,67 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
             "sample": f"{n} PISO steps of {w.name} (V=P=50, C=2), single-thread FP32 oracle"}
 
 
+def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
+    """The 24-case uneven ensemble: cases/s of one ensemble graph per step
+    against the same cases stepped one after another (each with its own
+    graph), and the FP32 ensemble's velocity error against FP64 runs."""
+    import numpy as np
+
+    from unitywindsimulation_amd import FvmSolverHip, workloads
+    from unitywindsimulation_amd import _native as N
+    from unitywindsimulation_amd.solver import FvmEnsembleHip
+
+    ws = workloads.ensemble()
+    full_steps = ws[0].steps
+    t_plain = 0.0
+    for w in ws:
+        with FvmSolverHip(w.cf) as s:
+            s.InitFlags(w.flags)
+            s.init_noise(w.noise_seed, w.noise_amp)
+            s.step(2)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.synchronize()
+            t_plain += (time.perf_counter() - t0) / steps
+    with FvmEnsembleHip([w.cf for w in ws]) as e:
+        for c, w in enumerate(ws):
+            e.init_flags(c, w.flags)
+            e.init_noise(c, w.noise_seed, w.noise_amp)
+        ty, pers, _ = e.autotune(2)
+        e.step(1)
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.step(steps)
+        e.synchronize()
+        t_batch = (time.perf_counter() - t0) / steps
+    # accuracy: FP32 ensemble vs FP64 single runs after check_steps steps
+    errs = []
+    with FvmEnsembleHip([w.cf for w in ws]) as e:
+        for c, w in enumerate(ws):
+            e.init_flags(c, w.flags)
+            e.init_noise(c, w.noise_seed, w.noise_amp)
+        e.step(check_steps)
+        for c, w in enumerate(ws):
+            w.cf.precision = N.PREC_FP64
+            with FvmSolverHip(w.cf) as d:
+                d.InitFlags(w.flags)
+                d.init_noise(w.noise_seed, w.noise_amp)
+                d.step(check_steps)
+                ref = d.field64(N.FIELD_VEL)
+            w.cf.precision = N.PREC_FP32
+            got = e.velocity(c, 3).reshape(-1, 3).astype(np.float64)
+            errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    cells = sum(w.ncells for w in ws)
+    return {"cases": len(ws), "cells": cells, "steps_per_case": full_steps,
+            "plain_cases_per_s": round(len(ws) / (t_plain * full_steps), 4),
+            "batched_cases_per_s": round(len(ws) / (t_batch * full_steps), 4),
+            "speedup": round(t_plain / t_batch, 3),
+            "plain_ms_per_step_all_cases": round(t_plain * 1e3, 3), "batched_ms_per_step": round(t_batch * 1e3, 3),
+            "launch": {"tile_rows": ty, "persistent": pers},
+            "max_rel_l2_u_vs_fp64": max(errs), "compared_after_steps": check_steps}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +125,7 @@ def main() -> int:
     ap.add_argument("--workload", default="urban")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 24-case ensemble leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM counters)")
     args = ap.parse_args()
@@ -144,6 +206,8 @@ def main() -> int:
                      "kernel_ms": {"k_vel_sweep": round(t_vel, 5), "k_pres_sweep": round(t_pres, 5)}},
         "finite": finite,
     }
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_report()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(out))

@@ -254,6 +254,37 @@ int uws_get_norms(uws_handle h, double* out, int32_t n);
 enum { UWS_KERNEL_VEL_SWEEP = 0, UWS_KERNEL_PRES_SWEEP = 1 };
 int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms);
 
+/* ---- ensemble of uneven cases (extension, no reference counterpart) ----
+ * Several independent solvers (different grids, buildings, inflow) stepped
+ * together: every kernel of the step runs once for all cases, over one tile
+ * table spanning every case's grid, and each step is one captured HIP graph.
+ * All cases must share vel_max_num_iter, pres_max_num_iter, the solver type,
+ * the PISO corrector count, the turbulence model, precision FP32 and
+ * compute_norms, and use no residual-based early exit.  Case c of the
+ * ensemble behaves exactly like a uws_handle created from cfgs[c]. */
+typedef struct uws_batch* uws_batch_handle;
+int uws_batch_create(const uws_config* cfgs, int32_t ncases, uws_batch_handle* out);
+int uws_batch_destroy(uws_batch_handle b);
+int uws_batch_num_cases(uws_batch_handle b, int32_t* n);
+int uws_batch_init_flags(uws_batch_handle b, int32_t c, const int32_t* flags, int64_t n);
+int uws_batch_init_noise(uws_batch_handle b, int32_t c, uint64_t seed, float amp);
+int uws_batch_set_fixed_value_vel_bnd_cond(uws_batch_handle b, int32_t c, int32_t inflow_type, float wind_speed,
+                                           const float vel_zn[3], float ref_height, float alpha);
+int uws_batch_set_velocity(uws_batch_handle b, int32_t c, const float* src, int32_t ncomp, int64_t n);
+int uws_batch_get_velocity(uws_batch_handle b, int32_t c, float* dst, int32_t ncomp, int64_t n);
+int uws_batch_get_pressure(uws_batch_handle b, int32_t c, float* dst, int64_t n);
+int uws_batch_get_norms(uws_batch_handle b, int32_t c, double* out, int32_t n);
+int uws_batch_step(uws_batch_handle b, int32_t nsteps);
+int uws_batch_synchronize(uws_batch_handle b);
+/* Launch shape of the ensemble kernels: tile rows ty in {1, 2, 4} (tiles of
+ * 64 x ty cells) and persistent != 0 for a grid-stride launch of a fixed
+ * number of workgroups instead of one workgroup per tile. */
+int uws_batch_set_launch(uws_batch_handle b, int32_t ty, int32_t persistent);
+/* Autotuner: times `steps` ensemble steps for every launch shape and keeps the
+ * fastest; the solver state is saved before and restored after the trials.
+ * best[0] = ty, best[1] = persistent, best_ms = ms per step of the winner. */
+int uws_batch_autotune(uws_batch_handle b, int32_t steps, int32_t best[2], double* best_ms);
+
 #ifdef __cplusplus
 }
 #endif

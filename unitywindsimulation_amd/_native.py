@@ -107,6 +107,21 @@ SIGNATURES = {
     "uws_get_field_f64": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
     "uws_get_norms": (c_int, [c_void_p, c_void_p, c_int32]),
     "uws_time_kernel": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_double)]),
+    "uws_batch_create": (c_int, [c_void_p, c_int32, POINTER(c_void_p)]),
+    "uws_batch_destroy": (c_int, [c_void_p]),
+    "uws_batch_num_cases": (c_int, [c_void_p, POINTER(c_int32)]),
+    "uws_batch_init_flags": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
+    "uws_batch_init_noise": (c_int, [c_void_p, c_int32, c_uint64, c_float]),
+    "uws_batch_set_fixed_value_vel_bnd_cond": (c_int, [c_void_p, c_int32, c_int32, c_float, c_void_p, c_float,
+                                                       c_float]),
+    "uws_batch_set_velocity": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_int64]),
+    "uws_batch_get_velocity": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_int64]),
+    "uws_batch_get_pressure": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
+    "uws_batch_get_norms": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
+    "uws_batch_step": (c_int, [c_void_p, c_int32]),
+    "uws_batch_synchronize": (c_int, [c_void_p]),
+    "uws_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32]),
+    "uws_batch_autotune": (c_int, [c_void_p, c_int32, c_void_p, POINTER(c_double)]),
 }
 
 _lib = None

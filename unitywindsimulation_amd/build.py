@@ -17,7 +17,7 @@ REPO_DIR = PKG_DIR.parent
 CSRC = PKG_DIR / "csrc"
 LIB_PATH = PKG_DIR / "libuws_hip.so"
 
-SOURCES = ["uws_kernels.hip", "uws_api.cpp"]
+SOURCES = ["uws_kernels.hip", "uws_api.cpp", "uws_batch.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -77,7 +77,8 @@ template <class T>
 void fill_params(const uws_config& c, ParamsT<T>& p) {
     p.X = c.grid_res[0]; p.Y = c.grid_res[1]; p.Z = c.grid_res[2];
     p.N = p.X * p.Y * p.Z;
-    p.tiles = tiles_of(p.X, p.Y, p.Z);
+    p.ty = 4;
+    p.tiles = tiles_of(p.X, p.Y, p.Z, p.ty);
     // FluidSimConfig.Init (FluidSimConfig.cs:325-341), in the run's arithmetic
     p.dx = (T)c.dx;
     p.ds = p.dx * p.dx;
@@ -146,7 +147,7 @@ size_t carve(const ParamsT<T>& p, FieldsT<T>& f, char* base, bool lowp) {
         f.rho = nullptr;
         f.e16[0] = f.e16[1] = nullptr;
     }
-    f.partial = (double*)take((size_t)p.tiles * sizeof(double));
+    f.partial = (double*)take((size_t)tiles_of(p.X, p.Y, p.Z, 1) * sizeof(double));  // room for any ty
     f.norms = (double*)take(UWS_NORM_COUNT * sizeof(double));
     f.scalar = fl(64);
     return off + 256;

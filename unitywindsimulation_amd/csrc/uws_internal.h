@@ -15,6 +15,7 @@ template <class T>
 struct ParamsT {
     int X, Y, Z;          // gridRes
     int N;                // X*Y*Z (< 2^29, checked at creation)
+    int ty;               // workgroup rows: tiles are 64 x ty cells of one z-plane
     int tiles;            // workgroup tiles of this grid
     T dx, dt, ds, dv, nu, den, cs;
     T fext[3];
@@ -70,6 +71,8 @@ template <class T>
 struct BatchTable {
     int ncases;
     int total_tiles;
+    int ty;                     // workgroup rows (every case uses the same)
+    int persistent;             // 1: grid-stride launch of a fixed number of workgroups
     const int* first_tile;      // [ncases + 1]
     const ParamsT<T>* params;   // [ncases]
     const FieldsT<T>* fields;   // [ncases]
@@ -109,8 +112,8 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
 
 // Ensemble launch: one grid over every case's tiles.
 template <class T>
-void launch_batch(Kernel k, const BatchTable<T>& tab, int total_tiles, const Roles& r, hipStream_t s, int slot = 0);
+void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot = 0);
 
-inline int tiles_of(int X, int Y, int Z) { return ((X + 63) / 64) * ((Y + 3) / 4) * Z; }
+inline int tiles_of(int X, int Y, int Z, int ty) { return ((X + 63) / 64) * ((Y + ty - 1) / ty) * Z; }
 
 }  // namespace uws

@@ -24,7 +24,9 @@ namespace uws {
 
 namespace {
 
-constexpr int TX = 64, TY = 4;
+// workgroups are 64 x ty threads: one wave per x-row of 64 cells, ty rows
+// (ty in {1, 2, 4}, chosen per solver or per ensemble)
+constexpr int TX = 64;
 
 enum { E_ = 0, W_ = 1, N_ = 2, S_ = 3, U_ = 4, D_ = 5 };
 __constant__ int c_off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
@@ -49,12 +51,13 @@ struct Cell {
 
 template <class T>
 __device__ __forceinline__ Cell cell_of(const ParamsT<T>& p, int t) {
-    const int gx = (p.X + TX - 1) / TX, gy = (p.Y + TY - 1) / TY;
+    const int ty = p.ty;
+    const int gx = (p.X + TX - 1) / TX, gy = (p.Y + ty - 1) / ty;
     const int bx = t % gx, r = t / gx, by = r % gy, bz = r / gy;
     Cell c;
     c.tile = t;
     c.x = bx * TX + (int)threadIdx.x;
-    c.y = by * TY + (int)threadIdx.y;
+    c.y = by * ty + (int)threadIdx.y;
     c.z = bz;
     c.ok = c.x < p.X && c.y < p.Y;
     c.P = (c.z * p.Y + c.y) * p.X + c.x;
@@ -951,14 +954,16 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-// 256-thread block sum, result valid in every thread
+// block sum over blockDim.y waves in a fixed order, result valid in every thread
 __device__ __forceinline__ double block_sum(double v) {
     __shared__ double ws[4];
     v = wave_sum(v);
-    const int tid = threadIdx.y * TX + threadIdx.x;
-    if ((tid & 63) == 0) ws[tid >> 6] = v;
+    if (threadIdx.x == 0) ws[threadIdx.y] = v;
     __syncthreads();
-    return ws[0] + ws[1] + ws[2] + ws[3];
+    double s = 0.0;
+    for (int i = 0; i < (int)blockDim.y; ++i) s += ws[i];
+    __syncthreads();
+    return s;
 }
 
 template <class T>
@@ -968,19 +973,37 @@ __device__ __forceinline__ int case_of(const BatchTable<T>& tab, int t) {
     return c;
 }
 
+// Persistent ensemble launch: the grid is a multiple of 8 workgroups, XCD j
+// (workgroups j, j+8, ...) walks its own contiguous range of tiles.
+__device__ __forceinline__ void xcd_range(int total, int xcd, int* lo, int* hi) {
+    const int q = total >> 3, r = total & 7;
+    *lo = xcd * q + min(xcd, r);
+    *hi = *lo + q + (xcd < r ? 1 : 0);
+}
+
 template <class T, int K>
 __global__ __launch_bounds__(256) void k_single(ParamsT<T> p, FieldsT<T> f, Roles r) {
     const Cell c = cell_of(p, xcd_tile((int)blockIdx.x, p.tiles));
     run_body<T, K>(p, f, c, r);
 }
 template <class T, int K>
-__global__ __launch_bounds__(256) void k_batch(BatchTable<T> tab, Roles r) {
-    const int t = xcd_tile((int)blockIdx.x, tab.total_tiles);
+__device__ __forceinline__ void batch_tile(const BatchTable<T>& tab, const Roles& r, int t) {
     const int cid = case_of(tab, t);
     const ParamsT<T>& p = tab.params[cid];
     const FieldsT<T>& f = tab.fields[cid];
     const Cell c = cell_of(p, t - tab.first_tile[cid]);
     run_body<T, K>(p, f, c, r);
+}
+template <class T, int K>
+__global__ __launch_bounds__(256) void k_batch(BatchTable<T> tab, Roles r) {
+    if (!tab.persistent) {
+        batch_tile<T, K>(tab, r, xcd_tile((int)blockIdx.x, tab.total_tiles));
+        return;
+    }
+    int lo, hi;
+    xcd_range(tab.total_tiles, (int)blockIdx.x & 7, &lo, &hi);
+    const int per = (int)gridDim.x >> 3;
+    for (int t = lo + ((int)blockIdx.x >> 3); t < hi; t += per) batch_tile<T, K>(tab, r, t);
 }
 template <class T, int K>
 __global__ __launch_bounds__(256) void k_single_reduce(ParamsT<T> p, FieldsT<T> f, Roles r) {
@@ -989,14 +1012,24 @@ __global__ __launch_bounds__(256) void k_single_reduce(ParamsT<T> p, FieldsT<T> 
     if (threadIdx.x == 0 && threadIdx.y == 0) f.partial[c.tile] = s;
 }
 template <class T, int K>
-__global__ __launch_bounds__(256) void k_batch_reduce(BatchTable<T> tab, Roles r) {
-    const int t = xcd_tile((int)blockIdx.x, tab.total_tiles);
+__device__ __forceinline__ void batch_reduce_tile(const BatchTable<T>& tab, const Roles& r, int t) {
     const int cid = case_of(tab, t);
     const ParamsT<T>& p = tab.params[cid];
     const FieldsT<T>& f = tab.fields[cid];
     const Cell c = cell_of(p, t - tab.first_tile[cid]);
     const double s = block_sum(run_reduce<T, K>(p, f, c, r));
     if (threadIdx.x == 0 && threadIdx.y == 0) f.partial[c.tile] = s;
+}
+template <class T, int K>
+__global__ __launch_bounds__(256) void k_batch_reduce(BatchTable<T> tab, Roles r) {
+    if (!tab.persistent) {
+        batch_reduce_tile<T, K>(tab, r, xcd_tile((int)blockIdx.x, tab.total_tiles));
+        return;
+    }
+    int lo, hi;
+    xcd_range(tab.total_tiles, (int)blockIdx.x & 7, &lo, &hi);
+    const int per = (int)gridDim.x >> 3;
+    for (int t = lo + ((int)blockIdx.x >> 3); t < hi; t += per) batch_reduce_tile<T, K>(tab, r, t);
 }
 
 // sum one case's partials into norms[slot]; with `scale` also store the
@@ -1005,7 +1038,7 @@ template <class T>
 __device__ void sum_partials(const ParamsT<T>& p, const FieldsT<T>& f, int slot, bool scale) {
     double s = 0.0;
     const int tid = threadIdx.y * TX + threadIdx.x;
-    for (int i = tid; i < p.tiles; i += 256) s += f.partial[i];
+    for (int i = tid; i < p.tiles; i += TX * (int)blockDim.y) s += f.partial[i];
     s = block_sum(s);
     if (tid == 0) {
         f.norms[slot] = s;
@@ -1027,19 +1060,19 @@ __global__ __launch_bounds__(256) void k_batch_sum(BatchTable<T> tab, int slot, 
 
 template <class T, int K>
 void go_single(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s) {
-    hipLaunchKernelGGL((k_single<T, K>), dim3(p.tiles), dim3(TX, TY, 1), 0, s, p, f, r);
+    hipLaunchKernelGGL((k_single<T, K>), dim3(p.tiles), dim3(TX, p.ty, 1), 0, s, p, f, r);
 }
 template <class T, int K>
 void go_single_reduce(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s) {
-    hipLaunchKernelGGL((k_single_reduce<T, K>), dim3(p.tiles), dim3(TX, TY, 1), 0, s, p, f, r);
+    hipLaunchKernelGGL((k_single_reduce<T, K>), dim3(p.tiles), dim3(TX, p.ty, 1), 0, s, p, f, r);
 }
 template <class T, int K>
-void go_batch(const BatchTable<T>& tab, int tiles, const Roles& r, hipStream_t s) {
-    hipLaunchKernelGGL((k_batch<T, K>), dim3(tiles), dim3(TX, TY, 1), 0, s, tab, r);
+void go_batch(const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s) {
+    hipLaunchKernelGGL((k_batch<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r);
 }
 template <class T, int K>
-void go_batch_reduce(const BatchTable<T>& tab, int tiles, const Roles& r, hipStream_t s) {
-    hipLaunchKernelGGL((k_batch_reduce<T, K>), dim3(tiles), dim3(TX, TY, 1), 0, s, tab, r);
+void go_batch_reduce(const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s) {
+    hipLaunchKernelGGL((k_batch_reduce<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r);
 }
 
 }  // namespace
@@ -1072,7 +1105,7 @@ void go_batch_reduce(const BatchTable<T>& tab, int tiles, const Roles& r, hipStr
 template <class T>
 void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s, int slot) {
     if (k == KN_SUM_PARTIALS || k == KN_DEFECT_SCALE) {
-        hipLaunchKernelGGL(k_single_sum<T>, dim3(1), dim3(TX, TY, 1), 0, s, p, f, slot,
+        hipLaunchKernelGGL(k_single_sum<T>, dim3(1), dim3(TX, 4, 1), 0, s, p, f, slot,
                            k == KN_DEFECT_SCALE ? 1 : 0);
         return;
     }
@@ -1084,14 +1117,14 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
 }
 
 template <class T>
-void launch_batch(Kernel k, const BatchTable<T>& tab, int total_tiles, const Roles& r, hipStream_t s, int slot) {
+void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot) {
     if (k == KN_SUM_PARTIALS || k == KN_DEFECT_SCALE) {
-        hipLaunchKernelGGL(k_batch_sum<T>, dim3(tab.ncases), dim3(TX, TY, 1), 0, s, tab, slot,
+        hipLaunchKernelGGL(k_batch_sum<T>, dim3(tab.ncases), dim3(TX, 4, 1), 0, s, tab, slot,
                            k == KN_DEFECT_SCALE ? 1 : 0);
         return;
     }
-#define GOB(K) go_batch<T, K>(tab, total_tiles, r, s)
-#define GORB(K) go_batch_reduce<T, K>(tab, total_tiles, r, s)
+#define GOB(K) go_batch<T, K>(tab, grid, r, s)
+#define GORB(K) go_batch_reduce<T, K>(tab, grid, r, s)
     UWS_KERNEL_CASES(GOB, GORB)
 #undef GOB
 #undef GORB

@@ -214,3 +214,85 @@ class FvmSolverHip(IFvmSolver):
         out = np.empty(self.n * nc, dtype=np.float64)
         N.check(self._lib.uws_get_field_f64(self._h, fid, N.ptr(out), out.size), f"uws_get_field_f64({fid})")
         return out.reshape(self.n, nc) if nc == 3 else out
+
+
+class FvmEnsembleHip:
+    """Several uneven cases stepped together through ``uws_batch_*``: one
+    launch per kernel for the whole ensemble, one HIP graph per step.  Case
+    ``c`` behaves exactly like ``FvmSolverHip(cfs[c])``."""
+
+    def __init__(self, cfs):
+        self._lib = N.lib()
+        self.cfs = list(cfs)
+        for cf in self.cfs:
+            cf.Init()
+        arr = (N.uws_config * len(self.cfs))(*[cf.to_c() for cf in self.cfs])
+        h = c_void_p()
+        N.check(self._lib.uws_batch_create(ctypes.cast(arr, c_void_p), len(self.cfs), byref(h)), "uws_batch_create")
+        self._h = h
+        self.shapes = [(cf.gridRes[2], cf.gridRes[1], cf.gridRes[0]) for cf in self.cfs]
+        for c, cf in enumerate(self.cfs):
+            vz = (ctypes.c_float * 3)(*[float(v) for v in cf.velZn])
+            N.check(self._lib.uws_batch_set_fixed_value_vel_bnd_cond(
+                self._h, c, int(cf.inflowType), float(cf.velX0[0]), ctypes.cast(vz, c_void_p),
+                float(cf.inflowRefHeight), float(cf.inflowAlpha)), "uws_batch_set_fixed_value_vel_bnd_cond")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.uws_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ncells(self, c: int) -> int:
+        z, y, x = self.shapes[c]
+        return x * y * z
+
+    def init_flags(self, c: int, flags) -> None:
+        a = np.ascontiguousarray(np.asarray(flags, dtype=np.int32).reshape(-1))
+        N.check(self._lib.uws_batch_init_flags(self._h, c, N.ptr(a), a.size), "uws_batch_init_flags")
+
+    def init_noise(self, c: int, seed: int, amp: float) -> None:
+        N.check(self._lib.uws_batch_init_noise(self._h, c, int(seed), float(amp)), "uws_batch_init_noise")
+
+    def set_velocity(self, c: int, u) -> None:
+        a = np.ascontiguousarray(np.asarray(u, dtype=np.float32).reshape(self.ncells(c), -1))
+        N.check(self._lib.uws_batch_set_velocity(self._h, c, N.ptr(a), a.shape[1], a.shape[0]),
+                "uws_batch_set_velocity")
+
+    def velocity(self, c: int, ncomp: int = 3) -> np.ndarray:
+        out = np.empty((self.ncells(c), ncomp), dtype=np.float32)
+        N.check(self._lib.uws_batch_get_velocity(self._h, c, N.ptr(out), ncomp, out.shape[0]),
+                "uws_batch_get_velocity")
+        return out.reshape(*self.shapes[c], ncomp)
+
+    def pressure(self, c: int) -> np.ndarray:
+        out = np.empty(self.ncells(c), dtype=np.float32)
+        N.check(self._lib.uws_batch_get_pressure(self._h, c, N.ptr(out), out.size), "uws_batch_get_pressure")
+        return out.reshape(self.shapes[c])
+
+    def step(self, nsteps: int) -> None:
+        N.check(self._lib.uws_batch_step(self._h, int(nsteps)), "uws_batch_step")
+
+    def synchronize(self) -> None:
+        N.check(self._lib.uws_batch_synchronize(self._h), "uws_batch_synchronize")
+
+    def set_launch(self, ty: int, persistent: bool) -> None:
+        N.check(self._lib.uws_batch_set_launch(self._h, int(ty), int(bool(persistent))), "uws_batch_set_launch")
+
+    def autotune(self, steps: int = 3):
+        best = (ctypes.c_int32 * 2)()
+        ms = c_double()
+        N.check(self._lib.uws_batch_autotune(self._h, int(steps), ctypes.cast(best, c_void_p), byref(ms)),
+                "uws_batch_autotune")
+        return int(best[0]), bool(best[1]), float(ms.value)

@@ -24,7 +24,7 @@ import numpy as np
 
 from .config import (FluidSimConfig, FvmSolverType, InflowType, PresBndCond, TurbulenceModel, VelBndCond,
                      WallModel)
-from .geometry import Box, lattice_layout, rasterise
+from .geometry import Box, SplitMix64, lattice_layout, rasterise
 
 U_REF, H_REF, ALPHA = 5.0, 16.0, 0.25
 
@@ -108,6 +108,34 @@ def tiny(grid=(16, 8, 12), seed=7, les=True, piso=True, vel_iters=6, pres_iters=
         cf.fvmSolverType = FvmSolverType.SIMPLE
     cf.Init()
     return Workload(f"tiny_{X}x{Y}x{Z}", cf, rasterise(grid, boxes), noise_seed=seed, steps=4, boxes=boxes)
+
+
+ENSEMBLE_GRIDS = [(64, 64, 64), (128, 64, 64), (160, 64, 96), (192, 64, 96)]
+
+
+def ensemble(n_cases=24, seed=2, grids=ENSEMBLE_GRIDS, buildings=(5, 30), steps=500) -> list:
+    """The uneven ensemble: each case draws its grid from `grids` (x, height,
+    span: 64^3, 128 x 64 span x 64 high, 160 x 96 x 64, 192 x 96 x 64), a
+    building count in `buildings` and an inflow direction 15 deg * U{0..23};
+    the inflow always enters at X0, so the building layout is rotated into the
+    solver frame by the negative inflow angle about the domain centre."""
+    rng = SplitMix64(seed)
+    out = []
+    for c in range(n_cases):
+        grid = grids[rng.randint(0, len(grids) - 1)]
+        nb = rng.randint(buildings[0], buildings[1])
+        angle = 15.0 * rng.randint(0, 23)
+        bseed = rng.next()
+        X, Y, Z = grid
+        lat = (6, 5)
+        boxes = lattice_layout(grid, nb, lat, bseed, foot=(4, 12), height=(4, 24),
+                               region=((0.2, 0.8), (0.2, 0.8)))
+        cf = base_config(grid)
+        w = Workload(f"ens{c:02d}_{X}x{Y}x{Z}_{nb}bldg_{int(angle)}deg", cf, rasterise(grid, boxes, angle),
+                     noise_seed=bseed & 0xFFFFFFFF, steps=steps, boxes=boxes)
+        w.angle = angle
+        out.append(w)
+    return out
 
 
 WORKLOADS = {
