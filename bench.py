@@ -83,7 +83,8 @@ def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
         for c, w in enumerate(ws):
             e.init_flags(c, w.flags)
             e.init_noise(c, w.noise_seed, w.noise_amp)
-        ty, pers, _ = e.autotune(2)
+        ty, pers, gbytes, _ = e.autotune(2)
+        ngroups = e.num_groups()
         e.step(1)
         e.synchronize()
         t0 = time.perf_counter()
@@ -113,7 +114,7 @@ def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
             "batched_cases_per_s": round(len(ws) / (t_batch * full_steps), 4),
             "speedup": round(t_plain / t_batch, 3),
             "plain_ms_per_step_all_cases": round(t_plain * 1e3, 3), "batched_ms_per_step": round(t_batch * 1e3, 3),
-            "launch": {"tile_rows": ty, "persistent": pers},
+            "launch": {"tile_rows": ty, "persistent": pers, "group_MiB": gbytes >> 20, "groups": ngroups},
             "max_rel_l2_u_vs_fp64": max(errs), "compared_after_steps": check_steps}
 
 

@@ -277,13 +277,19 @@ int uws_batch_get_norms(uws_batch_handle b, int32_t c, double* out, int32_t n);
 int uws_batch_step(uws_batch_handle b, int32_t nsteps);
 int uws_batch_synchronize(uws_batch_handle b);
 /* Launch shape of the ensemble kernels: tile rows ty in {1, 2, 4} (tiles of
- * 64 x ty cells) and persistent != 0 for a grid-stride launch of a fixed
- * number of workgroups instead of one workgroup per tile. */
-int uws_batch_set_launch(uws_batch_handle b, int32_t ty, int32_t persistent);
-/* Autotuner: times `steps` ensemble steps for every launch shape and keeps the
- * fastest; the solver state is saved before and restored after the trials.
- * best[0] = ty, best[1] = persistent, best_ms = ms per step of the winner. */
-int uws_batch_autotune(uws_batch_handle b, int32_t steps, int32_t best[2], double* best_ms);
+ * 64 x ty cells), persistent != 0 for a grid-stride launch of a fixed number
+ * of workgroups instead of one workgroup per tile, and group_bytes: consecutive
+ * cases are grouped while their momentum-sweep working set (77 B per cell)
+ * stays within group_bytes (<= 0: one group), each group runs its whole step
+ * (one launch per kernel for the group) before the next group, so that a
+ * group's Jacobi sweeps are served from the last-level cache. */
+int uws_batch_set_launch(uws_batch_handle b, int32_t ty, int32_t persistent, int64_t group_bytes);
+int uws_batch_num_groups(uws_batch_handle b, int32_t* n);
+/* Autotuner: times `steps` ensemble steps for each candidate launch shape
+ * (tile rows, persistent, group budget) and keeps the fastest; every case's
+ * state and clock are saved before and restored after the trials.
+ * best = {ty, persistent, group_bytes}, best_ms = ms per step of the winner. */
+int uws_batch_autotune(uws_batch_handle b, int32_t steps, int64_t best[3], double* best_ms);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,7 @@ def singles(ws, steps):
     return out
 
 
-def ensemble_run(ws, steps, ty=None, persistent=False, autotune=False):
+def ensemble_run(ws, steps, ty=None, persistent=False, autotune=False, group_bytes=0):
     with FvmEnsembleHip([w.cf for w in ws]) as e:
         for c, w in enumerate(ws):
             e.init_flags(c, w.flags)
@@ -37,16 +37,17 @@ def ensemble_run(ws, steps, ty=None, persistent=False, autotune=False):
         if autotune:
             e.autotune(2)
         if ty is not None:
-            e.set_launch(ty, persistent)
+            e.set_launch(ty, persistent, group_bytes)
         e.step(steps)
         return [(e.velocity(c, 3), e.pressure(c)) for c in range(len(ws))]
 
 
-@pytest.mark.parametrize("ty,persistent", [(4, False), (2, False), (1, False), (4, True), (1, True)])
-def test_ensemble_matches_single_cases_bitwise(gpu, ty, persistent):
+@pytest.mark.parametrize("ty,persistent,group_bytes", [(4, False, 0), (2, False, 0), (1, False, 0), (4, True, 0),
+                                                       (1, True, 0), (4, False, 1), (2, True, 400_000)])
+def test_ensemble_matches_single_cases_bitwise(gpu, ty, persistent, group_bytes):
     ws = small_cases()
     ref = singles(ws, 4)
-    got = ensemble_run(ws, 4, ty, persistent)
+    got = ensemble_run(ws, 4, ty, persistent, group_bytes=group_bytes)
     for (ug, pg), (ur, pr) in zip(got, ref):
         np.testing.assert_array_equal(ug, ur)
         np.testing.assert_array_equal(pg, pr)

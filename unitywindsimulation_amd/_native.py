@@ -120,7 +120,8 @@ SIGNATURES = {
     "uws_batch_get_norms": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
     "uws_batch_step": (c_int, [c_void_p, c_int32]),
     "uws_batch_synchronize": (c_int, [c_void_p]),
-    "uws_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32]),
+    "uws_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int64]),
+    "uws_batch_num_groups": (c_int, [c_void_p, POINTER(c_int32)]),
     "uws_batch_autotune": (c_int, [c_void_p, c_int32, c_void_p, POINTER(c_double)]),
 }
 

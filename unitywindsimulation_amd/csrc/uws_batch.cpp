@@ -13,6 +13,25 @@
 
 using namespace uws;
 
+// One launch group: cases whose sweep working set fits the budget share
+// every launch of the step.
+struct BatchGroup {
+    std::vector<int> cases;
+    BatchTable<float> tab{};
+    int grid = 0;
+    int* d_first = nullptr;
+    ParamsT<float>* d_params = nullptr;
+    FieldsT<float>* d_fields = nullptr;
+    void release() {
+        if (d_first) (void)hipFree(d_first);
+        if (d_params) (void)hipFree(d_params);
+        if (d_fields) (void)hipFree(d_fields);
+        d_first = nullptr;
+        d_params = nullptr;
+        d_fields = nullptr;
+    }
+};
+
 struct uws_batch {
     std::vector<std::unique_ptr<SolverBase>> cases;
     hipStream_t stream = nullptr;
@@ -21,20 +40,15 @@ struct uws_batch {
     RoleState rs{};
     int ty = 4;
     int persistent = 0;
-    int grid = 0;
-    BatchTable<float> tab{};
-    int* d_first = nullptr;
-    ParamsT<float>* d_params = nullptr;
-    FieldsT<float>* d_fields = nullptr;
+    int64_t group_bytes = 0;  // <= 0: one group for the whole ensemble
+    std::vector<BatchGroup> groups;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
 
     ~uws_batch() {
         drop_graphs();
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& g : groups) g.release();
         cases.clear();
-        if (d_first) (void)hipFree(d_first);
-        if (d_params) (void)hipFree(d_params);
-        if (d_fields) (void)hipFree(d_fields);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void drop_graphs() {
@@ -44,57 +58,82 @@ struct uws_batch {
     ParamsT<float>& params(int c) { return *static_cast<ParamsT<float>*>(cases[c]->device_params()); }
     FieldsT<float>& fields(int c) { return *static_cast<FieldsT<float>*>(cases[c]->device_fields()); }
 
-    // rebuild the tile table for the current launch shape
+    // Rebuild the launch groups and their tile tables for the current launch
+    // shape.  A group collects consecutive cases while the momentum sweep's
+    // working set (77 B per cell) stays within group_bytes.
     int build_table() {
+        drop_graphs();
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& g : groups) g.release();
+        groups.clear();
         const int n = (int)cases.size();
-        std::vector<int> first(n + 1, 0);
-        std::vector<ParamsT<float>> ps(n);
-        std::vector<FieldsT<float>> fs(n);
         for (int c = 0; c < n; ++c) {
             ParamsT<float>& p = params(c);
             p.ty = ty;
             p.tiles = tiles_of(p.X, p.Y, p.Z, ty);
-            first[c + 1] = first[c] + p.tiles;
-            ps[c] = p;
-            fs[c] = fields(c);
         }
-        if (hipMemcpyAsync(d_first, first.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, stream) != hipSuccess ||
-            hipMemcpyAsync(d_params, ps.data(), sizeof(ParamsT<float>) * n, hipMemcpyHostToDevice, stream) !=
-                hipSuccess ||
-            hipMemcpyAsync(d_fields, fs.data(), sizeof(FieldsT<float>) * n, hipMemcpyHostToDevice, stream) !=
-                hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess)
-            return UWS_ERR_DEVICE;
-        tab.ncases = n;
-        tab.total_tiles = first[n];
-        tab.ty = ty;
-        tab.persistent = persistent;
-        tab.first_tile = d_first;
-        tab.params = d_params;
-        tab.fields = d_fields;
-        if (persistent) {
-            // 8 XCDs x 32 CUs, up to 8 resident 256-thread workgroups per CU
-            int g = 256 * 8 * (4 / ty);
-            if (g > tab.total_tiles) g = (tab.total_tiles + 7) / 8 * 8;
-            grid = g < 8 ? 8 : g;
-        } else {
-            grid = tab.total_tiles;
+        int64_t acc = 0;
+        for (int c = 0; c < n; ++c) {
+            const int64_t b = 77LL * cases[c]->ncells();
+            if (groups.empty() || (group_bytes > 0 && acc + b > group_bytes)) {
+                groups.emplace_back();
+                acc = 0;
+            }
+            groups.back().cases.push_back(c);
+            acc += b;
         }
-        drop_graphs();
-        return UWS_OK;
+        for (auto& g : groups) {
+            const int m = (int)g.cases.size();
+            std::vector<int> first(m + 1, 0);
+            std::vector<ParamsT<float>> ps(m);
+            std::vector<FieldsT<float>> fs(m);
+            for (int i = 0; i < m; ++i) {
+                ps[i] = params(g.cases[i]);
+                fs[i] = fields(g.cases[i]);
+                first[i + 1] = first[i] + ps[i].tiles;
+            }
+            if (hipMalloc(&g.d_first, sizeof(int) * (m + 1)) != hipSuccess ||
+                hipMalloc(&g.d_params, sizeof(ParamsT<float>) * m) != hipSuccess ||
+                hipMalloc(&g.d_fields, sizeof(FieldsT<float>) * m) != hipSuccess)
+                return UWS_ERR_ALLOC;
+            if (hipMemcpyAsync(g.d_first, first.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
+                    hipSuccess ||
+                hipMemcpyAsync(g.d_params, ps.data(), sizeof(ParamsT<float>) * m, hipMemcpyHostToDevice, stream) !=
+                    hipSuccess ||
+                hipMemcpyAsync(g.d_fields, fs.data(), sizeof(FieldsT<float>) * m, hipMemcpyHostToDevice, stream) !=
+                    hipSuccess)
+                return UWS_ERR_DEVICE;
+            g.tab.ncases = m;
+            g.tab.total_tiles = first[m];
+            g.tab.ty = ty;
+            g.tab.persistent = persistent;
+            g.tab.first_tile = g.d_first;
+            g.tab.params = g.d_params;
+            g.tab.fields = g.d_fields;
+            if (persistent) {
+                // 8 XCDs x 32 CUs, up to 8 resident 256-thread workgroups per CU
+                int gr = 256 * 8 * (4 / ty);
+                if (gr > g.tab.total_tiles) gr = (g.tab.total_tiles + 7) / 8 * 8;
+                g.grid = gr < 8 ? 8 : gr;
+            } else {
+                g.grid = g.tab.total_tiles;
+            }
+        }
+        return hipStreamSynchronize(stream) == hipSuccess ? UWS_OK : UWS_ERR_DEVICE;
     }
 
     struct L {
         uws_batch* b;
+        BatchGroup* g;
         void operator()(Kernel k, const Roles& r, int slot = 0) {
-            launch_batch<float>(k, b->tab, b->grid, r, b->stream, slot);
+            launch_batch<float>(k, g->tab, g->grid, r, b->stream, slot);
         }
         int copy_vel(int dst, int src) {
-            for (auto& cs : b->cases) {
-                FieldsT<float>& f = *static_cast<FieldsT<float>*>(cs->device_fields());
-                const size_t n = (size_t)cs->ncells();
-                for (int c = 0; c < 3; ++c)
-                    if (hipMemcpyAsync(f.vel[dst][c], f.vel[src][c], sizeof(float) * n, hipMemcpyDeviceToDevice,
+            for (int c : g->cases) {
+                FieldsT<float>& f = b->fields(c);
+                const size_t n = (size_t)b->cases[c]->ncells();
+                for (int k = 0; k < 3; ++k)
+                    if (hipMemcpyAsync(f.vel[dst][k], f.vel[src][k], sizeof(float) * n, hipMemcpyDeviceToDevice,
                                        b->stream) != hipSuccess)
                         return UWS_ERR_DEVICE;
             }
@@ -112,14 +151,27 @@ struct uws_batch {
         }
     }
 
+    // every group runs its whole step before the next group starts, so a
+    // group's sweeps reuse its data from the last-level cache
+    int enqueue_all() {
+        RoleState out = rs;
+        for (auto& g : groups) {
+            RoleState r = rs;
+            L l{this, &g};
+            if (int rc = enqueue_step(l, sc, r)) return rc;
+            out = r;
+        }
+        rs = out;
+        return UWS_OK;
+    }
+
     int step_once() {
         const int s0 = rs.state;
         if (!graph[s0]) {
             const RoleState saved = rs;
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return UWS_ERR_DEVICE;
-            L l{this};
-            int rc = enqueue_step(l, sc, rs);
+            int rc = enqueue_all();
             hipError_t e = hipStreamEndCapture(stream, &g);
             rs = saved;
             if (rc == UWS_OK && hipGetLastError() != hipSuccess) rc = UWS_ERR_DEVICE;
@@ -182,10 +234,6 @@ int uws_batch_create(const uws_config* cfgs, int32_t ncases, uws_batch_handle* o
     b->sc = step_cfg(cfgs[0]);
     b->sc.normalise = false;
     for (int c = 0; c < ncases; ++c) b->sc.normalise |= step_cfg(cfgs[c]).normalise;
-    if (hipMalloc(&b->d_first, sizeof(int) * (ncases + 1)) != hipSuccess ||
-        hipMalloc(&b->d_params, sizeof(ParamsT<float>) * ncases) != hipSuccess ||
-        hipMalloc(&b->d_fields, sizeof(FieldsT<float>) * ncases) != hipSuccess)
-        return UWS_ERR_ALLOC;
     if (int rc = b->build_table()) return rc;
     *out = b.release();
     return UWS_OK;
@@ -273,35 +321,49 @@ int uws_batch_synchronize(uws_batch_handle b) {
     return hipStreamSynchronize(b->stream) == hipSuccess ? UWS_OK : UWS_ERR_DEVICE;
 }
 
-int uws_batch_set_launch(uws_batch_handle b, int32_t ty, int32_t persistent) {
+int uws_batch_set_launch(uws_batch_handle b, int32_t ty, int32_t persistent, int64_t group_bytes) {
     if (!b || (ty != 1 && ty != 2 && ty != 4) || (persistent != 0 && persistent != 1)) return UWS_ERR_INVALID_ARG;
     if (hipSetDevice(b->device) != hipSuccess) return UWS_ERR_DEVICE;
     if (hipStreamSynchronize(b->stream) != hipSuccess) return UWS_ERR_DEVICE;
     b->ty = ty;
     b->persistent = persistent;
+    b->group_bytes = group_bytes;
     return b->build_table();
 }
 
-int uws_batch_autotune(uws_batch_handle b, int32_t steps, int32_t best[2], double* best_ms) {
+int uws_batch_num_groups(uws_batch_handle b, int32_t* n) {
+    if (!b || !n) return UWS_ERR_INVALID_ARG;
+    *n = (int32_t)b->groups.size();
+    return UWS_OK;
+}
+
+int uws_batch_autotune(uws_batch_handle b, int32_t steps, int64_t best[3], double* best_ms) {
     if (!b || steps <= 0 || !best || !best_ms) return UWS_ERR_INVALID_ARG;
     if (hipSetDevice(b->device) != hipSuccess) return UWS_ERR_DEVICE;
-    // save every case's state (u^n, p^n) so the trials leave no trace
+    // save every case's state (u^n, p^n) and clock so the trials leave no trace
     const int n = (int)b->cases.size();
     std::vector<std::vector<float>> su(n), sp(n);
+    std::vector<int64_t> st(n);
+    std::vector<double> pt(n);
     for (int c = 0; c < n; ++c) {
         SolverBase& s = *b->cases[c];
         s.rs.state = b->rs.state;
         su[c].resize((size_t)s.ncells() * 3);
         sp[c].resize((size_t)s.ncells());
+        st[c] = s.step;
+        pt[c] = s.phys_time;
         if (int rc = s.get_velocity(su[c].data(), 3)) return rc;
         if (int rc = s.get_plane(PLANE_P, 0, sp[c].data(), sp[c].size())) return rc;
     }
-    const int tys[3] = {4, 2, 1};
+    // candidate launch shapes: (tile rows, persistent, group budget in bytes)
+    const int64_t MB = 1 << 20;
+    const int64_t budgets[4] = {0, 384 * MB, 192 * MB, 96 * MB};
+    const int shapes[3][2] = {{4, 0}, {2, 0}, {4, 1}};
     double bestv = 1e300;
-    int bty = 4, bp = 0;
-    for (int pers = 0; pers < 2; ++pers)
-        for (int ty : tys) {
-            if (int rc = uws_batch_set_launch(b, ty, pers)) return rc;
+    int64_t bb[3] = {4, 0, 0};
+    for (int64_t gb : budgets)
+        for (const auto& sh : shapes) {
+            if (int rc = uws_batch_set_launch(b, sh[0], sh[1], gb)) return rc;
             if (int rc = uws_batch_step(b, 1)) return rc;  // capture + warm
             if (hipStreamSynchronize(b->stream) != hipSuccess) return UWS_ERR_DEVICE;
             const auto t0 = std::chrono::steady_clock::now();
@@ -309,20 +371,21 @@ int uws_batch_autotune(uws_batch_handle b, int32_t steps, int32_t best[2], doubl
             if (hipStreamSynchronize(b->stream) != hipSuccess) return UWS_ERR_DEVICE;
             const double ms =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
-            if (ms < bestv) { bestv = ms; bty = ty; bp = pers; }
+            if (ms < bestv) { bestv = ms; bb[0] = sh[0]; bb[1] = sh[1]; bb[2] = gb; }
         }
-    if (int rc = uws_batch_set_launch(b, bty, bp)) return rc;
+    if (int rc = uws_batch_set_launch(b, (int)bb[0], (int)bb[1], bb[2])) return rc;
     for (int c = 0; c < n; ++c) {
         SolverBase& s = *b->cases[c];
         s.rs.state = b->rs.state;
         if (int rc = s.set_velocity_all(su[c].data(), 3)) return rc;
         if (int rc = s.set_plane(PLANE_P, 0, sp[c].data(), sp[c].size())) return rc;
-        s.step -= (int64_t)(steps + 1) * 6;
-        s.phys_time -= (double)s.cfg.dt * (steps + 1) * 6;
+        s.step = st[c];
+        s.phys_time = pt[c];
         s.rs.state = b->rs.state;
     }
-    best[0] = bty;
-    best[1] = bp;
+    best[0] = bb[0];
+    best[1] = bb[1];
+    best[2] = bb[2];
     *best_ms = bestv;
     return UWS_OK;
 }

@@ -27,6 +27,7 @@ namespace {
 // workgroups are 64 x ty threads: one wave per x-row of 64 cells, ty rows
 // (ty in {1, 2, 4}, chosen per solver or per ensemble)
 constexpr int TX = 64;
+constexpr int SINGLE_TY = 4;  // tile rows of single-case launches (ParamsT::ty of a handle)
 
 enum { E_ = 0, W_ = 1, N_ = 2, S_ = 3, U_ = 4, D_ = 5 };
 __constant__ int c_off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
@@ -49,9 +50,10 @@ struct Cell {
     bool ok;
 };
 
-template <class T>
+// TYC > 0: tile rows known at compile time (single-case launches use 4)
+template <int TYC = 0, class T>
 __device__ __forceinline__ Cell cell_of(const ParamsT<T>& p, int t) {
-    const int ty = p.ty;
+    const int ty = TYC > 0 ? TYC : p.ty;
     const int gx = (p.X + TX - 1) / TX, gy = (p.Y + ty - 1) / ty;
     const int bx = t % gx, r = t / gx, by = r % gy, bz = r / gy;
     Cell c;
@@ -966,11 +968,15 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
+// the case owning tile t: the last c with first_tile[c] <= t (binary search)
 template <class T>
 __device__ __forceinline__ int case_of(const BatchTable<T>& tab, int t) {
-    int c = 0;
-    while (c + 1 < tab.ncases && tab.first_tile[c + 1] <= t) ++c;
-    return c;
+    int lo = 0, hi = tab.ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab.first_tile[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Persistent ensemble launch: the grid is a multiple of 8 workgroups, XCD j
@@ -983,7 +989,7 @@ __device__ __forceinline__ void xcd_range(int total, int xcd, int* lo, int* hi) 
 
 template <class T, int K>
 __global__ __launch_bounds__(256) void k_single(ParamsT<T> p, FieldsT<T> f, Roles r) {
-    const Cell c = cell_of(p, xcd_tile((int)blockIdx.x, p.tiles));
+    const Cell c = cell_of<SINGLE_TY>(p, xcd_tile((int)blockIdx.x, p.tiles));
     run_body<T, K>(p, f, c, r);
 }
 template <class T, int K>
@@ -1007,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchTable<T> tab, Roles r) {
 }
 template <class T, int K>
 __global__ __launch_bounds__(256) void k_single_reduce(ParamsT<T> p, FieldsT<T> f, Roles r) {
-    const Cell c = cell_of(p, xcd_tile((int)blockIdx.x, p.tiles));
+    const Cell c = cell_of<SINGLE_TY>(p, xcd_tile((int)blockIdx.x, p.tiles));
     const double s = block_sum(run_reduce<T, K>(p, f, c, r));
     if (threadIdx.x == 0 && threadIdx.y == 0) f.partial[c.tile] = s;
 }
@@ -1060,11 +1066,11 @@ __global__ __launch_bounds__(256) void k_batch_sum(BatchTable<T> tab, int slot, 
 
 template <class T, int K>
 void go_single(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s) {
-    hipLaunchKernelGGL((k_single<T, K>), dim3(p.tiles), dim3(TX, p.ty, 1), 0, s, p, f, r);
+    hipLaunchKernelGGL((k_single<T, K>), dim3(p.tiles), dim3(TX, SINGLE_TY, 1), 0, s, p, f, r);
 }
 template <class T, int K>
 void go_single_reduce(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s) {
-    hipLaunchKernelGGL((k_single_reduce<T, K>), dim3(p.tiles), dim3(TX, p.ty, 1), 0, s, p, f, r);
+    hipLaunchKernelGGL((k_single_reduce<T, K>), dim3(p.tiles), dim3(TX, SINGLE_TY, 1), 0, s, p, f, r);
 }
 template <class T, int K>
 void go_batch(const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s) {

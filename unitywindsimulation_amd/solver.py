@@ -287,12 +287,19 @@ class FvmEnsembleHip:
     def synchronize(self) -> None:
         N.check(self._lib.uws_batch_synchronize(self._h), "uws_batch_synchronize")
 
-    def set_launch(self, ty: int, persistent: bool) -> None:
-        N.check(self._lib.uws_batch_set_launch(self._h, int(ty), int(bool(persistent))), "uws_batch_set_launch")
+    def set_launch(self, ty: int, persistent: bool, group_bytes: int = 0) -> None:
+        N.check(self._lib.uws_batch_set_launch(self._h, int(ty), int(bool(persistent)), int(group_bytes)),
+                "uws_batch_set_launch")
+
+    def num_groups(self) -> int:
+        n = ctypes.c_int32()
+        N.check(self._lib.uws_batch_num_groups(self._h, byref(n)), "uws_batch_num_groups")
+        return int(n.value)
 
     def autotune(self, steps: int = 3):
-        best = (ctypes.c_int32 * 2)()
+        """Pick the fastest launch shape; returns (tile rows, persistent, group bytes, ms per step)."""
+        best = (ctypes.c_int64 * 3)()
         ms = c_double()
         N.check(self._lib.uws_batch_autotune(self._h, int(steps), ctypes.cast(best, c_void_p), byref(ms)),
                 "uws_batch_autotune")
-        return int(best[0]), bool(best[1]), float(ms.value)
+        return int(best[0]), bool(best[1]), int(best[2]), float(ms.value)
