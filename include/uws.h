@@ -111,8 +111,11 @@ typedef struct uws_config {
  * the same step in double on the GPU; FP32_FP16 / FP32_BF16 solve every
  * pressure equation as FP32 defect correction (refine_rounds rounds of
  * refine_sweeps Jacobi sweeps) with the correction iterate stored in 2 bytes,
- * instead of pres_max_num_iter plain FP32 sweeps. */
-enum { UWS_PREC_FP32 = 0, UWS_PREC_FP64 = 1, UWS_PREC_FP32_FP16 = 2, UWS_PREC_FP32_BF16 = 3 };
+ * instead of pres_max_num_iter plain FP32 sweeps.  CSTEP runs the FP64 step
+ * with every value a complex number (complex-step differentiation: the
+ * imaginary parts carry h times the derivative along a chosen direction,
+ * see uws_cstep_set_imag). */
+enum { UWS_PREC_FP32 = 0, UWS_PREC_FP64 = 1, UWS_PREC_FP32_FP16 = 2, UWS_PREC_FP32_BF16 = 3, UWS_PREC_CSTEP = 4 };
 
 typedef struct uws_solver* uws_handle;
 
@@ -290,6 +293,47 @@ int uws_batch_num_groups(uws_batch_handle b, int32_t* n);
  * state and clock are saved before and restored after the trials.
  * best = {ty, persistent, group_bytes}, best_ms = ms per step of the winner. */
 int uws_batch_autotune(uws_batch_handle b, int32_t steps, int64_t best[3], double* best_ms);
+
+/* ---- differentiation (extension, no reference counterpart) ---- */
+
+/* FluidSimConfig.smagorinskyConstant (FluidSimConfig.cs:156) of a live
+ * handle, in double; used from the next step on. */
+int uws_set_smagorinsky_constant(uws_handle h, double cs);
+/* Boundary texture upload in double (exact for FP64 and complex-step runs);
+ * layout as uws_set_bnd_field. */
+int uws_set_bnd_field_f64(uws_handle h, int32_t face, const double* xyz, int64_t n);
+
+/* Complex-step runs (precision UWS_PREC_CSTEP).  The real-valued setters set
+ * real parts with a zero imaginary part; these set or read the imaginary
+ * parts only.  what = UWS_CSTEP_VEL ([cell][3], every velocity buffer),
+ * UWS_CSTEP_PRES ([cell]), UWS_CSTEP_BND (boundary texture `face`, [j][i][3]),
+ * UWS_CSTEP_CS (one value: the Smagorinsky constant). */
+enum { UWS_CSTEP_VEL = 0, UWS_CSTEP_PRES = 1, UWS_CSTEP_BND = 2, UWS_CSTEP_CS = 3 };
+int uws_cstep_set_imag(uws_handle h, int32_t what, int32_t face, const double* im, int64_t n);
+int uws_cstep_get_imag(uws_handle h, int32_t what, int32_t face, double* im, int64_t n);
+
+/* Discrete adjoint of nsteps solver steps (FP32 or FP64 handles, no
+ * residual-based early exit, no low-precision mode, no pressure
+ * normalisation).  uws_adjoint_forward runs the nsteps steps on the handle
+ * from its current state, keeping (u, p) every ckpt_every steps; the handle
+ * ends at the final state.  uws_adjoint_backward takes the adjoint of the
+ * final state, lam_u = dJ/du ([cell][3]) and lam_p = dJ/dp ([cell]; NULL for
+ * zero), and walks the steps backwards: each checkpoint segment is recomputed
+ * with every iterate kept, and every step is reversed: the Jacobi sweeps as
+ * transposed sweeps, the one-shot kernels through exact local Jacobians.
+ * Gradients are then read with uws_adjoint_get_grad: UWS_GRAD_VEL0 (dJ/du at
+ * the start, [cell][3]), UWS_GRAD_PRES0 ([cell]), UWS_GRAD_BND (boundary
+ * texture `face`, [j][i][3]), UWS_GRAD_CS (one value). */
+typedef struct uws_adjoint* uws_adjoint_handle;
+enum { UWS_GRAD_VEL0 = 0, UWS_GRAD_PRES0 = 1, UWS_GRAD_BND = 2, UWS_GRAD_CS = 3 };
+int uws_adjoint_create(uws_handle h, int32_t nsteps, int32_t ckpt_every, uws_adjoint_handle* out);
+int uws_adjoint_destroy(uws_adjoint_handle a);
+int uws_adjoint_forward(uws_adjoint_handle a);
+int uws_adjoint_backward(uws_adjoint_handle a, const double* lam_u, int64_t n_u, const double* lam_p, int64_t n_p);
+int uws_adjoint_get_grad(uws_adjoint_handle a, int32_t what, int32_t face, double* dst, int64_t n);
+/* device bytes held by the adjoint (checkpoints, one segment of step
+ * records, adjoint fields) */
+int uws_adjoint_memory(uws_adjoint_handle a, int64_t* bytes);
 
 #ifdef __cplusplus
 }

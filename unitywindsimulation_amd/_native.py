@@ -29,7 +29,9 @@ FIELD_EDDY_VIS, FIELD_WALL_VIS, FIELD_B, FIELD_D, FIELD_DP, FIELD_BP, FIELD_AODU
 FIELD_NCOMP = {0: 3, 1: 1, 2: 3, 3: 3, 4: 1, 5: 1, 6: 3, 7: 3, 8: 3, 9: 1, 10: 1, 11: 3}
 
 KERNEL_VEL_SWEEP, KERNEL_PRES_SWEEP = 0, 1
-PREC_FP32, PREC_FP64, PREC_FP32_FP16, PREC_FP32_BF16 = 0, 1, 2, 3
+PREC_FP32, PREC_FP64, PREC_FP32_FP16, PREC_FP32_BF16, PREC_CSTEP = 0, 1, 2, 3, 4
+CSTEP_VEL, CSTEP_PRES, CSTEP_BND, CSTEP_CS = 0, 1, 2, 3
+GRAD_VEL0, GRAD_PRES0, GRAD_BND, GRAD_CS = 0, 1, 2, 3
 NORM_VEL_RESIDUAL, NORM_PRES_RESIDUAL, NORM_DIVERGENCE, NORM_NONFINITE, NORM_REFINE0, NORM_COUNT = 0, 1, 2, 3, 4, 64
 
 (STAGE_WALL_FUNC, STAGE_EDDY_VIS, STAGE_LES_TERM, STAGE_VEL_PRECOMP, STAGE_VEL_SWEEPS, STAGE_PRES_PRECOMP,
@@ -123,6 +125,16 @@ SIGNATURES = {
     "uws_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int64]),
     "uws_batch_num_groups": (c_int, [c_void_p, POINTER(c_int32)]),
     "uws_batch_autotune": (c_int, [c_void_p, c_int32, c_void_p, POINTER(c_double)]),
+    "uws_set_smagorinsky_constant": (c_int, [c_void_p, c_double]),
+    "uws_set_bnd_field_f64": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
+    "uws_cstep_set_imag": (c_int, [c_void_p, c_int32, c_int32, c_void_p, c_int64]),
+    "uws_cstep_get_imag": (c_int, [c_void_p, c_int32, c_int32, c_void_p, c_int64]),
+    "uws_adjoint_create": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_void_p)]),
+    "uws_adjoint_destroy": (c_int, [c_void_p]),
+    "uws_adjoint_forward": (c_int, [c_void_p]),
+    "uws_adjoint_backward": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64]),
+    "uws_adjoint_get_grad": (c_int, [c_void_p, c_int32, c_int32, c_void_p, c_int64]),
+    "uws_adjoint_memory": (c_int, [c_void_p, POINTER(c_int64)]),
 }
 
 _lib = None

@@ -1,0 +1,187 @@
+"""Discrete adjoint of the solver step (``uws_adjoint_*``) and its
+``torch.autograd.Function``.
+
+``FvmAdjoint(solver, nsteps, ckpt_every)`` wraps one handle: ``forward()``
+runs ``nsteps`` steps keeping a checkpoint every ``ckpt_every`` steps,
+``backward(lam_u, lam_p)`` takes dJ/du and dJ/dp of the final state and
+returns the gradients with respect to the initial state, every boundary
+texture and the Smagorinsky constant.  ``FvmUnroll`` exposes the same
+forward/backward pair to torch autograd, so objectives and inflow
+parameterisations can be written in torch and differentiated end to end.
+The arithmetic is the handle's (FP32 or FP64); the forward of an FP32 handle
+is the plain FP32 step, bit for bit.
+"""
+from __future__ import annotations
+
+from ctypes import byref, c_double, c_int64, c_void_p
+
+import numpy as np
+
+from . import _native as N
+
+
+class FvmAdjoint:
+    """Checkpointed discrete adjoint of ``nsteps`` steps of a ``FvmSolverHip``."""
+
+    def __init__(self, solver, nsteps: int, ckpt_every: int = 8):
+        self.solver = solver
+        self.nsteps = int(nsteps)
+        self.ckpt_every = int(ckpt_every)
+        self._lib = N.lib()
+        h = c_void_p()
+        N.check(self._lib.uws_adjoint_create(solver._h, self.nsteps, self.ckpt_every, byref(h)),
+                "uws_adjoint_create")
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.uws_adjoint_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def memory_bytes(self) -> int:
+        b = c_int64()
+        N.check(self._lib.uws_adjoint_memory(self._h, byref(b)), "uws_adjoint_memory")
+        return int(b.value)
+
+    def forward(self) -> None:
+        """Run the steps on the solver (which ends at the final state)."""
+        N.check(self._lib.uws_adjoint_forward(self._h), "uws_adjoint_forward")
+        s = self.solver
+        st, t = c_int64(), c_double()
+        s._lib.uws_get_time(s._h, byref(st), byref(t))
+        s.cf._advance(int(st.value), float(t.value))
+
+    def backward(self, lam_u, lam_p=None) -> None:
+        n = self.solver.n
+        lu = np.ascontiguousarray(np.asarray(lam_u, dtype=np.float64).reshape(n, 3))
+        if lam_p is None:
+            N.check(self._lib.uws_adjoint_backward(self._h, N.ptr(lu), lu.size, None, 0), "uws_adjoint_backward")
+        else:
+            lp = np.ascontiguousarray(np.asarray(lam_p, dtype=np.float64).reshape(n))
+            N.check(self._lib.uws_adjoint_backward(self._h, N.ptr(lu), lu.size, N.ptr(lp), lp.size),
+                    "uws_adjoint_backward")
+
+    def _grad(self, what: int, face: int, size: int) -> np.ndarray:
+        out = np.empty(size, dtype=np.float64)
+        N.check(self._lib.uws_adjoint_get_grad(self._h, what, face, N.ptr(out), size), "uws_adjoint_get_grad")
+        return out
+
+    def grad_velocity0(self) -> np.ndarray:
+        return self._grad(N.GRAD_VEL0, 0, 3 * self.solver.n).reshape(-1, 3)
+
+    def grad_pressure0(self) -> np.ndarray:
+        return self._grad(N.GRAD_PRES0, 0, self.solver.n)
+
+    def grad_bnd(self, face: int) -> np.ndarray:
+        return self._grad(N.GRAD_BND, face, 3 * self.solver.bnd_size(face)).reshape(-1, 3)
+
+    def grad_cs(self) -> float:
+        return float(self._grad(N.GRAD_CS, 0, 1)[0])
+
+
+def _unroll_function():
+    import torch
+
+    class FvmUnroll(torch.autograd.Function):
+        """``u_T, p_T = FvmUnroll.apply(adj, bnd_x0, cs, u0, p0)``.
+
+        ``bnd_x0``: the X0 boundary texture as a ``(Y*Z, 3)`` float64 tensor
+        (element ``z*Y + y``), ``cs``: a 0-d tensor; ``u0`` ``(N, 3)`` and
+        ``p0`` ``(N,)`` set the initial state (uploaded as float32) or are
+        ``None`` to start from the solver's current state.  Outputs are float64
+        CPU tensors ``(N, 3)`` and ``(N,)`` in ``[z][y][x]`` cell order."""
+
+        @staticmethod
+        def forward(ctx, adj, bnd_x0, cs, u0=None, p0=None):
+            s = adj.solver
+            s.set_bnd_field64(N.X0, bnd_x0.detach().cpu().double().numpy())
+            s.set_smagorinsky_constant(float(cs))
+            if u0 is not None:
+                s.set_velocity(u0.detach().cpu().float().numpy())
+            if p0 is not None:
+                s.set_pressure(p0.detach().cpu().float().numpy())
+            adj.forward()
+            ctx.adj = adj
+            ctx.want = (u0 is not None, p0 is not None)
+            u = torch.from_numpy(np.ascontiguousarray(s.field64(N.FIELD_VEL)))
+            p = torch.from_numpy(np.ascontiguousarray(s.field64(N.FIELD_PRES)))
+            return u, p
+
+        @staticmethod
+        def backward(ctx, gu, gp):
+            adj = ctx.adj
+            n = adj.solver.n
+            lam_u = gu.detach().cpu().double().numpy() if gu is not None else np.zeros((n, 3))
+            lam_p = gp.detach().cpu().double().numpy() if gp is not None else None
+            adj.backward(lam_u, lam_p)
+            g_bnd = torch.from_numpy(adj.grad_bnd(N.X0))
+            g_cs = torch.tensor(adj.grad_cs(), dtype=torch.float64)
+            g_u0 = torch.from_numpy(adj.grad_velocity0()) if ctx.want[0] else None
+            g_p0 = torch.from_numpy(adj.grad_pressure0()) if ctx.want[1] else None
+            return None, g_bnd, g_cs, g_u0, g_p0
+
+    return FvmUnroll
+
+
+_FVM_UNROLL = None
+
+
+def fvm_unroll(adj: FvmAdjoint, bnd_x0, cs, u0=None, p0=None):
+    """Differentiable ``nsteps``-step unroll: see ``FvmUnroll``."""
+    global _FVM_UNROLL
+    if _FVM_UNROLL is None:
+        _FVM_UNROLL = _unroll_function()
+    return _FVM_UNROLL.apply(adj, bnd_x0, cs, u0, p0)
+
+
+# ---------------------------------------------------------------------------
+# the wind-comfort objective of the differentiation workload
+# ---------------------------------------------------------------------------
+def layer_objective(u: np.ndarray, grid, region, layers=(1, 2)):
+    """J = mean over the (x, z) region of |u_bar|^2, u_bar the average of the
+    velocity over the given y layers (layers 1 and 2 of a dx = 1 m grid have
+    their centres at 1.5 m and 2.5 m: their average is the linear
+    interpolation at 2 m).  Returns (J, dJ/du) with dJ/du shaped like u
+    ``(N, 3)``."""
+    X, Y, Z = grid
+    x0, z0, w, d = region
+    uu = np.asarray(u, dtype=np.float64).reshape(Z, Y, X, 3)
+    ub = sum(uu[z0:z0 + d, j, x0:x0 + w, :] for j in layers) / len(layers)
+    m = float(w * d)
+    J = float((ub * ub).sum() / m)
+    g = np.zeros_like(uu)
+    for j in layers:
+        g[z0:z0 + d, j, x0:x0 + w, :] += 2.0 * ub / (m * len(layers))
+    return J, g.reshape(-1, 3)
+
+
+def power_law_bnd(grid, u_ref, alpha, delta, ref_height=16.0, dx=1.0, xp=np):
+    """X0 boundary texture (element z*Y + y, 3 components) of the power-law
+    inlet u = u_ref ((j + 1/2) dx / ref_height)^alpha + delta_j, written with
+    the array module ``xp`` (numpy or torch) so that it can be differentiated."""
+    X, Y, Z = grid
+    if xp is np:
+        yc = (np.arange(Y) + 0.5) * dx
+        col = u_ref * (yc / ref_height) ** alpha + np.asarray(delta, dtype=np.float64)
+        out = np.zeros((Z * Y, 3))
+        out[:, 0] = np.tile(col, Z)
+        return out
+    import torch
+    yc = (torch.arange(Y, dtype=torch.float64) + 0.5) * dx
+    col = u_ref * (yc / ref_height) ** alpha + delta
+    ux = col.repeat(Z)
+    zero = torch.zeros_like(ux)
+    return torch.stack([ux, zero, zero], dim=1)

@@ -17,7 +17,7 @@ REPO_DIR = PKG_DIR.parent
 CSRC = PKG_DIR / "csrc"
 LIB_PATH = PKG_DIR / "libuws_hip.so"
 
-SOURCES = ["uws_kernels.hip", "uws_api.cpp", "uws_batch.cpp"]
+SOURCES = ["uws_kernels.hip", "uws_adjoint.hip", "uws_api.cpp", "uws_batch.cpp", "uws_adjoint.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

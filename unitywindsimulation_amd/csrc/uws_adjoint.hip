@@ -1,0 +1,343 @@
+// Kernels of the discrete adjoint (see uws_adjoint.h).  Every kernel is one
+// thread per cell (or boundary element) in gather form: each thread writes
+// only its own cell, so the adjoint is deterministic without atomics.
+#include "uws_adjoint.h"
+
+namespace uws {
+namespace {
+
+constexpr int BLK = 256;
+inline int blocks(int n) { return (n + BLK - 1) / BLK; }
+
+__device__ __forceinline__ int gid() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
+
+// ---------------------------------------------------------------------------
+// Transposed momentum sweep.  For cell Q and component a:
+//   lam_in(Q) = sum over the cells P that read Q of dout(P)/du(Q) lam_out(P)
+// with dout(P)/du(E of P) = invD(P) (-q S_E(P) + G_E(P)) and
+// dout(P)/du(W of P) = invD(P) (q S_E(W) + G_E(W)) (and alike in y, z);
+// coefficients of faces that are not fluid-fluid are zero, and invD is zero
+// off the fluid, so no flags are needed for this part.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(BLK) void k_adj_vel_sweep(AdjVelSweep<T> a, int N) {
+    const int i = gid();
+    if (i >= N) return;
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const int x = i % X, t = i / X, y = t % Y, z = t / Y;
+    const int sy = X, sz = X * Y;
+    const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
+    const bool isF = a.flag[i] == UWS_CELL_FLUID;
+    const T q = a.q;
+    T rP[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T* lo = a.lam_out[c];
+        const T* iv = a.invD[c];
+        rP[c] = lo[i] * iv[i];
+        T acc = 0;
+        if (hW) acc += lo[i - 1] * iv[i - 1] * (-q * a.S[0][i - 1] + a.G[0][i - 1]);
+        if (hE) acc += lo[i + 1] * iv[i + 1] * (q * a.S[0][i] + a.G[0][i]);
+        if (hS) acc += lo[i - sy] * iv[i - sy] * (-q * a.S[1][i - sy] + a.G[1][i - sy]);
+        if (hN) acc += lo[i + sy] * iv[i + sy] * (q * a.S[1][i] + a.G[1][i]);
+        if (hD) acc += lo[i - sz] * iv[i - sz] * (-q * a.S[2][i - sz] + a.G[2][i - sz]);
+        if (hU) acc += lo[i + sz] * iv[i + sz] * (q * a.S[2][i] + a.G[2][i]);
+        a.lam_in[c][i] = acc;
+    }
+    if (isF) {
+        // the bracket A + Df + b of the forward sweep, on the input iterate
+        const int iE = hE ? i + 1 : i, iW = hW ? i - 1 : i, iN = hN ? i + sy : i, iS = hS ? i - sy : i;
+        const int iU = hU ? i + sz : i, iD = hD ? i - sz : i;
+        const T SE = a.S[0][i], SN = a.S[1][i], SU = a.S[2][i];
+        const T GE = a.G[0][i], GN = a.G[1][i], GU = a.G[2][i];
+        const T SW = hW ? a.S[0][iW] : (T)0, GW = hW ? a.G[0][iW] : (T)0;
+        const T SS = hS ? a.S[1][iS] : (T)0, GS = hS ? a.G[1][iS] : (T)0;
+        const T SD = hD ? a.S[2][iD] : (T)0, GD = hD ? a.G[2][iD] : (T)0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T* k = a.in[c];
+            const T vE = k[iE], vW = k[iW], vN = k[iN], vS = k[iS], vU = k[iU], vD = k[iD];
+            T A = 0;
+            A -= q * vE * SE;
+            A += q * vW * SW;
+            A -= q * vN * SN;
+            A += q * vS * SS;
+            A -= q * vU * SU;
+            A += q * vD * SD;
+            T Df = 0;
+            Df += GE * vE;
+            Df += GW * vW;
+            Df += GN * vN;
+            Df += GS * vS;
+            Df += GU * vU;
+            Df += GD * vD;
+            a.lam_invD[c][i] += a.lam_out[c][i] * (A + Df + a.b[c][i]);
+            a.lam_b[c][i] += rP[c];
+        }
+    }
+    // owned faces E, N, U: S and G enter out(P) and out(P + e)
+    const int offs[3] = {1, sy, sz};
+    const bool has[3] = {hE, hN, hU};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        if (!has[e]) continue;
+        const int j = i + offs[e];
+        T sS = 0, sG = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T rQ = a.lam_out[c][j] * a.invD[c][j];
+            const T uP = a.in[c][i], uQ = a.in[c][j];
+            sS += rP[c] * (-q * uQ) + rQ * (q * uP);
+            sG += rP[c] * uQ + rQ * uP;
+        }
+        a.lam_S[e][i] += sS;
+        a.lam_G[e][i] += sG;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transposed pressure sweep.  out(P) = invDp(P) (term(P) + rhs(P)) with
+// term(P) = -sum_f c_f x(nb); c_f of a face is stored on the cell owning it.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(BLK) void k_adj_pres_sweep(AdjPresSweep<T> a, int N) {
+    const int i = gid();
+    if (i >= N) return;
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const int x = i % X, t = i / X, y = t % Y, z = t / Y;
+    const int sy = X, sz = X * Y;
+    const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
+    const T* lo = a.lam_out;
+    const T* iv = a.invDp;
+    const T rP = lo[i] * iv[i];
+    const T* xin = a.in;
+    if (xin) {
+        T acc = 0;
+        if (hW) acc -= lo[i - 1] * iv[i - 1] * a.cE[i - 1];
+        if (hE) acc -= lo[i + 1] * iv[i + 1] * a.cE[i];
+        if (hS) acc -= lo[i - sy] * iv[i - sy] * a.cN[i - sy];
+        if (hN) acc -= lo[i + sy] * iv[i + sy] * a.cN[i];
+        if (hD) acc -= lo[i - sz] * iv[i - sz] * a.cU[i - sz];
+        if (hU) acc -= lo[i + sz] * iv[i + sz] * a.cU[i];
+        a.lam_in[i] = acc;
+    }
+    if (a.flag[i] == UWS_CELL_FLUID) {
+        T term = 0;
+        if (xin) {
+            const int iE = hE ? i + 1 : i, iW = hW ? i - 1 : i, iN = hN ? i + sy : i, iS = hS ? i - sy : i;
+            const int iU = hU ? i + sz : i, iD = hD ? i - sz : i;
+            const T cW = hW ? a.cE[iW] : (T)0, cS = hS ? a.cN[iS] : (T)0, cD = hD ? a.cU[iD] : (T)0;
+            term -= a.cE[i] * xin[iE];
+            term -= cW * xin[iW];
+            term -= a.cN[i] * xin[iN];
+            term -= cS * xin[iS];
+            term -= a.cU[i] * xin[iU];
+            term -= cD * xin[iD];
+        }
+        a.lam_invDp[i] += lo[i] * (term + a.rhs[i]);
+        a.lam_rhs[i] += rP;
+    }
+    if (xin) {
+        if (hE) a.lam_cE[i] -= rP * xin[i + 1] + lo[i + 1] * iv[i + 1] * xin[i];
+        if (hN) a.lam_cN[i] -= rP * xin[i + sy] + lo[i + sy] * iv[i + sy] * xin[i];
+        if (hU) a.lam_cU[i] -= rP * xin[i + sz] + lo[i + sz] * iv[i + sz] * xin[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward-mode machinery
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(BLK) void k_dual_pack(const T* src, Dual<T>* dst, int n) {
+    const int i = gid();
+    if (i < n) dst[i] = Dual<T>(src ? src[i] : (T)0, (T)0);
+}
+
+__device__ __forceinline__ int colour_at(const Colour& c, int x, int y, int z) {
+    return c.m == 1 ? 0 : (x + c.wy * y + c.wz * z) % c.m;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLK) void k_dual_seed(Dual<T>* dst, int W, int H, int D, Colour c, int k) {
+    const int i = gid();
+    if (i >= W * H * D) return;
+    const int x = i % W, t = i / W, y = t % H, z = t / H;
+    dst[i].d = (k >= 0 && colour_at(c, x, y, z) == k) ? (T)1 : (T)0;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLK) void k_dual_contract(Pairs<T> pr, T* t, int n) {
+    const int i = gid();
+    if (i >= n) return;
+    T s = 0;
+    for (int j = 0; j < pr.n; ++j) s += pr.lam[j][i] * pr.d[j][i].d;
+    t[i] = s;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLK) void k_adj_gather(const T* t, int X, int Y, int Z, Stencil st, Colour c, int k,
+                                                   T* dst) {
+    const int i = gid();
+    if (i >= X * Y * Z) return;
+    const int x = i % X, r = i / X, y = r % Y, z = r / Y;
+    if (colour_at(c, x, y, z) != k) return;
+    T s = 0;
+    for (int j = 0; j < st.n; ++j) {
+        const int nx = x + st.off[j][0], ny = y + st.off[j][1], nz = z + st.off[j][2];
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= X || ny >= Y || nz >= Z) continue;
+        s += t[(nz * Y + ny) * X + nx];
+    }
+    dst[i] += s;
+}
+
+// element (i, j) of a boundary texture and the cell next to it: X faces are
+// indexed (y, z), Y faces (x, z), Z faces (x, y)
+__device__ __forceinline__ void bnd_cell(int face, int X, int Y, int Z, int a, int b, int* x, int* y, int* z) {
+    switch (face) {
+    case UWS_X0: *x = 0; *y = a; *z = b; break;
+    case UWS_XN: *x = X - 1; *y = a; *z = b; break;
+    case UWS_Y0: *x = a; *y = 0; *z = b; break;
+    case UWS_YN: *x = a; *y = Y - 1; *z = b; break;
+    case UWS_Z0: *x = a; *y = b; *z = 0; break;
+    default: *x = a; *y = b; *z = Z - 1; break;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(BLK) void k_adj_gather_bnd(const T* t, int X, int Y, int Z, int face, int W, int H,
+                                                       Stencil st, Colour c, int k, T* dst) {
+    const int e = gid();
+    if (e >= W * H) return;
+    const int a = e % W, b = e / W;
+    if (colour_at(c, a, b, 0) != k) return;
+    int x, y, z;
+    bnd_cell(face, X, Y, Z, a, b, &x, &y, &z);
+    T s = 0;
+    for (int j = 0; j < st.n; ++j) {
+        const int nx = x + st.off[j][0], ny = y + st.off[j][1], nz = z + st.off[j][2];
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= X || ny >= Y || nz >= Z) continue;
+        s += t[(nz * Y + ny) * X + nx];
+    }
+    dst[e] += s;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLK) void k_adj_add_fluid(const uint8_t* flag, const T* a, const T* b, T* out, int n) {
+    const int i = gid();
+    if (i >= n) return;
+    T v = a ? a[i] : (T)0;
+    if (b && (!flag || flag[i] == UWS_CELL_FLUID)) v += b[i];
+    out[i] = v;
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_adj_sum(const T* t, int n, double* acc) {
+    __shared__ double part[1024];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += (double)t[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += part[0];
+}
+
+}  // namespace
+
+Stencil make_stencil(int kind) {
+    Stencil s{};
+    s.n = 0;
+    for (int z = -2; z <= 2; ++z)
+        for (int y = -2; y <= 2; ++y)
+            for (int x = -2; x <= 2; ++x) {
+                const int l1 = (x < 0 ? -x : x) + (y < 0 ? -y : y) + (z < 0 ? -z : z);
+                const int axes = (x != 0) + (y != 0) + (z != 0);
+                bool on = false;
+                if (kind == ST_OWN) on = l1 == 0;
+                else if (kind == ST_STAR7) on = l1 <= 1;
+                else if (kind == ST_DIAMOND25) on = l1 <= 2;
+                else on = axes <= 1 && l1 <= 2;  // ST_AXIS13
+                if (!on) continue;
+                s.off[s.n][0] = (int8_t)x;
+                s.off[s.n][1] = (int8_t)y;
+                s.off[s.n][2] = (int8_t)z;
+                ++s.n;
+            }
+    return s;
+}
+
+// Linear colourings x + wy*y + wz*z (mod m) under which any two distinct
+// cells whose stencils overlap (difference in S - S) get different colours;
+// m is the smallest modulus with that property for the set (found by
+// exhaustive search over m, wy, wz): 7 is the perfect Lee code of the
+// 7-point star.
+Colour colour_of(int kind) {
+    switch (kind) {
+    case ST_OWN: return Colour{1, 0, 0};
+    case ST_STAR7: return Colour{7, 2, 3};
+    case ST_DIAMOND25: return Colour{27, 4, 10};
+    default: return Colour{13, 3, 4};  // ST_AXIS13
+    }
+}
+// boundary elements reach the cells of a stencil around their boundary cell;
+// for the 7-point star two elements interfere within L1 distance 2 in the
+// texture plane: (i + 2j) mod 5 separates them
+Colour colour_bnd(int kind) { return kind == ST_OWN ? Colour{1, 0, 0} : Colour{5, 2, 0}; }
+
+template <class T>
+void adj_vel_sweep(const AdjVelSweep<T>& a, int N, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_vel_sweep<T>, dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+}
+template <class T>
+void adj_pres_sweep(const AdjPresSweep<T>& a, int N, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_pres_sweep<T>, dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+}
+template <class T>
+void dual_pack(const T* src, Dual<T>* dst, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_dual_pack<T>, dim3(blocks(n)), dim3(BLK), 0, s, src, dst, n);
+}
+template <class T>
+void dual_seed(Dual<T>* dst, int W, int H, int D, Colour c, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_dual_seed<T>, dim3(blocks(W * H * D)), dim3(BLK), 0, s, dst, W, H, D, c, k);
+}
+template <class T>
+void dual_contract(const Pairs<T>& pr, T* t, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_dual_contract<T>, dim3(blocks(n)), dim3(BLK), 0, s, pr, t, n);
+}
+template <class T>
+void adj_gather(const T* t, int X, int Y, int Z, const Stencil& st, Colour c, int k, T* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_gather<T>, dim3(blocks(X * Y * Z)), dim3(BLK), 0, s, t, X, Y, Z, st, c, k, dst);
+}
+template <class T>
+void adj_gather_bnd(const T* t, int X, int Y, int Z, int face, const Stencil& st, Colour c, int k, T* dst,
+                    hipStream_t s) {
+    const int W = face < 2 ? Y : X, H = face < 4 ? Z : Y;
+    hipLaunchKernelGGL(k_adj_gather_bnd<T>, dim3(blocks(W * H)), dim3(BLK), 0, s, t, X, Y, Z, face, W, H, st, c, k,
+                       dst);
+}
+template <class T>
+void adj_add_fluid(const uint8_t* flag, const T* a, const T* b, T* out, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_add_fluid<T>, dim3(blocks(n)), dim3(BLK), 0, s, flag, a, b, out, n);
+}
+template <class T>
+void adj_sum(const T* t, int n, double* acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_sum<T>, dim3(1), dim3(1024), 0, s, t, n, acc);
+}
+
+#define UWS_ADJ_INST(T)                                                                                         \
+    template void adj_vel_sweep<T>(const AdjVelSweep<T>&, int, hipStream_t);                                    \
+    template void adj_pres_sweep<T>(const AdjPresSweep<T>&, int, hipStream_t);                                  \
+    template void dual_pack<T>(const T*, Dual<T>*, int, hipStream_t);                                           \
+    template void dual_seed<T>(Dual<T>*, int, int, int, Colour, int, hipStream_t);                              \
+    template void dual_contract<T>(const Pairs<T>&, T*, int, hipStream_t);                                      \
+    template void adj_gather<T>(const T*, int, int, int, const Stencil&, Colour, int, T*, hipStream_t);         \
+    template void adj_gather_bnd<T>(const T*, int, int, int, int, const Stencil&, Colour, int, T*, hipStream_t); \
+    template void adj_add_fluid<T>(const uint8_t*, const T*, const T*, T*, int, hipStream_t);                   \
+    template void adj_sum<T>(const T*, int, double*, hipStream_t);
+UWS_ADJ_INST(float)
+UWS_ADJ_INST(double)
+#undef UWS_ADJ_INST
+
+}  // namespace uws

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "uws_host.h"
+#include "uws_solver.h"
 
 using namespace uws;
 
@@ -39,8 +40,8 @@ int validate(const uws_config* c, bool allow_residual_break) {
     if (c->wall_slots != 1 && c->wall_slots != 3) return UWS_ERR_CONFIG;
     if (c->wall_model != UWS_WALL_SMOOTH && c->wall_model != UWS_WALL_ROUGH) return UWS_ERR_CONFIG;
     if (c->wall_model == UWS_WALL_ROUGH && !(c->wall_z0 > 0.f)) return UWS_ERR_CONFIG;
-    if (c->precision < UWS_PREC_FP32 || c->precision > UWS_PREC_FP32_BF16) return UWS_ERR_CONFIG;
-    if (c->precision >= UWS_PREC_FP32_FP16) {
+    if (c->precision < UWS_PREC_FP32 || c->precision > UWS_PREC_CSTEP) return UWS_ERR_CONFIG;
+    if (c->precision == UWS_PREC_FP32_FP16 || c->precision == UWS_PREC_FP32_BF16) {
         if (c->refine_rounds < 0 || c->refine_rounds > 8 || c->refine_sweeps < 0) return UWS_ERR_CONFIG;
         if (c->cal_residual) return UWS_ERR_UNSUPPORTED;
     }
@@ -57,7 +58,7 @@ StepCfg step_cfg(const uws_config& c) {
     s.C = c.fvm_solver_type == UWS_PISO ? c.piso_num_correctors : 0;
     s.les = c.turbulence_model == UWS_TURB_SMAGORINSKY;
     s.norms = c.compute_norms != 0;
-    s.lowp = c.precision >= UWS_PREC_FP32_FP16;
+    s.lowp = c.precision == UWS_PREC_FP32_FP16 || c.precision == UWS_PREC_FP32_BF16;
     s.bf16 = c.precision == UWS_PREC_FP32_BF16;
     s.R = c.refine_rounds;
     s.K = c.refine_sweeps;
@@ -104,6 +105,7 @@ void fill_params(const uws_config& c, ParamsT<T>& p) {
 }
 template void fill_params<float>(const uws_config&, ParamsT<float>&);
 template void fill_params<double>(const uws_config&, ParamsT<double>&);
+template void fill_params<CStep>(const uws_config&, ParamsT<CStep>&);
 
 int bnd_w(int X, int Y, int face) { return face < 2 ? Y : X; }
 int bnd_h(int Y, int Z, int face) { return face < 4 ? Z : Y; }
@@ -154,278 +156,13 @@ size_t carve(const ParamsT<T>& p, FieldsT<T>& f, char* base, bool lowp) {
 }
 template size_t carve<float>(const ParamsT<float>&, FieldsT<float>&, char*, bool);
 template size_t carve<double>(const ParamsT<double>&, FieldsT<double>&, char*, bool);
+template size_t carve<CStep>(const ParamsT<CStep>&, FieldsT<CStep>&, char*, bool);
+template size_t carve<Dual<float>>(const ParamsT<Dual<float>>&, FieldsT<Dual<float>>&, char*, bool);
+template size_t carve<Dual<double>>(const ParamsT<Dual<double>>&, FieldsT<Dual<double>>&, char*, bool);
 
 }  // namespace uws
 
 namespace {
-
-#define HIP_OK(expr)                                     \
-    do {                                                 \
-        if ((expr) != hipSuccess) return UWS_ERR_DEVICE; \
-    } while (0)
-
-int check_launch() { return hipGetLastError() == hipSuccess ? UWS_OK : UWS_ERR_DEVICE; }
-
-// Everything a handle owns, for one arithmetic type.
-template <class T>
-struct Solver : SolverBase {
-    ParamsT<T> p{};
-    FieldsT<T> f{};
-    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // one per starting state buffer
-
-    ~Solver() override {
-        for (auto& g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
-    }
-
-    // single-case launcher for the step driver
-    struct L {
-        Solver* s;
-        void operator()(Kernel k, const Roles& r, int slot = 0) { launch<T>(k, s->p, s->f, r, s->stream, slot); }
-        int copy_vel(int dst, int src) {
-            for (int c = 0; c < 3; ++c)
-                if (hipMemcpyAsync(s->f.vel[dst][c], s->f.vel[src][c], sizeof(T) * s->p.N, hipMemcpyDeviceToDevice,
-                                   s->stream) != hipSuccess)
-                    return UWS_ERR_DEVICE;
-            return UWS_OK;
-        }
-        int read_norm(int slot, double* out) {
-            if (hipMemcpyAsync(s->h_norms, s->f.norms + slot, sizeof(double), hipMemcpyDeviceToHost, s->stream) !=
-                    hipSuccess ||
-                hipStreamSynchronize(s->stream) != hipSuccess)
-                return UWS_ERR_DEVICE;
-            *out = s->h_norms[0];
-            return UWS_OK;
-        }
-        double ncells() const { return (double)s->p.N; }
-    };
-
-    int init(const uws_config& c) {
-        fill_params(c, p);
-        slab_bytes = carve(p, f, nullptr, step_cfg(c).lowp);
-        if (hipMalloc(&slab, slab_bytes) != hipSuccess) { slab = nullptr; return UWS_ERR_ALLOC; }
-        carve(p, f, (char*)slab, step_cfg(c).lowp);
-        if (hipMemsetAsync(slab, 0, slab_bytes, stream) != hipSuccess) return UWS_ERR_DEVICE;
-        return sync();
-    }
-
-    int ncells() const override { return p.N; }
-    int dims(int i) const override { return i == 0 ? p.X : (i == 1 ? p.Y : p.Z); }
-    void* device_fields() override { return &f; }
-    void* device_params() override { return &p; }
-
-    // ---- uploads / downloads (host arrays are float or double) ----
-    template <class H>
-    int up(T* dst, const H* src, size_t n) {
-        std::vector<T> tmp(n);
-        for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
-        HIP_OK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
-        return sync();
-    }
-    template <class H>
-    int down(H* dst, const T* src, size_t n) {
-        std::vector<T> tmp(n);
-        HIP_OK(hipMemcpyAsync(tmp.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
-        if (int rc = sync()) return rc;
-        for (size_t i = 0; i < n; ++i) dst[i] = (H)tmp[i];
-        return UWS_OK;
-    }
-    template <class H>
-    int set_vec(T* const dst[3], const H* src, int ncomp, size_t n) {
-        std::vector<H> plane(n);
-        for (int c = 0; c < 3; ++c) {
-            for (size_t i = 0; i < n; ++i) plane[i] = src[i * ncomp + c];
-            if (int rc = up(dst[c], plane.data(), n)) return rc;
-        }
-        return UWS_OK;
-    }
-    template <class H>
-    int get_vec(H* dst, T* const src[3], int ncomp, size_t n) {
-        std::vector<H> plane(n);
-        for (int c = 0; c < 3; ++c) {
-            if (int rc = down(plane.data(), src[c], n)) return rc;
-            for (size_t i = 0; i < n; ++i) dst[i * ncomp + c] = plane[i];
-        }
-        if (ncomp == 4)
-            for (size_t i = 0; i < n; ++i) dst[i * 4 + 3] = 0;
-        return UWS_OK;
-    }
-
-    int set_flags_u8(const uint8_t* fl) override {
-        HIP_OK(hipMemcpyAsync(f.flag, fl, (size_t)p.N, hipMemcpyHostToDevice, stream));
-        launch<T>(KN_ZERO_SOLID_VEL, p, f, Roles{}, stream);
-        if (int rc = check_launch()) return rc;
-        rs.cur = -1;
-        rs.pc_latest = -1;
-        return sync();
-    }
-    int get_flags_u8(uint8_t* fl) override {
-        HIP_OK(hipMemcpyAsync(fl, f.flag, (size_t)p.N, hipMemcpyDeviceToHost, stream));
-        return sync();
-    }
-    int set_velocity_all(const float* src, int ncomp) override {
-        for (int b = 0; b < 3; ++b)
-            if (int rc = set_vec(f.vel[b], src, ncomp, p.N)) return rc;
-        rs.cur = -1;
-        return UWS_OK;
-    }
-    int get_velocity(float* dst, int ncomp) override { return get_vec(dst, f.vel[rs.state], ncomp, p.N); }
-    T* plane_ptr(int which, int comp) {
-        if (which == PLANE_P) return f.p;
-        if (which >= PLANE_VEL0 && which < PLANE_VEL0 + 9) return f.vel[(which - PLANE_VEL0) / 3][(which - PLANE_VEL0) % 3];
-        if (which == PLANE_UCORR) return f.ucorr[comp];
-        if (which >= PLANE_BND0 && which < PLANE_BND0 + 6) return f.bnd[which - PLANE_BND0][comp];
-        return nullptr;
-    }
-    int set_plane(int which, int comp, const float* src, size_t n) override {
-        T* d = plane_ptr(which, comp);
-        return d ? up(d, src, n) : UWS_ERR_INVALID_ARG;
-    }
-    int set_plane_d(int which, int comp, const double* src, size_t n) override {
-        T* d = plane_ptr(which, comp);
-        return d ? up(d, src, n) : UWS_ERR_INVALID_ARG;
-    }
-    int get_plane(int which, int comp, float* dst, size_t n) override {
-        T* d = nullptr;
-        if (which == PLANE_P) d = f.p;
-        else if (which >= PLANE_BND0 && which < PLANE_BND0 + 6) d = f.bnd[which - PLANE_BND0][comp];
-        if (!d) return UWS_ERR_INVALID_ARG;
-        return down(dst, d, n);
-    }
-
-    // resolve a field id to device planes; returns ncomp, or an error
-    int field_planes(int field, T* planes[3], bool for_write) {
-        planes[0] = planes[1] = planes[2] = nullptr;
-        switch (field) {
-        case UWS_FIELD_VEL: for (int c = 0; c < 3; ++c) planes[c] = f.vel[rs.state][c]; return 3;
-        case UWS_FIELD_VEL_WORK:
-            if (rs.cur < 0) return UWS_ERR_INVALID_ARG;
-            for (int c = 0; c < 3; ++c) planes[c] = f.vel[rs.cur][c];
-            return 3;
-        case UWS_FIELD_VEL_CORR: for (int c = 0; c < 3; ++c) planes[c] = f.ucorr[c]; return 3;
-        case UWS_FIELD_WALL_VIS: for (int c = 0; c < 3; ++c) planes[c] = f.nuw[c]; return 3;
-        case UWS_FIELD_B: for (int c = 0; c < 3; ++c) planes[c] = f.b[c]; return 3;
-        case UWS_FIELD_D: for (int c = 0; c < 3; ++c) planes[c] = f.D[c]; return 3;
-        case UWS_FIELD_AODU: for (int c = 0; c < 3; ++c) planes[c] = f.aodu[c]; return 3;
-        case UWS_FIELD_PRES: planes[0] = f.p; return 1;
-        case UWS_FIELD_PRES_CORR:
-            if (for_write) return UWS_ERR_INVALID_ARG;
-            planes[0] = rs.pc_latest >= 0 ? f.pc[rs.pc_latest] : nullptr;  // nullptr = zero
-            return 1;
-        case UWS_FIELD_EDDY_VIS: planes[0] = f.nut; return 1;
-        case UWS_FIELD_DP: planes[0] = f.Dp; return 1;
-        case UWS_FIELD_BP: planes[0] = f.bp; return 1;
-        default: return UWS_ERR_INVALID_ARG;
-        }
-    }
-    template <class H>
-    int get_field_t(int field, H* dst, int64_t n) {
-        T* planes[3];
-        const int nc = field_planes(field, planes, false);
-        if (nc < 0) return nc;
-        if (n != (int64_t)nc * p.N) return UWS_ERR_INVALID_ARG;
-        if (nc == 1) {
-            if (!planes[0]) {
-                for (int64_t i = 0; i < n; ++i) dst[i] = 0;
-                return UWS_OK;
-            }
-            return down(dst, planes[0], p.N);
-        }
-        return get_vec(dst, planes, 3, p.N);
-    }
-    int get_field(int field, float* dst, int64_t n) override { return get_field_t(field, dst, n); }
-    int get_field64(int field, double* dst, int64_t n) override { return get_field_t(field, dst, n); }
-    int set_field(int field, const float* src, int64_t n) override {
-        if (field == UWS_FIELD_VEL) return n == 3LL * p.N ? set_velocity_all(src, 3) : UWS_ERR_INVALID_ARG;
-        T* planes[3];
-        const int nc = field_planes(field, planes, true);
-        if (nc < 0) return nc;
-        if (n != (int64_t)nc * p.N) return UWS_ERR_INVALID_ARG;
-        if (nc == 1) return up(planes[0], src, p.N);
-        return set_vec(planes, src, 3, p.N);
-    }
-    int get_norms(double* out) override {
-        HIP_OK(hipMemcpyAsync(h_norms, f.norms, sizeof(double) * UWS_NORM_COUNT, hipMemcpyDeviceToHost, stream));
-        if (int rc = sync()) return rc;
-        for (int i = 0; i < UWS_NORM_COUNT; ++i) out[i] = h_norms[i];
-        return UWS_OK;
-    }
-
-    // ---- stepping ----
-    int run_stage(int stage) override {
-        L l{this};
-        int rc = uws::run_stage(l, sc, rs, stage);
-        if (rc == UWS_OK) rc = check_launch();
-        if (rc == UWS_OK && stage == UWS_STAGE_FINISH) { step += 1; phys_time += (double)cfg.dt; }
-        return rc;
-    }
-    int enqueue_step() {
-        L l{this};
-        int rc = uws::enqueue_step(l, sc, rs);
-        if (rc == UWS_OK) rc = check_launch();
-        return rc;
-    }
-    int step_graph() {
-        const int s0 = rs.state;
-        if (!graph[s0]) {
-            const RoleState saved = rs;
-            hipGraph_t g = nullptr;
-            HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_step();
-            hipError_t e = hipStreamEndCapture(stream, &g);
-            rs = saved;
-            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-            if (e != hipSuccess) return UWS_ERR_DEVICE;
-            e = hipGraphInstantiate(&graph[s0], g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) { graph[s0] = nullptr; return UWS_ERR_DEVICE; }
-        }
-        HIP_OK(hipGraphLaunch(graph[s0], stream));
-        advance_roles(sc, rs);
-        return UWS_OK;
-    }
-    int step_n(int n) override {
-        if (rs.cur >= 0) return UWS_ERR_INVALID_ARG;  // a staged step is in progress
-        const bool graphable = !sc.cal_residual && std::getenv("UWS_NO_GRAPH") == nullptr;
-        for (int i = 0; i < n; ++i) {
-            int rc = graphable ? step_graph() : enqueue_step();
-            if (rc) return rc;
-            step += 1;
-            phys_time += (double)cfg.dt;
-        }
-        return UWS_OK;
-    }
-    int time_kernel(int kernel, int reps, double* avg_ms) override {
-        if (rs.cur >= 0) return UWS_ERR_INVALID_ARG;
-        hipEvent_t e0, e1;
-        HIP_OK(hipEventCreate(&e0));
-        if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return UWS_ERR_DEVICE; }
-        const int w0 = (rs.state + 1) % 3;
-        auto go = [&](int i) {
-            Roles r{};
-            if (kernel == UWS_KERNEL_VEL_SWEEP) {
-                r.state = rs.state; r.vin = rs.state; r.vout = w0;
-                launch<T>(KN_VEL_SWEEP, p, f, r, stream);
-            } else {
-                r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
-                launch<T>(KN_PRES_SWEEP, p, f, r, stream);
-            }
-        };
-        int rc = UWS_OK;
-        go(0);
-        if (hipEventRecord(e0, stream) != hipSuccess) rc = UWS_ERR_DEVICE;
-        for (int i = 0; i < reps && rc == UWS_OK; ++i) go(i);
-        if (rc == UWS_OK && check_launch()) rc = UWS_ERR_DEVICE;
-        if (rc == UWS_OK && hipEventRecord(e1, stream) != hipSuccess) rc = UWS_ERR_DEVICE;
-        if (rc == UWS_OK && hipEventSynchronize(e1) != hipSuccess) rc = UWS_ERR_DEVICE;
-        float ms = 0.f;
-        if (rc == UWS_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = UWS_ERR_DEVICE;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (rc == UWS_OK) *avg_ms = (double)ms / reps;
-        return rc;
-    }
-};
 
 template <class F>
 int with_solver(uws_handle h, F&& fn) {
@@ -464,6 +201,7 @@ int create_solver(const uws_config& cfg, bool allow_residual_break, hipStream_t 
     if (int rc = validate(&cfg, allow_residual_break)) return rc;
     std::unique_ptr<SolverBase> s;
     if (cfg.precision == UWS_PREC_FP64) s.reset(new (std::nothrow) Solver<double>());
+    else if (cfg.precision == UWS_PREC_CSTEP) s.reset(new (std::nothrow) Solver<CStep>());
     else s.reset(new (std::nothrow) Solver<float>());
     if (!s) return UWS_ERR_ALLOC;
     s->cfg = cfg;
@@ -480,8 +218,9 @@ int create_solver(const uws_config& cfg, bool allow_residual_break, hipStream_t 
         s->h_norms = nullptr;
         return UWS_ERR_ALLOC;
     }
-    int rc = cfg.precision == UWS_PREC_FP64 ? static_cast<Solver<double>*>(s.get())->init(cfg)
-                                            : static_cast<Solver<float>*>(s.get())->init(cfg);
+    int rc = cfg.precision == UWS_PREC_FP64    ? static_cast<Solver<double>*>(s.get())->init(cfg)
+             : cfg.precision == UWS_PREC_CSTEP ? static_cast<Solver<CStep>*>(s.get())->init(cfg)
+                                               : static_cast<Solver<float>*>(s.get())->init(cfg);
     if (rc) return rc;
     *out = std::move(s);
     return UWS_OK;
@@ -553,7 +292,7 @@ int apply_fixed_value_bnd_t(SolverBase& s, int32_t inflow_type, float wind_speed
 
 int apply_fixed_value_bnd(SolverBase& s, int32_t inflow_type, float wind_speed, const float vel_zn[3],
                           float ref_height, float alpha) {
-    if (s.cfg.precision == UWS_PREC_FP64)
+    if (wide_host(s.cfg.precision))
         return apply_fixed_value_bnd_t<double>(s, inflow_type, wind_speed, vel_zn, ref_height, alpha);
     return apply_fixed_value_bnd_t<float>(s, inflow_type, wind_speed, vel_zn, ref_height, alpha);
 }
@@ -615,7 +354,7 @@ int init_vel_pres_t(SolverBase& s, int32_t inflow_type, float wind_speed) {
 }
 
 int init_vel_pres(SolverBase& s, int32_t inflow_type, float wind_speed) {
-    if (s.cfg.precision == UWS_PREC_FP64) return init_vel_pres_t<double>(s, inflow_type, wind_speed);
+    if (wide_host(s.cfg.precision)) return init_vel_pres_t<double>(s, inflow_type, wind_speed);
     return init_vel_pres_t<float>(s, inflow_type, wind_speed);
 }
 
@@ -761,6 +500,33 @@ int uws_set_bnd_field(uws_handle h, int32_t face, const float* xyz, int64_t n) {
         }
         return (int)UWS_OK;
     });
+}
+
+int uws_set_bnd_field_f64(uws_handle h, int32_t face, const double* xyz, int64_t n) {
+    if (!xyz || face < 0 || face > 5) return UWS_ERR_INVALID_ARG;
+    return with_solver(h, [&](SolverBase& s) {
+        if (n != (int64_t)bnd_w(s.dims(0), s.dims(1), face) * bnd_h(s.dims(1), s.dims(2), face))
+            return (int)UWS_ERR_INVALID_ARG;
+        std::vector<double> plane((size_t)n);
+        for (int c = 0; c < 3; ++c) {
+            for (int64_t i = 0; i < n; ++i) plane[i] = xyz[i * 3 + c];
+            if (int rc = s.set_plane_d(PLANE_BND0 + face, c, plane.data(), n)) return rc;
+        }
+        return (int)UWS_OK;
+    });
+}
+
+int uws_set_smagorinsky_constant(uws_handle h, double cs) {
+    if (!std::isfinite(cs) || cs < 0.0) return UWS_ERR_INVALID_ARG;
+    return with_solver(h, [&](SolverBase& s) { return s.set_cs(cs, 0.0); });
+}
+
+int uws_cstep_set_imag(uws_handle h, int32_t what, int32_t face, const double* im, int64_t n) {
+    return with_solver(h, [&](SolverBase& s) { return s.cstep_imag(true, what, face, const_cast<double*>(im), n); });
+}
+
+int uws_cstep_get_imag(uws_handle h, int32_t what, int32_t face, double* im, int64_t n) {
+    return with_solver(h, [&](SolverBase& s) { return s.cstep_imag(false, what, face, im, n); });
 }
 
 int uws_get_bnd_field(uws_handle h, int32_t face, float* xyz, int64_t n) {

@@ -69,7 +69,11 @@ struct SolverBase {
     virtual int run_stage(int stage) = 0;
     virtual int step_n(int n) = 0;
     virtual int time_kernel(int kernel, int reps, double* avg_ms) = 0;
+    virtual int set_cs(double re, double im) = 0;
+    virtual int cstep_imag(bool set, int what, int face, double* buf, int64_t n) = 0;
 };
+// runs whose host-side profiles are evaluated in double
+inline bool wide_host(int precision) { return precision == UWS_PREC_FP64 || precision == UWS_PREC_CSTEP; }
 
 int validate(const uws_config* c, bool allow_residual_break);
 StepCfg step_cfg(const uws_config& c);

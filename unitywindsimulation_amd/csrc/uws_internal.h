@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "../../include/uws.h"
+#include "uws_scalar.h"
 
 namespace uws {
 
@@ -93,7 +94,8 @@ struct Roles {
 };
 enum {
     K_WALL = 1, K_EDDY = 2, K_FUSE_LES = 4, K_LOWP_FIRST = 16, K_LOWP_LAST = 32,
-    K_BF16 = 64
+    K_BF16 = 64,
+    K_NO_INPLACE = 128   // corrections write u' only, not u += u' / p += p'
 };
 
 // kernel identifiers understood by launch()

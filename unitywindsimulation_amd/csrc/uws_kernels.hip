@@ -695,6 +695,7 @@ __device__ void body_vel_correct(const ParamsT<T>& p, const FieldsT<T>& f, const
     const T cy = -dvden * ((T)1.0f / f.D[1][P]) * (pf[N_] - pf[S_]) / p.dx;
     const T cz = -dvden * ((T)1.0f / f.D[2][P]) * (pf[U_] - pf[D_]) / p.dx;
     f.ucorr[0][P] = cx; f.ucorr[1][P] = cy; f.ucorr[2][P] = cz;
+    if (r.flags & K_NO_INPLACE) return;
     T* const* u = f.vel[r.vout];
     u[0][P] += cx; u[1][P] += cy; u[2][P] += cz;
 }
@@ -778,7 +779,8 @@ __device__ void body_piso_correct(const ParamsT<T>& p, const FieldsT<T>& f, cons
     if (!fluid(f, P)) return;
     const T* __restrict__ pc = f.pc[r.pin];
     const T pcP = pc[P];
-    f.p[P] += pcP;
+    const bool inplace = (r.flags & K_NO_INPLACE) == 0;
+    if (inplace) f.p[P] += pcP;
     T pf[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
@@ -792,6 +794,7 @@ __device__ void body_piso_correct(const ParamsT<T>& p, const FieldsT<T>& f, cons
     const T cy = -f.invD[1][P] * (f.aodu[1][P] + dvden * gy);
     const T cz = -f.invD[2][P] * (f.aodu[2][P] + dvden * gz);
     f.ucorr[0][P] = cx; f.ucorr[1][P] = cy; f.ucorr[2][P] = cz;
+    if (!inplace) return;
     T* const* u = f.vel[r.vout];
     u[0][P] += cx; u[1][P] += cy; u[2][P] += cz;
 }
@@ -1139,6 +1142,11 @@ void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, 
 template void launch<float>(Kernel, const ParamsT<float>&, const FieldsT<float>&, const Roles&, hipStream_t, int);
 template void launch<double>(Kernel, const ParamsT<double>&, const FieldsT<double>&, const Roles&, hipStream_t,
                              int);
+template void launch<CStep>(Kernel, const ParamsT<CStep>&, const FieldsT<CStep>&, const Roles&, hipStream_t, int);
+template void launch<Dual<float>>(Kernel, const ParamsT<Dual<float>>&, const FieldsT<Dual<float>>&, const Roles&,
+                                  hipStream_t, int);
+template void launch<Dual<double>>(Kernel, const ParamsT<Dual<double>>&, const FieldsT<Dual<double>>&,
+                                   const Roles&, hipStream_t, int);
 template void launch_batch<float>(Kernel, const BatchTable<float>&, int, const Roles&, hipStream_t, int);
 
 }  // namespace uws

@@ -1,0 +1,351 @@
+// The per-type solver object behind a uws_handle (shared by the C ABI, the
+// ensemble and the adjoint).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <type_traits>
+#include <vector>
+
+#include "uws_host.h"
+
+namespace uws {
+
+#define HIP_OK(expr)                                     \
+    do {                                                 \
+        if ((expr) != hipSuccess) return UWS_ERR_DEVICE; \
+    } while (0)
+
+inline int check_launch() { return hipGetLastError() == hipSuccess ? UWS_OK : UWS_ERR_DEVICE; }
+
+// Everything a handle owns, for one arithmetic type.
+template <class T>
+struct Solver : SolverBase {
+    ParamsT<T> p{};
+    FieldsT<T> f{};
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // one per starting state buffer
+
+    ~Solver() override { drop_graphs(); }
+    void drop_graphs() {
+        for (auto& g : graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+
+    // single-case launcher for the step driver
+    struct L {
+        Solver* s;
+        void operator()(Kernel k, const Roles& r, int slot = 0) { launch<T>(k, s->p, s->f, r, s->stream, slot); }
+        int copy_vel(int dst, int src) {
+            for (int c = 0; c < 3; ++c)
+                if (hipMemcpyAsync(s->f.vel[dst][c], s->f.vel[src][c], sizeof(T) * s->p.N, hipMemcpyDeviceToDevice,
+                                   s->stream) != hipSuccess)
+                    return UWS_ERR_DEVICE;
+            return UWS_OK;
+        }
+        int read_norm(int slot, double* out) {
+            if (hipMemcpyAsync(s->h_norms, s->f.norms + slot, sizeof(double), hipMemcpyDeviceToHost, s->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s->stream) != hipSuccess)
+                return UWS_ERR_DEVICE;
+            *out = s->h_norms[0];
+            return UWS_OK;
+        }
+        double ncells() const { return (double)s->p.N; }
+    };
+
+    int init(const uws_config& c) {
+        fill_params(c, p);
+        slab_bytes = carve(p, f, nullptr, step_cfg(c).lowp);
+        if (hipMalloc(&slab, slab_bytes) != hipSuccess) { slab = nullptr; return UWS_ERR_ALLOC; }
+        carve(p, f, (char*)slab, step_cfg(c).lowp);
+        if (hipMemsetAsync(slab, 0, slab_bytes, stream) != hipSuccess) return UWS_ERR_DEVICE;
+        return sync();
+    }
+
+    int ncells() const override { return p.N; }
+    int dims(int i) const override { return i == 0 ? p.X : (i == 1 ? p.Y : p.Z); }
+    void* device_fields() override { return &f; }
+    void* device_params() override { return &p; }
+
+    // ---- uploads / downloads (host arrays are float or double) ----
+    template <class H>
+    int up(T* dst, const H* src, size_t n) {
+        std::vector<T> tmp(n);
+        for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
+        HIP_OK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
+        return sync();
+    }
+    template <class H>
+    int down(H* dst, const T* src, size_t n) {
+        std::vector<T> tmp(n);
+        HIP_OK(hipMemcpyAsync(tmp.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+        if (int rc = sync()) return rc;
+        for (size_t i = 0; i < n; ++i) dst[i] = (H)tmp[i];
+        return UWS_OK;
+    }
+    template <class H>
+    int set_vec(T* const dst[3], const H* src, int ncomp, size_t n) {
+        std::vector<H> plane(n);
+        for (int c = 0; c < 3; ++c) {
+            for (size_t i = 0; i < n; ++i) plane[i] = src[i * ncomp + c];
+            if (int rc = up(dst[c], plane.data(), n)) return rc;
+        }
+        return UWS_OK;
+    }
+    template <class H>
+    int get_vec(H* dst, T* const src[3], int ncomp, size_t n) {
+        std::vector<H> plane(n);
+        for (int c = 0; c < 3; ++c) {
+            if (int rc = down(plane.data(), src[c], n)) return rc;
+            for (size_t i = 0; i < n; ++i) dst[i * ncomp + c] = plane[i];
+        }
+        if (ncomp == 4)
+            for (size_t i = 0; i < n; ++i) dst[i * 4 + 3] = 0;
+        return UWS_OK;
+    }
+
+    int set_flags_u8(const uint8_t* fl) override {
+        HIP_OK(hipMemcpyAsync(f.flag, fl, (size_t)p.N, hipMemcpyHostToDevice, stream));
+        launch<T>(KN_ZERO_SOLID_VEL, p, f, Roles{}, stream);
+        if (int rc = check_launch()) return rc;
+        rs.cur = -1;
+        rs.pc_latest = -1;
+        return sync();
+    }
+    int get_flags_u8(uint8_t* fl) override {
+        HIP_OK(hipMemcpyAsync(fl, f.flag, (size_t)p.N, hipMemcpyDeviceToHost, stream));
+        return sync();
+    }
+    int set_velocity_all(const float* src, int ncomp) override {
+        for (int b = 0; b < 3; ++b)
+            if (int rc = set_vec(f.vel[b], src, ncomp, p.N)) return rc;
+        rs.cur = -1;
+        return UWS_OK;
+    }
+    int get_velocity(float* dst, int ncomp) override { return get_vec(dst, f.vel[rs.state], ncomp, p.N); }
+    T* plane_ptr(int which, int comp) {
+        if (which == PLANE_P) return f.p;
+        if (which >= PLANE_VEL0 && which < PLANE_VEL0 + 9) return f.vel[(which - PLANE_VEL0) / 3][(which - PLANE_VEL0) % 3];
+        if (which == PLANE_UCORR) return f.ucorr[comp];
+        if (which >= PLANE_BND0 && which < PLANE_BND0 + 6) return f.bnd[which - PLANE_BND0][comp];
+        return nullptr;
+    }
+    int set_plane(int which, int comp, const float* src, size_t n) override {
+        T* d = plane_ptr(which, comp);
+        return d ? up(d, src, n) : UWS_ERR_INVALID_ARG;
+    }
+    int set_plane_d(int which, int comp, const double* src, size_t n) override {
+        T* d = plane_ptr(which, comp);
+        return d ? up(d, src, n) : UWS_ERR_INVALID_ARG;
+    }
+    int get_plane(int which, int comp, float* dst, size_t n) override {
+        T* d = nullptr;
+        if (which == PLANE_P) d = f.p;
+        else if (which >= PLANE_BND0 && which < PLANE_BND0 + 6) d = f.bnd[which - PLANE_BND0][comp];
+        if (!d) return UWS_ERR_INVALID_ARG;
+        return down(dst, d, n);
+    }
+
+    // resolve a field id to device planes; returns ncomp, or an error
+    int field_planes(int field, T* planes[3], bool for_write) {
+        planes[0] = planes[1] = planes[2] = nullptr;
+        switch (field) {
+        case UWS_FIELD_VEL: for (int c = 0; c < 3; ++c) planes[c] = f.vel[rs.state][c]; return 3;
+        case UWS_FIELD_VEL_WORK:
+            if (rs.cur < 0) return UWS_ERR_INVALID_ARG;
+            for (int c = 0; c < 3; ++c) planes[c] = f.vel[rs.cur][c];
+            return 3;
+        case UWS_FIELD_VEL_CORR: for (int c = 0; c < 3; ++c) planes[c] = f.ucorr[c]; return 3;
+        case UWS_FIELD_WALL_VIS: for (int c = 0; c < 3; ++c) planes[c] = f.nuw[c]; return 3;
+        case UWS_FIELD_B: for (int c = 0; c < 3; ++c) planes[c] = f.b[c]; return 3;
+        case UWS_FIELD_D: for (int c = 0; c < 3; ++c) planes[c] = f.D[c]; return 3;
+        case UWS_FIELD_AODU: for (int c = 0; c < 3; ++c) planes[c] = f.aodu[c]; return 3;
+        case UWS_FIELD_PRES: planes[0] = f.p; return 1;
+        case UWS_FIELD_PRES_CORR:
+            if (for_write) return UWS_ERR_INVALID_ARG;
+            planes[0] = rs.pc_latest >= 0 ? f.pc[rs.pc_latest] : nullptr;  // nullptr = zero
+            return 1;
+        case UWS_FIELD_EDDY_VIS: planes[0] = f.nut; return 1;
+        case UWS_FIELD_DP: planes[0] = f.Dp; return 1;
+        case UWS_FIELD_BP: planes[0] = f.bp; return 1;
+        default: return UWS_ERR_INVALID_ARG;
+        }
+    }
+    template <class H>
+    int get_field_t(int field, H* dst, int64_t n) {
+        T* planes[3];
+        const int nc = field_planes(field, planes, false);
+        if (nc < 0) return nc;
+        if (n != (int64_t)nc * p.N) return UWS_ERR_INVALID_ARG;
+        if (nc == 1) {
+            if (!planes[0]) {
+                for (int64_t i = 0; i < n; ++i) dst[i] = 0;
+                return UWS_OK;
+            }
+            return down(dst, planes[0], p.N);
+        }
+        return get_vec(dst, planes, 3, p.N);
+    }
+    int get_field(int field, float* dst, int64_t n) override { return get_field_t(field, dst, n); }
+    int get_field64(int field, double* dst, int64_t n) override { return get_field_t(field, dst, n); }
+    int set_field(int field, const float* src, int64_t n) override {
+        if (field == UWS_FIELD_VEL) return n == 3LL * p.N ? set_velocity_all(src, 3) : UWS_ERR_INVALID_ARG;
+        T* planes[3];
+        const int nc = field_planes(field, planes, true);
+        if (nc < 0) return nc;
+        if (n != (int64_t)nc * p.N) return UWS_ERR_INVALID_ARG;
+        if (nc == 1) return up(planes[0], src, p.N);
+        return set_vec(planes, src, 3, p.N);
+    }
+    int get_norms(double* out) override {
+        HIP_OK(hipMemcpyAsync(h_norms, f.norms, sizeof(double) * UWS_NORM_COUNT, hipMemcpyDeviceToHost, stream));
+        if (int rc = sync()) return rc;
+        for (int i = 0; i < UWS_NORM_COUNT; ++i) out[i] = h_norms[i];
+        return UWS_OK;
+    }
+
+    // ---- differentiation support ----
+    // a captured step holds the parameters by value: drop the graphs
+    int set_cs(double re, double im) override {
+        if constexpr (std::is_same<T, CStep>::value) {
+            p.cs = CStep(re, im);
+        } else {
+            if (im != 0.0) return UWS_ERR_UNSUPPORTED;
+            p.cs = (T)re;
+        }
+        cfg.smagorinsky_constant = (float)re;
+        if (int rc = sync()) return rc;
+        drop_graphs();
+        return UWS_OK;
+    }
+    int cstep_imag(bool set, int what, int face, double* buf, int64_t n) override {
+        if constexpr (!std::is_same<T, CStep>::value) {
+            (void)set; (void)what; (void)face; (void)buf; (void)n;
+            return UWS_ERR_UNSUPPORTED;
+        } else {
+            if (!buf) return UWS_ERR_INVALID_ARG;
+            if (what == UWS_CSTEP_CS) {
+                if (n != 1) return UWS_ERR_INVALID_ARG;
+                if (set) return set_cs(p.cs.re, buf[0]);
+                buf[0] = p.cs.im;
+                return UWS_OK;
+            }
+            std::vector<T*> planes;  // [buffer][comp]
+            int nc = 0;
+            size_t np = 0;
+            if (what == UWS_CSTEP_VEL) {
+                nc = 3; np = (size_t)p.N;
+                for (int b = 0; b < 3; ++b) {
+                    if (!set && b != rs.state) continue;
+                    for (int c = 0; c < 3; ++c) planes.push_back(f.vel[b][c]);
+                }
+            } else if (what == UWS_CSTEP_PRES) {
+                nc = 1; np = (size_t)p.N;
+                planes.push_back(f.p);
+            } else if (what == UWS_CSTEP_BND) {
+                if (face < 0 || face > 5) return UWS_ERR_INVALID_ARG;
+                nc = 3; np = (size_t)bnd_w(p.X, p.Y, face) * bnd_h(p.Y, p.Z, face);
+                for (int c = 0; c < 3; ++c) planes.push_back(f.bnd[face][c]);
+            } else {
+                return UWS_ERR_INVALID_ARG;
+            }
+            if (n != (int64_t)(nc * np)) return UWS_ERR_INVALID_ARG;
+            std::vector<CStep> tmp(np);
+            for (size_t k = 0; k < planes.size(); ++k) {
+                const int c = (int)(k % nc);
+                HIP_OK(hipMemcpyAsync(tmp.data(), planes[k], np * sizeof(CStep), hipMemcpyDeviceToHost, stream));
+                if (int rc = sync()) return rc;
+                if (set) {
+                    for (size_t i = 0; i < np; ++i) tmp[i].im = buf[i * nc + c];
+                    HIP_OK(hipMemcpyAsync(planes[k], tmp.data(), np * sizeof(CStep), hipMemcpyHostToDevice, stream));
+                    if (int rc = sync()) return rc;
+                } else {
+                    for (size_t i = 0; i < np; ++i) buf[i * nc + c] = tmp[i].im;
+                }
+            }
+            return UWS_OK;
+        }
+    }
+
+    // ---- stepping ----
+    int run_stage(int stage) override {
+        L l{this};
+        int rc = uws::run_stage(l, sc, rs, stage);
+        if (rc == UWS_OK) rc = check_launch();
+        if (rc == UWS_OK && stage == UWS_STAGE_FINISH) { step += 1; phys_time += (double)cfg.dt; }
+        return rc;
+    }
+    int enqueue_step() {
+        L l{this};
+        int rc = uws::enqueue_step(l, sc, rs);
+        if (rc == UWS_OK) rc = check_launch();
+        return rc;
+    }
+    int step_graph() {
+        const int s0 = rs.state;
+        if (!graph[s0]) {
+            const RoleState saved = rs;
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_step();
+            hipError_t e = hipStreamEndCapture(stream, &g);
+            rs = saved;
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return UWS_ERR_DEVICE;
+            e = hipGraphInstantiate(&graph[s0], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { graph[s0] = nullptr; return UWS_ERR_DEVICE; }
+        }
+        HIP_OK(hipGraphLaunch(graph[s0], stream));
+        advance_roles(sc, rs);
+        return UWS_OK;
+    }
+    int step_n(int n) override {
+        if (rs.cur >= 0) return UWS_ERR_INVALID_ARG;  // a staged step is in progress
+        const bool graphable = !sc.cal_residual && std::getenv("UWS_NO_GRAPH") == nullptr;
+        for (int i = 0; i < n; ++i) {
+            int rc = graphable ? step_graph() : enqueue_step();
+            if (rc) return rc;
+            step += 1;
+            phys_time += (double)cfg.dt;
+        }
+        return UWS_OK;
+    }
+    int time_kernel(int kernel, int reps, double* avg_ms) override {
+        if (rs.cur >= 0) return UWS_ERR_INVALID_ARG;
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return UWS_ERR_DEVICE; }
+        const int w0 = (rs.state + 1) % 3;
+        auto go = [&](int i) {
+            Roles r{};
+            if (kernel == UWS_KERNEL_VEL_SWEEP) {
+                r.state = rs.state; r.vin = rs.state; r.vout = w0;
+                launch<T>(KN_VEL_SWEEP, p, f, r, stream);
+            } else {
+                r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
+                launch<T>(KN_PRES_SWEEP, p, f, r, stream);
+            }
+        };
+        int rc = UWS_OK;
+        go(0);
+        if (hipEventRecord(e0, stream) != hipSuccess) rc = UWS_ERR_DEVICE;
+        for (int i = 0; i < reps && rc == UWS_OK; ++i) go(i);
+        if (rc == UWS_OK && check_launch()) rc = UWS_ERR_DEVICE;
+        if (rc == UWS_OK && hipEventRecord(e1, stream) != hipSuccess) rc = UWS_ERR_DEVICE;
+        if (rc == UWS_OK && hipEventSynchronize(e1) != hipSuccess) rc = UWS_ERR_DEVICE;
+        float ms = 0.f;
+        if (rc == UWS_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = UWS_ERR_DEVICE;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc == UWS_OK) *avg_ms = (double)ms / reps;
+        return rc;
+    }
+};
+
+
+}  // namespace uws

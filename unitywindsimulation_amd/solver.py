@@ -194,6 +194,41 @@ class FvmSolverHip(IFvmSolver):
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float32).reshape(-1, 3))
         N.check(self._lib.uws_set_bnd_field(self._h, face, N.ptr(a), a.shape[0]), "uws_set_bnd_field")
 
+    def bnd_size(self, face: int) -> int:
+        X, Y, Z = self.cf.gridRes
+        return (Y if face < 2 else X) * (Z if face < 4 else Y)
+
+    def set_bnd_field64(self, face: int, xyz) -> None:
+        """Boundary texture upload in double (exact for FP64 and complex-step runs)."""
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+        N.check(self._lib.uws_set_bnd_field_f64(self._h, face, N.ptr(a), a.shape[0]), "uws_set_bnd_field_f64")
+
+    def set_smagorinsky_constant(self, cs: float) -> None:
+        """smagorinskyConstant of the live solver, used from the next step on."""
+        N.check(self._lib.uws_set_smagorinsky_constant(self._h, float(cs)), "uws_set_smagorinsky_constant")
+        self.cf.smagorinskyConstant = float(cs)
+
+    def _cstep_len(self, what: int, face: int) -> int:
+        if what == N.CSTEP_VEL:
+            return 3 * self.n
+        if what == N.CSTEP_PRES:
+            return self.n
+        if what == N.CSTEP_BND:
+            return 3 * self.bnd_size(face)
+        return 1
+
+    def cstep_set_imag(self, what: int, im, face: int = 0) -> None:
+        """Imaginary parts of a complex-step run (precision PREC_CSTEP)."""
+        a = np.ascontiguousarray(np.asarray(im, dtype=np.float64).reshape(-1))
+        if a.size != self._cstep_len(what, face):
+            raise ValueError(f"expected {self._cstep_len(what, face)} values, got {a.size}")
+        N.check(self._lib.uws_cstep_set_imag(self._h, what, face, N.ptr(a), a.size), "uws_cstep_set_imag")
+
+    def cstep_get_imag(self, what: int, face: int = 0) -> np.ndarray:
+        out = np.empty(self._cstep_len(what, face), dtype=np.float64)
+        N.check(self._lib.uws_cstep_get_imag(self._h, what, face, N.ptr(out), out.size), "uws_cstep_get_imag")
+        return out.reshape(-1, 3) if what in (N.CSTEP_VEL, N.CSTEP_BND) else out
+
     def run_stage(self, stage: int) -> None:
         N.check(self._lib.uws_run_stage(self._h, int(stage)), f"uws_run_stage({stage})")
 

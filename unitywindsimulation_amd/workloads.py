@@ -138,6 +138,47 @@ def ensemble(n_cases=24, seed=2, grids=ENSEMBLE_GRIDS, buildings=(5, 30), steps=
     return out
 
 
+@dataclass
+class DiffWorkload:
+    """The differentiation case: a workload plus its objective and parameters."""
+    base: Workload
+    nsteps: int
+    ckpt_every: int
+    region: tuple          # (x0, z0, width, depth) of the objective's region
+    delta: np.ndarray      # per-height-level inlet perturbations [m/s]
+    u_ref: float = U_REF
+    alpha: float = ALPHA
+    cs: float = 0.1
+
+
+def normal_pairs(rng: SplitMix64, n: int) -> np.ndarray:
+    """n standard normal draws from SplitMix64 uniforms (Box-Muller, cosine branch)."""
+    out = np.empty(n)
+    for i in range(n):
+        u1 = max(rng.uniform(), 1.0 / 16777216.0)
+        u2 = rng.uniform()
+        out[i] = math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+    return out
+
+
+def differentiation(grid=(128, 64, 96), n=20, lattice=(5, 4), seed=4, nsteps=64, ckpt_every=8,
+                    region_size=32) -> DiffWorkload:
+    """128 x 96 span x 64 high with n seeded buildings; J = mean |u|^2 on the
+    2 m layer (average of the cell layers j = 1, 2) over a seeded
+    region_size^2 (x, z) region; parameters U_ref, alpha, Cs and one inlet
+    perturbation per height level, N(0, 0.05 U_ref)."""
+    boxes = lattice_layout(grid, n, lattice, seed, foot=(4, 12), height=(4, 24))
+    cf = base_config(grid)
+    X, Y, Z = grid
+    w = Workload(f"differentiation_{n}bldg_{X}x{Y}x{Z}", cf, rasterise(grid, boxes), noise_seed=seed,
+                 steps=nsteps, boxes=boxes)
+    rng = SplitMix64(seed ^ 0xD1FF)
+    rs = min(region_size, X, Z)
+    region = (rng.randint(0, X - rs), rng.randint(0, Z - rs), rs, rs)
+    delta = 0.05 * U_REF * normal_pairs(rng, Y)
+    return DiffWorkload(w, nsteps, ckpt_every, region, delta)
+
+
 WORKLOADS = {
     "isolated_building": isolated_building,
     "urban": urban,
