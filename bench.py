@@ -118,6 +118,121 @@ def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
             "max_rel_l2_u_vs_fp64": max(errs), "compared_after_steps": check_steps}
 
 
+def adjoint_report(ndir: int = 16, h_fd=(1e-3, 1e-6)) -> dict:
+    """The differentiation case: 64 steps with a checkpoint every 8, J = mean
+    |u|^2 on the 2 m layer (average of cell layers 1 and 2) of a seeded 32 x 32
+    region; gradient w.r.t. U_ref, alpha, Cs and the 64 per-level inlet
+    perturbations through the torch autograd function on FP32 and FP64
+    handles.  Checked against FP64 complex step (Im J(theta + i h v) / h,
+    h = 1e-20) along the three unit directions of U_ref, alpha, Cs and ndir
+    random unit directions, and against FP64 central differences along the
+    same ndir random directions for each step h in h_fd (the truncation error
+    of the 64-step objective is O(h^2) with a large constant: h = 1e-3 is
+    reported as specified, the smaller h shows the convergence)."""
+    import copy
+
+    import numpy as np
+    import torch
+
+    from unitywindsimulation_amd import FvmSolverHip, workloads
+    from unitywindsimulation_amd import _native as N
+    from unitywindsimulation_amd.adjoint import FvmAdjoint, fvm_unroll, layer_objective, power_law_bnd
+
+    d = workloads.differentiation()
+    w = d.base
+    grid = w.grid
+    X, Y, Z = grid
+    theta0 = np.concatenate([[d.u_ref, d.alpha, d.cs], d.delta])
+    x0, z0, rw, rd = d.region
+    yc = (np.arange(Y) + 0.5) * float(w.cf.dx) / float(w.cf.inflowRefHeight)
+
+    def solver(prec, theta=theta0):
+        cf = copy.deepcopy(w.cf)
+        cf.precision = prec
+        cf.Init()
+        s = FvmSolverHip(cf)
+        s.InitFlags(w.flags)
+        s.init_noise(w.noise_seed, w.noise_amp)
+        s.set_bnd_field64(N.X0, power_law_bnd(grid, theta[0], theta[1], theta[3:]))
+        s.set_smagorinsky_constant(theta[2])
+        return s
+
+    out = {"workload": w.name, "grid_xyz": [X, Y, Z], "cells": w.ncells, "buildings": len(w.boxes),
+           "steps": d.nsteps, "checkpoint_every": d.ckpt_every, "region_x0_z0_w_d": list(d.region),
+           "objective": "mean |u|^2 over the region at y = 2 m (cell layers 1, 2 averaged)",
+           "parameters": "U_ref, alpha, Cs, 64 per-level inlet perturbations"}
+    grads = {}
+    for prec, name in ((N.PREC_FP32, "fp32"), (N.PREC_FP64, "fp64")):
+        with solver(prec) as s, FvmAdjoint(s, d.nsteps, d.ckpt_every) as adj:
+            th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+            bnd = power_law_bnd(grid, th[0], th[1], th[3:], xp=torch)
+            t0 = time.perf_counter()
+            u, _ = fvm_unroll(adj, bnd, th[2])
+            t1 = time.perf_counter()
+            uu = u.reshape(Z, Y, X, 3)
+            ub = 0.5 * (uu[z0:z0 + rd, 1, x0:x0 + rw] + uu[z0:z0 + rd, 2, x0:x0 + rw])
+            J = (ub * ub).sum() / (rw * rd)
+            J.backward()
+            t2 = time.perf_counter()
+            grads[name] = th.grad.numpy().copy()
+            out[name] = {"J": J.item(), "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+                         "device_GiB": round(adj.memory_bytes / 2 ** 30, 3),
+                         "dJ_dUref": float(grads[name][0]), "dJ_dalpha": float(grads[name][1]),
+                         "dJ_dCs": float(grads[name][2])}
+        print(f"[adjoint] {name}: J={out[name]['J']:.6e} forward {out[name]['forward_s']} s "
+              f"backward {out[name]['backward_s']} s", file=sys.stderr, flush=True)
+
+    def dbnd(v):
+        col = v[0] * yc ** theta0[1] + theta0[0] * yc ** theta0[1] * np.log(yc) * v[1] + v[3:]
+        b = np.zeros((Z * Y, 3))
+        b[:, 0] = np.tile(col, Z)
+        return b
+
+    H = 1e-20
+
+    def cstep(v):
+        with solver(N.PREC_CSTEP) as s:
+            s.cstep_set_imag(N.CSTEP_BND, H * dbnd(v), N.X0)
+            s.cstep_set_imag(N.CSTEP_CS, np.array([H * v[2]]))
+            s.step(d.nsteps)
+            re = s.field64(N.FIELD_VEL).reshape(Z, Y, X, 3)
+            im = s.cstep_get_imag(N.CSTEP_VEL).reshape(Z, Y, X, 3)
+        ubr = 0.5 * (re[z0:z0 + rd, 1, x0:x0 + rw] + re[z0:z0 + rd, 2, x0:x0 + rw])
+        ubi = 0.5 * (im[z0:z0 + rd, 1, x0:x0 + rw] + im[z0:z0 + rd, 2, x0:x0 + rw])
+        return float(np.sum(2.0 * ubr * ubi) / (rw * rd) / H)
+
+    def J64(theta):
+        with solver(N.PREC_FP64, theta) as s:
+            s.step(d.nsteps)
+            return layer_objective(s.field64(N.FIELD_VEL), grid, d.region)[0]
+
+    rng = np.random.default_rng(4)
+    rand = [v / np.linalg.norm(v) for v in rng.standard_normal((ndir, theta0.size))]
+    dirs = [np.eye(theta0.size)[i] for i in range(3)] + rand
+    t0 = time.perf_counter()
+    ref = np.array([cstep(v) for v in dirs])
+    t_cs = time.perf_counter() - t0
+    scale = float(np.sqrt(np.mean(ref ** 2)))
+    for name in ("fp64", "fp32"):
+        ad = np.array([grads[name] @ v for v in dirs])
+        out[name]["vs_complex_step_max_rel"] = float(np.max(np.abs(ad - ref) / np.abs(ref)))
+        out[name]["vs_complex_step_max_abs_over_rms"] = float(np.max(np.abs(ad - ref)) / scale)
+    out["complex_step"] = {"directions": len(dirs), "seconds": round(t_cs, 2),
+                           "unit_UrefAlphaCs": [float(x) for x in ref[:3]]}
+    out["central_fd"] = []
+    for h in h_fd:
+        t0 = time.perf_counter()
+        fd = np.array([(J64(theta0 + h * v) - J64(theta0 - h * v)) / (2 * h) for v in rand])
+        t_fd = time.perf_counter() - t0
+        rec = {"h": h, "directions": len(rand), "seconds": round(t_fd, 2),
+               "fd_vs_complex_step_max_rel": float(np.max(np.abs(fd - ref[3:]) / np.abs(ref[3:])))}
+        for name in ("fp64", "fp32"):
+            ad = np.array([grads[name] @ v for v in rand])
+            rec[f"{name}_adjoint_vs_fd_max_rel"] = float(np.max(np.abs(ad - fd) / np.abs(fd)))
+        out["central_fd"].append(rec)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +242,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 24-case ensemble leg")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiation leg (adjoint vs complex step and central differences)")
+    ap.add_argument("--adjoint-dirs", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM counters)")
     args = ap.parse_args()
@@ -209,6 +327,8 @@ def main() -> int:
     }
     if not args.no_ensemble:
         out["ensemble"] = ensemble_report()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_report(args.adjoint_dirs)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(out))

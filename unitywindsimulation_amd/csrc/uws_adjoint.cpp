@@ -57,7 +57,9 @@ ParamsT<Dual<T>> dual_params(const ParamsT<T>& p) {
 
 template <class T>
 struct Adjoint : AdjointBase {
+    using A = double;  // adjoint arithmetic (the records stay in T)
     using V3p = std::array<T*, 3>;
+    using V3a = std::array<A*, 3>;
     Solver<T>* s = nullptr;
     int nsteps = 0, K = 1, nseg = 0;
     int V = 0, P = 0, C = 0;
@@ -88,17 +90,17 @@ struct Adjoint : AdjointBase {
     V3p seg_u{};
     T* seg_p = nullptr;
     // adjoint fields
-    V3p lamU[2]{};
+    V3a lamU[2]{};
     int cur = 0;
-    T *lamP = nullptr, *lamPC[2] = {nullptr, nullptr};
-    V3p lamInvD{}, lamD{}, lamS{}, lamG{}, lamB{}, lamCP{}, lamU0{}, lamUc{}, lamUcNew{}, lamUT{}, lamAO{};
-    T *lamCPE = nullptr, *lamInvDp = nullptr, *lamBp = nullptr, *lamIDPI = nullptr, *lamBPI = nullptr;
-    T* tcon = nullptr;
-    V3p lamBnd[6]{};
+    A *lamP = nullptr, *lamPC[2] = {nullptr, nullptr};
+    V3a lamInvD{}, lamD{}, lamS{}, lamG{}, lamB{}, lamCP{}, lamU0{}, lamUc{}, lamUcNew{}, lamUT{}, lamAO{};
+    A *lamCPE = nullptr, *lamInvDp = nullptr, *lamBp = nullptr, *lamIDPI = nullptr, *lamBPI = nullptr;
+    A* tcon = nullptr;
+    V3a lamBnd[6]{};
     double* lamCs = nullptr;
-    // forward-mode mirror of the fields
-    ParamsT<Dual<T>> pd{};
-    FieldsT<Dual<T>> fd{};
+    // forward-mode mirror of the fields, in the adjoint arithmetic
+    ParamsT<Dual<A>> pd{};
+    FieldsT<Dual<A>> fd{};
 
     ~Adjoint() override {
         if (st) (void)hipStreamSynchronize(st);
@@ -120,6 +122,8 @@ struct Adjoint : AdjointBase {
         };
         auto pl = [&]() { return (T*)take((size_t)N * sizeof(T)); };
         auto v3 = [&]() { return V3p{pl(), pl(), pl()}; };
+        auto pa = [&]() { return (A*)take((size_t)N * sizeof(A)); };
+        auto a3 = [&]() { return V3a{pa(), pa(), pa()}; };
         tapes.assign(K, Tape{});
         for (auto& tp : tapes) {
             tp.U.resize(V + 1);
@@ -149,14 +153,14 @@ struct Adjoint : AdjointBase {
         w_nuw = v3();
         seg_u = v3();
         seg_p = pl();
-        lamU[0] = v3(); lamU[1] = v3();
-        lamP = pl(); lamPC[0] = pl(); lamPC[1] = pl();
-        lamInvD = v3(); lamD = v3(); lamS = v3(); lamG = v3(); lamB = v3(); lamCP = v3();
-        lamU0 = v3(); lamUc = v3(); lamUcNew = v3(); lamUT = v3(); lamAO = v3();
-        lamCPE = pl(); lamInvDp = pl(); lamBp = pl(); lamIDPI = pl(); lamBPI = pl();
-        tcon = pl();
+        lamU[0] = a3(); lamU[1] = a3();
+        lamP = pa(); lamPC[0] = pa(); lamPC[1] = pa();
+        lamInvD = a3(); lamD = a3(); lamS = a3(); lamG = a3(); lamB = a3(); lamCP = a3();
+        lamU0 = a3(); lamUc = a3(); lamUcNew = a3(); lamUT = a3(); lamAO = a3();
+        lamCPE = pa(); lamInvDp = pa(); lamBp = pa(); lamIDPI = pa(); lamBPI = pa();
+        tcon = pa();
         for (int f = 0; f < 6; ++f)
-            for (int c = 0; c < 3; ++c) lamBnd[f][c] = (T*)take((size_t)bw(f) * bh(f) * sizeof(T));
+            for (int c = 0; c < 3; ++c) lamBnd[f][c] = (A*)take((size_t)bw(f) * bh(f) * sizeof(A));
         lamCs = (double*)take(sizeof(double));
         return off + 256;
     }
@@ -173,7 +177,7 @@ struct Adjoint : AdjointBase {
         const size_t nb = layout(nullptr);
         if (hipMalloc(&slab, nb) != hipSuccess) { slab = nullptr; return UWS_ERR_ALLOC; }
         layout((char*)slab);
-        pd = dual_params(s->p);
+        set_dual_params();
         const size_t db = carve(pd, fd, nullptr, false);
         if (hipMalloc(&dslab, db) != hipSuccess) { dslab = nullptr; return UWS_ERR_ALLOC; }
         carve(pd, fd, (char*)dslab, false);
@@ -184,16 +188,27 @@ struct Adjoint : AdjointBase {
         return s->sync();
     }
 
+    // the solver's parameters in the adjoint arithmetic (Cs may have been
+    // changed on the handle since creation)
+    void set_dual_params() {
+        ParamsT<A> pa{};
+        fill_params(s->cfg, pa);
+        pa.cs = (A)s->p.cs;
+        pd = dual_params(pa);
+    }
+
     // ---- helpers ----
-    int copy(T* dst, const T* src, size_t n = 0) {
-        return hipMemcpyAsync(dst, src, (n ? n : (size_t)N) * sizeof(T), hipMemcpyDeviceToDevice, st) == hipSuccess
+    template <class E>
+    int copy(E* dst, const E* src, size_t n = 0) {
+        return hipMemcpyAsync(dst, src, (n ? n : (size_t)N) * sizeof(E), hipMemcpyDeviceToDevice, st) == hipSuccess
                    ? UWS_OK
                    : UWS_ERR_DEVICE;
     }
-    int clear(T* dst, size_t n = 0) {
-        return hipMemsetAsync(dst, 0, (n ? n : (size_t)N) * sizeof(T), st) == hipSuccess ? UWS_OK : UWS_ERR_DEVICE;
+    template <class E>
+    int clear(E* dst, size_t n = 0) {
+        return hipMemsetAsync(dst, 0, (n ? n : (size_t)N) * sizeof(E), st) == hipSuccess ? UWS_OK : UWS_ERR_DEVICE;
     }
-    int clear3(const V3p& v) {
+    int clear3(const V3a& v) {
         for (int c = 0; c < 3; ++c)
             if (int rc = clear(v[c])) return rc;
         return UWS_OK;
@@ -304,13 +319,13 @@ struct Adjoint : AdjointBase {
 
     // ---- reverse of the one-shot kernels ----
     struct Seed {
-        Dual<T>* arr;
-        T* lam;
+        Dual<A>* arr;
+        A* lam;
         int kind;   // StencilKind of the outputs one input cell reaches
         int face;   // -1: a cell field; else the boundary texture of this face
     };
     template <class Run>
-    void vjp(Run&& go, const Pairs<T>& pr, const std::vector<Seed>& seeds, bool cs) {
+    void vjp(Run&& go, const Pairs<A>& pr, const std::vector<Seed>& seeds, bool cs) {
         for (const Seed& sd : seeds) {
             const bool bnd = sd.face >= 0;
             const Colour col = bnd ? colour_bnd(sd.kind) : colour_of(sd.kind);
@@ -326,22 +341,22 @@ struct Adjoint : AdjointBase {
             dual_seed(sd.arr, W, H, D, col, -1, st);
         }
         if (cs) {
-            ParamsT<Dual<T>> q = pd;
-            q.cs.d = (T)1;
+            ParamsT<Dual<A>> q = pd;
+            q.cs.d = (A)1;
             go(q);
             dual_contract(pr, tcon, N, st);
             adj_sum(tcon, N, lamCs, st);
         }
     }
-    void pack(const T* src, Dual<T>* dst) { dual_pack(src, dst, N, st); }
-    void pack3(const V3p& src, Dual<T>* const dst[3]) {
+    void pack(const T* src, Dual<A>* dst) { dual_pack<T, A>(src, dst, N, st); }
+    void pack3(const V3p& src, Dual<A>* const dst[3]) {
         for (int a = 0; a < 3; ++a) pack(src[a], dst[a]);
     }
-    void dlaunch(Kernel k, const ParamsT<Dual<T>>& q, const Roles& r) { launch<Dual<T>>(k, q, fd, r, st); }
+    void dlaunch(Kernel k, const ParamsT<Dual<A>>& q, const Roles& r) { launch<Dual<A>>(k, q, fd, r, st); }
 
     // transposed pressure solve from the adjoint of its last iterate in lamPC[0]
     void reverse_pres(const Tape& tp, int solve) {
-        AdjPresSweep<T> a{};
+        AdjPresSweep<T, A> a{};
         a.X = X; a.Y = Y; a.Z = Z;
         a.flag = s->f.flag;
         if (solve == 0) {
@@ -375,34 +390,34 @@ struct Adjoint : AdjointBase {
         const uint8_t* flag = s->f.flag;
         for (auto* v : {&lamInvD, &lamD, &lamS, &lamG, &lamB, &lamCP, &lamU0, &lamUc})
             if (int rc = clear3(*v)) return rc;
-        for (T* v : {lamCPE, lamInvDp, lamBp, lamIDPI, lamBPI})
+        for (A* v : {lamCPE, lamInvDp, lamBp, lamIDPI, lamBPI})
             if (int rc = clear(v)) return rc;
-        V3p& lu = lamU[cur];
+        V3a& lu = lamU[cur];
         // ---- PISO correctors, last first ----
         for (int c = C; c >= 1; --c) {
             // u += u'(p', 1/D, aodu), p += p' (fluid cells)
             for (int a = 0; a < 3; ++a) adj_add_fluid(flag, lamUc[a], lu[a], lamUT[a], N, st);
-            adj_add_fluid<T>(flag, nullptr, lamP, lamPC[0], N, st);
+            adj_add_fluid<A>(flag, nullptr, lamP, lamPC[0], N, st);
             if (int rc = clear3(lamAO)) return rc;
             pack(pc_final(tp, c), fd.pc[0]);
             pack3(tp.invD, fd.invD);
             pack3(tp.AO[c], fd.aodu);
             {
-                Pairs<T> pr{};
+                Pairs<A> pr{};
                 for (int a = 0; a < 3; ++a) { pr.lam[pr.n] = lamUT[a]; pr.d[pr.n++] = fd.ucorr[a]; }
                 std::vector<Seed> sd{{fd.pc[0], lamPC[0], ST_STAR7, -1}};
                 for (int a = 0; a < 3; ++a) sd.push_back({fd.invD[a], lamInvD[a], ST_OWN, -1});
                 for (int a = 0; a < 3; ++a) sd.push_back({fd.aodu[a], lamAO[a], ST_OWN, -1});
                 Roles r{};
                 r.state = 0; r.pin = 0; r.vout = 1; r.flags = K_NO_INPLACE;
-                vjp([&](const ParamsT<Dual<T>>& q) { dlaunch(KN_PISO_CORRECT, q, r); }, pr, sd, false);
+                vjp([&](const ParamsT<Dual<A>>& q) { dlaunch(KN_PISO_CORRECT, q, r); }, pr, sd, false);
             }
             reverse_pres(tp, c);
             // PISO coefficients (1/diag, rhs) of this corrector
             pack(tp.cpe, fd.cpisoE);
             pack3(tp.cp, fd.cp);
             {
-                Pairs<T> pr{};
+                Pairs<A> pr{};
                 pr.lam[pr.n] = lamIDPI; pr.d[pr.n++] = fd.invDpiso;
                 pr.lam[pr.n] = lamBPI; pr.d[pr.n++] = fd.bpiso;
                 std::vector<Seed> sd;
@@ -412,7 +427,7 @@ struct Adjoint : AdjointBase {
                 sd.push_back({fd.cp[1], lamCP[1], ST_STAR7, -1});
                 sd.push_back({fd.cp[2], lamCP[2], ST_STAR7, -1});
                 Roles r{};
-                vjp([&](const ParamsT<Dual<T>>& q) { dlaunch(KN_PISO_PRECOMP, q, r); }, pr, sd, false);
+                vjp([&](const ParamsT<Dual<A>>& q) { dlaunch(KN_PISO_PRECOMP, q, r); }, pr, sd, false);
             }
             if (int rc = clear(lamIDPI)) return rc;
             if (int rc = clear(lamBPI)) return rc;
@@ -421,14 +436,14 @@ struct Adjoint : AdjointBase {
             pack3(tp.UC[c - 1], fd.ucorr);
             if (int rc = clear3(lamUcNew)) return rc;
             {
-                Pairs<T> pr{};
+                Pairs<A> pr{};
                 for (int a = 0; a < 3; ++a) { pr.lam[pr.n] = lamAO[a]; pr.d[pr.n++] = fd.aodu[a]; }
                 std::vector<Seed> sd;
                 for (int a = 0; a < 3; ++a) sd.push_back({fd.vel[0][a], lamU0[a], ST_STAR7, -1});
                 for (int a = 0; a < 3; ++a) sd.push_back({fd.ucorr[a], lamUcNew[a], ST_STAR7, -1});
                 Roles r{};
                 r.state = 0;
-                vjp([&](const ParamsT<Dual<T>>& q) { dlaunch(KN_PISO_AODU, q, r); }, pr, sd, false);
+                vjp([&](const ParamsT<Dual<A>>& q) { dlaunch(KN_PISO_AODU, q, r); }, pr, sd, false);
             }
             std::swap(lamUc, lamUcNew);
         }
@@ -438,13 +453,13 @@ struct Adjoint : AdjointBase {
         pack(pc_final(tp, 0), fd.pc[0]);
         pack3(tp.D, fd.D);
         {
-            Pairs<T> pr{};
+            Pairs<A> pr{};
             for (int a = 0; a < 3; ++a) { pr.lam[pr.n] = lamUT[a]; pr.d[pr.n++] = fd.ucorr[a]; }
             std::vector<Seed> sd{{fd.pc[0], lamPC[0], ST_STAR7, -1}};
             for (int a = 0; a < 3; ++a) sd.push_back({fd.D[a], lamD[a], ST_OWN, -1});
             Roles r{};
             r.pin = 0; r.vout = 1; r.flags = K_NO_INPLACE;
-            vjp([&](const ParamsT<Dual<T>>& q) { dlaunch(KN_VEL_CORRECT, q, r); }, pr, sd, false);
+            vjp([&](const ParamsT<Dual<A>>& q) { dlaunch(KN_VEL_CORRECT, q, r); }, pr, sd, false);
         }
         adj_add_fluid(flag, lamPC[0], lamP, lamPC[0], N, st);
         reverse_pres(tp, 0);
@@ -453,7 +468,7 @@ struct Adjoint : AdjointBase {
         pack(tp.P0, fd.p);
         pack3(tp.invD, fd.invD);
         {
-            Pairs<T> pr{};
+            Pairs<A> pr{};
             for (int a = 0; a < 3; ++a) { pr.lam[pr.n] = lamCP[a]; pr.d[pr.n++] = fd.cp[a]; }
             pr.lam[pr.n] = lamCPE; pr.d[pr.n++] = fd.cpisoE;
             pr.lam[pr.n] = lamInvDp; pr.d[pr.n++] = fd.invDp;
@@ -467,13 +482,13 @@ struct Adjoint : AdjointBase {
                 for (int a = 0; a < 3; ++a) sd.push_back({fd.bnd[f][a], lamBnd[f][a], ST_OWN, f});
             Roles r{};
             r.vout = 1;
-            vjp([&](const ParamsT<Dual<T>>& q) { dlaunch(KN_PRES_PRECOMP, q, r); }, pr, sd, false);
+            vjp([&](const ParamsT<Dual<A>>& q) { dlaunch(KN_PRES_PRECOMP, q, r); }, pr, sd, false);
         }
         // ---- momentum sweeps, transposed ----
         {
-            AdjVelSweep<T> a{};
+            AdjVelSweep<T, A> a{};
             a.X = X; a.Y = Y; a.Z = Z;
-            a.q = s->p.ds * (T)0.25f;
+            a.q = (A)(s->p.ds * (T)0.25f);  // the forward's own coefficient
             a.flag = flag;
             for (int c = 0; c < 3; ++c) {
                 a.S[c] = tp.S[c]; a.G[c] = tp.G[c]; a.invD[c] = tp.invD[c]; a.b[c] = tp.b[c];
@@ -490,12 +505,12 @@ struct Adjoint : AdjointBase {
             }
         }
         // lamU0 += adjoint of u^0 from the sweeps (every cell)
-        for (int a = 0; a < 3; ++a) adj_add_fluid<T>(nullptr, lamU0[a], lamU[cur][a], lamU0[a], N, st);
+        for (int a = 0; a < 3; ++a) adj_add_fluid<A>(nullptr, lamU0[a], lamU[cur][a], lamU0[a], N, st);
         // ---- Op1: wall function, eddy viscosity, LES term, momentum coefficients ----
         pack3(tp.U[0], fd.vel[0]);
         pack(tp.P0, fd.p);
         {
-            Pairs<T> pr{};
+            Pairs<A> pr{};
             for (int a = 0; a < 3; ++a) {
                 pr.lam[pr.n] = lamS[a]; pr.d[pr.n++] = fd.S[a];
                 pr.lam[pr.n] = lamG[a]; pr.d[pr.n++] = fd.G[a];
@@ -510,7 +525,7 @@ struct Adjoint : AdjointBase {
                 for (int a = 0; a < 3; ++a) sd.push_back({fd.bnd[f][a], lamBnd[f][a], les ? ST_STAR7 : ST_OWN, f});
             const bool use_les = les;
             vjp(
-                [&](const ParamsT<Dual<T>>& q) {
+                [&](const ParamsT<Dual<A>>& q) {
                     Roles r{};
                     r.state = 0;
                     if (use_les) {
@@ -530,31 +545,29 @@ struct Adjoint : AdjointBase {
 
     int backward(const double* lam_u, int64_t n_u, const double* lam_p, int64_t n_p) override {
         if (!fwd_done || !lam_u || n_u != 3LL * N || (lam_p && n_p != N)) return UWS_ERR_INVALID_ARG;
-        {
-            std::vector<T> plane((size_t)N);
-            for (int a = 0; a < 3; ++a) {
-                for (int i = 0; i < N; ++i) plane[i] = (T)lam_u[(size_t)i * 3 + a];
-                if (hipMemcpyAsync(lamU[0][a], plane.data(), (size_t)N * sizeof(T), hipMemcpyHostToDevice, st) !=
-                    hipSuccess)
-                    return UWS_ERR_DEVICE;
-                if (int rc = s->sync()) return rc;
-            }
-            if (lam_p) {
-                for (int i = 0; i < N; ++i) plane[i] = (T)lam_p[i];
-                if (hipMemcpyAsync(lamP, plane.data(), (size_t)N * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
-                    return UWS_ERR_DEVICE;
-                if (int rc = s->sync()) return rc;
-            } else if (int rc = clear(lamP)) {
-                return rc;
-            }
+        for (int a = 0; a < 3; ++a) {
+            std::vector<A> plane((size_t)N);
+            for (int i = 0; i < N; ++i) plane[i] = lam_u[(size_t)i * 3 + a];
+            if (hipMemcpyAsync(lamU[0][a], plane.data(), (size_t)N * sizeof(A), hipMemcpyHostToDevice, st) !=
+                hipSuccess)
+                return UWS_ERR_DEVICE;
+            if (int rc = s->sync()) return rc;
         }
+        if (lam_p) {
+            if (hipMemcpyAsync(lamP, lam_p, (size_t)N * sizeof(A), hipMemcpyHostToDevice, st) != hipSuccess)
+                return UWS_ERR_DEVICE;
+            if (int rc = s->sync()) return rc;
+        } else if (int rc = clear(lamP)) {
+            return rc;
+        }
+        set_dual_params();
         cur = 0;
         for (int f = 0; f < 6; ++f)
             for (int a = 0; a < 3; ++a)
                 if (int rc = clear(lamBnd[f][a], (size_t)bw(f) * bh(f))) return rc;
         if (hipMemsetAsync(lamCs, 0, sizeof(double), st) != hipSuccess) return UWS_ERR_DEVICE;
         for (int f = 0; f < 6; ++f)
-            for (int a = 0; a < 3; ++a) dual_pack(s->f.bnd[f][a], fd.bnd[f][a], bw(f) * bh(f), st);
+            for (int a = 0; a < 3; ++a) dual_pack<T, A>(s->f.bnd[f][a], fd.bnd[f][a], bw(f) * bh(f), st);
         for (int seg = nseg - 1; seg >= 0; --seg) {
             const int n0 = seg * K, ns = std::min(nsteps, n0 + K) - n0;
             Tape& t0 = tapes[0];
@@ -576,9 +589,9 @@ struct Adjoint : AdjointBase {
 
     int get_grad(int what, int face, double* dst, int64_t n) override {
         if (!have_grad || !dst) return UWS_ERR_INVALID_ARG;
-        auto down = [&](const T* src, size_t np, int nc, int c) -> int {
-            std::vector<T> tmp(np);
-            if (hipMemcpyAsync(tmp.data(), src, np * sizeof(T), hipMemcpyDeviceToHost, st) != hipSuccess)
+        auto down = [&](const A* src, size_t np, int nc, int c) -> int {
+            std::vector<A> tmp(np);
+            if (hipMemcpyAsync(tmp.data(), src, np * sizeof(A), hipMemcpyDeviceToHost, st) != hipSuccess)
                 return UWS_ERR_DEVICE;
             if (int rc = s->sync()) return rc;
             for (size_t i = 0; i < np; ++i) dst[i * nc + c] = (double)tmp[i];

@@ -24,30 +24,34 @@
 
 namespace uws {
 
+// The forward's records are in the run's type T; every adjoint quantity is
+// in A (double: an FP32 run keeps FP32 records but accumulates its adjoint in
+// FP64).
+//
 // Transposed momentum sweep.  The forward sweep is, for a fluid cell P and
 // component a,  out_a(P) = invD_a(P) (sum_nb c_nb u_a(nb) + b_a(P)),  with the
 // face coefficients built from S and G of the owned E/N/U faces.
-template <class T>
+template <class T, class A>
 struct AdjVelSweep {
     int X, Y, Z;
-    T q;                          // ds / 4
+    A q;                          // ds / 4
     const uint8_t* flag;
     const T* S[3];
     const T* G[3];
     const T* invD[3];
     const T* b[3];
     const T* in[3];               // the sweep's input iterate
-    const T* lam_out[3];          // adjoint of the sweep's output
-    T* lam_in[3];                 // adjoint of the input iterate (written)
-    T* lam_S[3];                  // accumulated
-    T* lam_G[3];
-    T* lam_invD[3];
-    T* lam_b[3];
+    const A* lam_out[3];          // adjoint of the sweep's output
+    A* lam_in[3];                 // adjoint of the input iterate (written)
+    A* lam_S[3];                  // accumulated
+    A* lam_G[3];
+    A* lam_invD[3];
+    A* lam_b[3];
 };
 
 // Transposed pressure sweep: out(P) = invDp(P) (-sum_f c_f x(nb) + rhs(P)) on
 // fluid cells, 0 elsewhere; in == nullptr is the zero first iterate.
-template <class T>
+template <class T, class A>
 struct AdjPresSweep {
     int X, Y, Z;
     const uint8_t* flag;
@@ -57,13 +61,13 @@ struct AdjPresSweep {
     const T* invDp;
     const T* rhs;
     const T* in;
-    const T* lam_out;
-    T* lam_in;                    // written when in != nullptr
-    T* lam_cE;                    // accumulated
-    T* lam_cN;
-    T* lam_cU;
-    T* lam_invDp;
-    T* lam_rhs;
+    const A* lam_out;
+    A* lam_in;                    // written when in != nullptr
+    A* lam_cE;                    // accumulated
+    A* lam_cN;
+    A* lam_cU;
+    A* lam_invDp;
+    A* lam_rhs;
 };
 
 // colour(x, y, z) = (x + wy*y + wz*z) mod m; m == 1 is a single colour
@@ -87,10 +91,10 @@ struct Pairs {
     const Dual<T>* d[16];
 };
 
-template <class T> void adj_vel_sweep(const AdjVelSweep<T>& a, int N, hipStream_t s);
-template <class T> void adj_pres_sweep(const AdjPresSweep<T>& a, int N, hipStream_t s);
-// dst[i] = (src[i] or 0, 0)
-template <class T> void dual_pack(const T* src, Dual<T>* dst, int n, hipStream_t s);
+template <class T, class A> void adj_vel_sweep(const AdjVelSweep<T, A>& a, int N, hipStream_t s);
+template <class T, class A> void adj_pres_sweep(const AdjPresSweep<T, A>& a, int N, hipStream_t s);
+// dst[i] = ((A)src[i] or 0, 0)
+template <class T, class A> void dual_pack(const T* src, Dual<A>* dst, int n, hipStream_t s);
 // dst[i].d = 1 on the cells of colour k, 0 elsewhere (k < 0: 0 everywhere);
 // the array is a W x H x D box in x-fastest order
 template <class T> void dual_seed(Dual<T>* dst, int W, int H, int D, Colour c, int k, hipStream_t s);

@@ -19,8 +19,8 @@ __device__ __forceinline__ int gid() { return (int)(blockIdx.x * blockDim.x + th
 // coefficients of faces that are not fluid-fluid are zero, and invD is zero
 // off the fluid, so no flags are needed for this part.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(BLK) void k_adj_vel_sweep(AdjVelSweep<T> a, int N) {
+template <class T, class A>
+__global__ __launch_bounds__(BLK) void k_adj_vel_sweep(AdjVelSweep<T, A> a, int N) {
     const int i = gid();
     if (i >= N) return;
     const int X = a.X, Y = a.Y, Z = a.Z;
@@ -28,50 +28,50 @@ __global__ __launch_bounds__(BLK) void k_adj_vel_sweep(AdjVelSweep<T> a, int N) 
     const int sy = X, sz = X * Y;
     const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
     const bool isF = a.flag[i] == UWS_CELL_FLUID;
-    const T q = a.q;
-    T rP[3];
+    const A q = a.q;
+    A rP[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const T* lo = a.lam_out[c];
+        const A* lo = a.lam_out[c];
         const T* iv = a.invD[c];
-        rP[c] = lo[i] * iv[i];
-        T acc = 0;
-        if (hW) acc += lo[i - 1] * iv[i - 1] * (-q * a.S[0][i - 1] + a.G[0][i - 1]);
-        if (hE) acc += lo[i + 1] * iv[i + 1] * (q * a.S[0][i] + a.G[0][i]);
-        if (hS) acc += lo[i - sy] * iv[i - sy] * (-q * a.S[1][i - sy] + a.G[1][i - sy]);
-        if (hN) acc += lo[i + sy] * iv[i + sy] * (q * a.S[1][i] + a.G[1][i]);
-        if (hD) acc += lo[i - sz] * iv[i - sz] * (-q * a.S[2][i - sz] + a.G[2][i - sz]);
-        if (hU) acc += lo[i + sz] * iv[i + sz] * (q * a.S[2][i] + a.G[2][i]);
+        rP[c] = lo[i] * (A)iv[i];
+        A acc = 0;
+        if (hW) acc += lo[i - 1] * (A)iv[i - 1] * (-q * (A)a.S[0][i - 1] + (A)a.G[0][i - 1]);
+        if (hE) acc += lo[i + 1] * (A)iv[i + 1] * (q * (A)a.S[0][i] + (A)a.G[0][i]);
+        if (hS) acc += lo[i - sy] * (A)iv[i - sy] * (-q * (A)a.S[1][i - sy] + (A)a.G[1][i - sy]);
+        if (hN) acc += lo[i + sy] * (A)iv[i + sy] * (q * (A)a.S[1][i] + (A)a.G[1][i]);
+        if (hD) acc += lo[i - sz] * (A)iv[i - sz] * (-q * (A)a.S[2][i - sz] + (A)a.G[2][i - sz]);
+        if (hU) acc += lo[i + sz] * (A)iv[i + sz] * (q * (A)a.S[2][i] + (A)a.G[2][i]);
         a.lam_in[c][i] = acc;
     }
     if (isF) {
         // the bracket A + Df + b of the forward sweep, on the input iterate
         const int iE = hE ? i + 1 : i, iW = hW ? i - 1 : i, iN = hN ? i + sy : i, iS = hS ? i - sy : i;
         const int iU = hU ? i + sz : i, iD = hD ? i - sz : i;
-        const T SE = a.S[0][i], SN = a.S[1][i], SU = a.S[2][i];
-        const T GE = a.G[0][i], GN = a.G[1][i], GU = a.G[2][i];
-        const T SW = hW ? a.S[0][iW] : (T)0, GW = hW ? a.G[0][iW] : (T)0;
-        const T SS = hS ? a.S[1][iS] : (T)0, GS = hS ? a.G[1][iS] : (T)0;
-        const T SD = hD ? a.S[2][iD] : (T)0, GD = hD ? a.G[2][iD] : (T)0;
+        const A SE = a.S[0][i], SN = a.S[1][i], SU = a.S[2][i];
+        const A GE = a.G[0][i], GN = a.G[1][i], GU = a.G[2][i];
+        const A SW = hW ? (A)a.S[0][iW] : (A)0, GW = hW ? (A)a.G[0][iW] : (A)0;
+        const A SS = hS ? (A)a.S[1][iS] : (A)0, GS = hS ? (A)a.G[1][iS] : (A)0;
+        const A SD = hD ? (A)a.S[2][iD] : (A)0, GD = hD ? (A)a.G[2][iD] : (A)0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const T* k = a.in[c];
-            const T vE = k[iE], vW = k[iW], vN = k[iN], vS = k[iS], vU = k[iU], vD = k[iD];
-            T A = 0;
-            A -= q * vE * SE;
-            A += q * vW * SW;
-            A -= q * vN * SN;
-            A += q * vS * SS;
-            A -= q * vU * SU;
-            A += q * vD * SD;
-            T Df = 0;
+            const A vE = k[iE], vW = k[iW], vN = k[iN], vS = k[iS], vU = k[iU], vD = k[iD];
+            A Av = 0;
+            Av -= q * vE * SE;
+            Av += q * vW * SW;
+            Av -= q * vN * SN;
+            Av += q * vS * SS;
+            Av -= q * vU * SU;
+            Av += q * vD * SD;
+            A Df = 0;
             Df += GE * vE;
             Df += GW * vW;
             Df += GN * vN;
             Df += GS * vS;
             Df += GU * vU;
             Df += GD * vD;
-            a.lam_invD[c][i] += a.lam_out[c][i] * (A + Df + a.b[c][i]);
+            a.lam_invD[c][i] += a.lam_out[c][i] * (Av + Df + (A)a.b[c][i]);
             a.lam_b[c][i] += rP[c];
         }
     }
@@ -82,11 +82,11 @@ __global__ __launch_bounds__(BLK) void k_adj_vel_sweep(AdjVelSweep<T> a, int N) 
     for (int e = 0; e < 3; ++e) {
         if (!has[e]) continue;
         const int j = i + offs[e];
-        T sS = 0, sG = 0;
+        A sS = 0, sG = 0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T rQ = a.lam_out[c][j] * a.invD[c][j];
-            const T uP = a.in[c][i], uQ = a.in[c][j];
+            const A rQ = a.lam_out[c][j] * (A)a.invD[c][j];
+            const A uP = a.in[c][i], uQ = a.in[c][j];
             sS += rP[c] * (-q * uQ) + rQ * (q * uP);
             sG += rP[c] * uQ + rQ * uP;
         }
@@ -99,58 +99,58 @@ __global__ __launch_bounds__(BLK) void k_adj_vel_sweep(AdjVelSweep<T> a, int N) 
 // Transposed pressure sweep.  out(P) = invDp(P) (term(P) + rhs(P)) with
 // term(P) = -sum_f c_f x(nb); c_f of a face is stored on the cell owning it.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(BLK) void k_adj_pres_sweep(AdjPresSweep<T> a, int N) {
+template <class T, class A>
+__global__ __launch_bounds__(BLK) void k_adj_pres_sweep(AdjPresSweep<T, A> a, int N) {
     const int i = gid();
     if (i >= N) return;
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int x = i % X, t = i / X, y = t % Y, z = t / Y;
     const int sy = X, sz = X * Y;
     const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
-    const T* lo = a.lam_out;
+    const A* lo = a.lam_out;
     const T* iv = a.invDp;
-    const T rP = lo[i] * iv[i];
+    const A rP = lo[i] * (A)iv[i];
     const T* xin = a.in;
     if (xin) {
-        T acc = 0;
-        if (hW) acc -= lo[i - 1] * iv[i - 1] * a.cE[i - 1];
-        if (hE) acc -= lo[i + 1] * iv[i + 1] * a.cE[i];
-        if (hS) acc -= lo[i - sy] * iv[i - sy] * a.cN[i - sy];
-        if (hN) acc -= lo[i + sy] * iv[i + sy] * a.cN[i];
-        if (hD) acc -= lo[i - sz] * iv[i - sz] * a.cU[i - sz];
-        if (hU) acc -= lo[i + sz] * iv[i + sz] * a.cU[i];
+        A acc = 0;
+        if (hW) acc -= lo[i - 1] * (A)iv[i - 1] * (A)a.cE[i - 1];
+        if (hE) acc -= lo[i + 1] * (A)iv[i + 1] * (A)a.cE[i];
+        if (hS) acc -= lo[i - sy] * (A)iv[i - sy] * (A)a.cN[i - sy];
+        if (hN) acc -= lo[i + sy] * (A)iv[i + sy] * (A)a.cN[i];
+        if (hD) acc -= lo[i - sz] * (A)iv[i - sz] * (A)a.cU[i - sz];
+        if (hU) acc -= lo[i + sz] * (A)iv[i + sz] * (A)a.cU[i];
         a.lam_in[i] = acc;
     }
     if (a.flag[i] == UWS_CELL_FLUID) {
-        T term = 0;
+        A term = 0;
         if (xin) {
             const int iE = hE ? i + 1 : i, iW = hW ? i - 1 : i, iN = hN ? i + sy : i, iS = hS ? i - sy : i;
             const int iU = hU ? i + sz : i, iD = hD ? i - sz : i;
-            const T cW = hW ? a.cE[iW] : (T)0, cS = hS ? a.cN[iS] : (T)0, cD = hD ? a.cU[iD] : (T)0;
-            term -= a.cE[i] * xin[iE];
-            term -= cW * xin[iW];
-            term -= a.cN[i] * xin[iN];
-            term -= cS * xin[iS];
-            term -= a.cU[i] * xin[iU];
-            term -= cD * xin[iD];
+            const A cW = hW ? (A)a.cE[iW] : (A)0, cS = hS ? (A)a.cN[iS] : (A)0, cD = hD ? (A)a.cU[iD] : (A)0;
+            term -= (A)a.cE[i] * (A)xin[iE];
+            term -= cW * (A)xin[iW];
+            term -= (A)a.cN[i] * (A)xin[iN];
+            term -= cS * (A)xin[iS];
+            term -= (A)a.cU[i] * (A)xin[iU];
+            term -= cD * (A)xin[iD];
         }
-        a.lam_invDp[i] += lo[i] * (term + a.rhs[i]);
+        a.lam_invDp[i] += lo[i] * (term + (A)a.rhs[i]);
         a.lam_rhs[i] += rP;
     }
     if (xin) {
-        if (hE) a.lam_cE[i] -= rP * xin[i + 1] + lo[i + 1] * iv[i + 1] * xin[i];
-        if (hN) a.lam_cN[i] -= rP * xin[i + sy] + lo[i + sy] * iv[i + sy] * xin[i];
-        if (hU) a.lam_cU[i] -= rP * xin[i + sz] + lo[i + sz] * iv[i + sz] * xin[i];
+        if (hE) a.lam_cE[i] -= rP * (A)xin[i + 1] + lo[i + 1] * (A)iv[i + 1] * (A)xin[i];
+        if (hN) a.lam_cN[i] -= rP * (A)xin[i + sy] + lo[i + sy] * (A)iv[i + sy] * (A)xin[i];
+        if (hU) a.lam_cU[i] -= rP * (A)xin[i + sz] + lo[i + sz] * (A)iv[i + sz] * (A)xin[i];
     }
 }
 
 // ---------------------------------------------------------------------------
 // forward-mode machinery
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(BLK) void k_dual_pack(const T* src, Dual<T>* dst, int n) {
+template <class T, class A>
+__global__ __launch_bounds__(BLK) void k_dual_pack(const T* src, Dual<A>* dst, int n) {
     const int i = gid();
-    if (i < n) dst[i] = Dual<T>(src ? src[i] : (T)0, (T)0);
+    if (i < n) dst[i] = Dual<A>(src ? (A)src[i] : (A)0, (A)0);
 }
 
 __device__ __forceinline__ int colour_at(const Colour& c, int x, int y, int z) {
@@ -286,17 +286,17 @@ Colour colour_of(int kind) {
 // texture plane: (i + 2j) mod 5 separates them
 Colour colour_bnd(int kind) { return kind == ST_OWN ? Colour{1, 0, 0} : Colour{5, 2, 0}; }
 
-template <class T>
-void adj_vel_sweep(const AdjVelSweep<T>& a, int N, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_vel_sweep<T>, dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+template <class T, class A>
+void adj_vel_sweep(const AdjVelSweep<T, A>& a, int N, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_vel_sweep<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, N);
 }
-template <class T>
-void adj_pres_sweep(const AdjPresSweep<T>& a, int N, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_pres_sweep<T>, dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+template <class T, class A>
+void adj_pres_sweep(const AdjPresSweep<T, A>& a, int N, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_pres_sweep<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, N);
 }
-template <class T>
-void dual_pack(const T* src, Dual<T>* dst, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_dual_pack<T>, dim3(blocks(n)), dim3(BLK), 0, s, src, dst, n);
+template <class T, class A>
+void dual_pack(const T* src, Dual<A>* dst, int n, hipStream_t s) {
+    hipLaunchKernelGGL((k_dual_pack<T, A>), dim3(blocks(n)), dim3(BLK), 0, s, src, dst, n);
 }
 template <class T>
 void dual_seed(Dual<T>* dst, int W, int H, int D, Colour c, int k, hipStream_t s) {
@@ -326,18 +326,19 @@ void adj_sum(const T* t, int n, double* acc, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_sum<T>, dim3(1), dim3(1024), 0, s, t, n, acc);
 }
 
-#define UWS_ADJ_INST(T)                                                                                         \
-    template void adj_vel_sweep<T>(const AdjVelSweep<T>&, int, hipStream_t);                                    \
-    template void adj_pres_sweep<T>(const AdjPresSweep<T>&, int, hipStream_t);                                  \
-    template void dual_pack<T>(const T*, Dual<T>*, int, hipStream_t);                                           \
-    template void dual_seed<T>(Dual<T>*, int, int, int, Colour, int, hipStream_t);                              \
-    template void dual_contract<T>(const Pairs<T>&, T*, int, hipStream_t);                                      \
-    template void adj_gather<T>(const T*, int, int, int, const Stencil&, Colour, int, T*, hipStream_t);         \
-    template void adj_gather_bnd<T>(const T*, int, int, int, int, const Stencil&, Colour, int, T*, hipStream_t); \
-    template void adj_add_fluid<T>(const uint8_t*, const T*, const T*, T*, int, hipStream_t);                   \
-    template void adj_sum<T>(const T*, int, double*, hipStream_t);
-UWS_ADJ_INST(float)
-UWS_ADJ_INST(double)
-#undef UWS_ADJ_INST
+// records in float or double, adjoint in double
+template void adj_vel_sweep<float, double>(const AdjVelSweep<float, double>&, int, hipStream_t);
+template void adj_vel_sweep<double, double>(const AdjVelSweep<double, double>&, int, hipStream_t);
+template void adj_pres_sweep<float, double>(const AdjPresSweep<float, double>&, int, hipStream_t);
+template void adj_pres_sweep<double, double>(const AdjPresSweep<double, double>&, int, hipStream_t);
+template void dual_pack<float, double>(const float*, Dual<double>*, int, hipStream_t);
+template void dual_pack<double, double>(const double*, Dual<double>*, int, hipStream_t);
+template void dual_seed<double>(Dual<double>*, int, int, int, Colour, int, hipStream_t);
+template void dual_contract<double>(const Pairs<double>&, double*, int, hipStream_t);
+template void adj_gather<double>(const double*, int, int, int, const Stencil&, Colour, int, double*, hipStream_t);
+template void adj_gather_bnd<double>(const double*, int, int, int, int, const Stencil&, Colour, int, double*,
+                                     hipStream_t);
+template void adj_add_fluid<double>(const uint8_t*, const double*, const double*, double*, int, hipStream_t);
+template void adj_sum<double>(const double*, int, double*, hipStream_t);
 
 }  // namespace uws
