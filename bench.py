@@ -118,6 +118,73 @@ def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
             "max_rel_l2_u_vs_fp64": max(errs), "compared_after_steps": check_steps}
 
 
+def precision_report(steps: int = 500, mean_last: int = 250) -> dict:
+    """The precision case (384 x 128 high x 256 span, 120 buildings): the
+    same run in FP64, FP32 and FP32 with FP16 / BF16 correction storage
+    (3 rounds x 20 sweeps of defect correction per pressure solve; the plain
+    runs do the 60 sweeps directly), FP64 norms every step.  Errors of u, p
+    after `steps` steps and of the time mean of u over the last `mean_last`
+    steps, relative L2 against the FP64 run."""
+    import copy
+
+    import numpy as np
+
+    from unitywindsimulation_amd import FvmSolverHip, workloads
+    from unitywindsimulation_amd import _native as N
+
+    w = workloads.precision(steps=steps, mean_last=mean_last)
+    X, Y, Z = w.grid
+    modes = [("fp64", N.PREC_FP64), ("fp32", N.PREC_FP32), ("fp32_fp16", N.PREC_FP32_FP16),
+             ("fp32_bf16", N.PREC_FP32_BF16)]
+    out = {"workload": w.name, "grid_xyz": [X, Y, Z], "cells": w.ncells, "buildings": len(w.boxes),
+           "steps": steps, "mean_over_last": mean_last, "refine": [w.cf.refineRounds, w.cf.refineSweeps]}
+    ref = None
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    for name, prec in modes:
+        cf = copy.deepcopy(w.cf)
+        cf.precision = prec
+        cf.computeNorms = True
+        cf.Init()
+        with FvmSolverHip(cf) as s:
+            s.InitFlags(w.flags)
+            s.init_noise(w.noise_seed, w.noise_amp)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(steps - mean_last)
+            s.mean_reset()
+            for _ in range(mean_last):
+                s.step(1)
+                s.mean_accumulate()
+            s.synchronize()
+            t = time.perf_counter() - t0
+            u = s.field64(N.FIELD_VEL)
+            p = s.field64(N.FIELD_PRES)
+            um, cnt = s.mean_velocity()
+            nrm = s.norms()
+        rec = {"ms_per_step": round(t / steps * 1e3, 3), "finite": bool(np.isfinite(u).all() and np.isfinite(p).all()),
+               "norms_last_step": {"vel_residual": float(nrm[N.NORM_VEL_RESIDUAL]),
+                                   "pres_residual": float(nrm[N.NORM_PRES_RESIDUAL]),
+                                   "divergence": float(nrm[N.NORM_DIVERGENCE]),
+                                   "nonfinite_cells": float(nrm[N.NORM_NONFINITE])}}
+        if prec in (N.PREC_FP32_FP16, N.PREC_FP32_BF16):
+            rec["norms_last_step"]["refine_defect_simple_rounds"] = [float(nrm[N.NORM_REFINE0 + r])
+                                                                     for r in range(w.cf.refineRounds)]
+        if ref is None:
+            ref = (u, p, um, rec["norms_last_step"])
+        else:
+            rec["rel_l2_vs_fp64"] = {"u": rel(u, ref[0]), "p": rel(p, ref[1]), "u_time_mean": rel(um, ref[2])}
+            rec["norm_rel_err_vs_fp64"] = {k: abs(rec["norms_last_step"][k] - ref[3][k]) / abs(ref[3][k])
+                                           for k in ("vel_residual", "pres_residual", "divergence")
+                                           if ref[3][k] != 0}
+        out[name] = rec
+        print(f"[precision] {name}: {rec['ms_per_step']} ms/step {rec.get('rel_l2_vs_fp64', '')}", file=sys.stderr,
+              flush=True)
+    return out
+
+
 def adjoint_report(ndir: int = 16, h_fd=(1e-3, 1e-6)) -> dict:
     """The differentiation case: 64 steps with a checkpoint every 8, J = mean
     |u|^2 on the 2 m layer (average of cell layers 1 and 2) of a seeded 32 x 32
@@ -245,6 +312,9 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation leg (adjoint vs complex step and central differences)")
     ap.add_argument("--adjoint-dirs", type=int, default=16)
+    ap.add_argument("--precision", action="store_true",
+                    help="add the precision leg (FP32 / FP16 / BF16 correction storage vs an FP64 run, 500 steps)")
+    ap.add_argument("--precision-steps", type=int, default=500)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM counters)")
     args = ap.parse_args()
@@ -327,6 +397,8 @@ def main() -> int:
     }
     if not args.no_ensemble:
         out["ensemble"] = ensemble_report()
+    if args.precision:
+        out["precision"] = precision_report(args.precision_steps, args.precision_steps // 2)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_report(args.adjoint_dirs)
     if not args.no_cpu_baseline:

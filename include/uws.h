@@ -294,6 +294,15 @@ int uws_batch_num_groups(uws_batch_handle b, int32_t* n);
  * best = {ty, persistent, group_bytes}, best_ms = ms per step of the winner. */
 int uws_batch_autotune(uws_batch_handle b, int32_t steps, int64_t best[3], double* best_ms);
 
+/* Time mean of the velocity (extension; the reference averages exported
+ * frames offline).  uws_mean_accumulate adds the current u^n to an FP64
+ * running sum held on the device (allocated by the first call of either
+ * function), uws_mean_reset clears it; uws_get_mean_velocity returns
+ * sum / count as [cell][3] doubles (n = 3 * cells) and the count. */
+int uws_mean_reset(uws_handle h);
+int uws_mean_accumulate(uws_handle h);
+int uws_get_mean_velocity(uws_handle h, double* dst, int64_t n, int64_t* count);
+
 /* ---- differentiation (extension, no reference counterpart) ---- */
 
 /* FluidSimConfig.smagorinskyConstant (FluidSimConfig.cs:156) of a live

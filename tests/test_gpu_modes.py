@@ -91,3 +91,21 @@ def test_time_mean_matches_oracle(gpu):
         e_mean = rel_l2(mg / 100, mo / 100)
         print(f"200 steps: instantaneous rel-L2 {e_last:.2e}, time-mean rel-L2 {e_mean:.2e}")
         assert e_mean <= 1e-2
+
+
+def test_time_mean_accumulator(gpu):
+    """uws_mean_accumulate / uws_get_mean_velocity: the FP64 mean of the
+    velocities after each step, equal to the mean of the downloaded states."""
+    w = workloads.tiny()
+    with solver_for(w) as s:
+        s.step(1)
+        s.mean_reset()
+        frames = []
+        for _ in range(4):
+            s.step(1)
+            s.mean_accumulate()
+            frames.append(s.field64(N.FIELD_VEL))
+        um, cnt = s.mean_velocity()
+    assert cnt == 4
+    ref = np.mean(frames, axis=0)
+    assert np.max(np.abs(um - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))

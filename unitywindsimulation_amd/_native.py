@@ -125,6 +125,9 @@ SIGNATURES = {
     "uws_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int64]),
     "uws_batch_num_groups": (c_int, [c_void_p, POINTER(c_int32)]),
     "uws_batch_autotune": (c_int, [c_void_p, c_int32, c_void_p, POINTER(c_double)]),
+    "uws_mean_reset": (c_int, [c_void_p]),
+    "uws_mean_accumulate": (c_int, [c_void_p]),
+    "uws_get_mean_velocity": (c_int, [c_void_p, c_void_p, c_int64, POINTER(c_int64)]),
     "uws_set_smagorinsky_constant": (c_int, [c_void_p, c_double]),
     "uws_set_bnd_field_f64": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
     "uws_cstep_set_imag": (c_int, [c_void_p, c_int32, c_int32, c_void_p, c_int64]),

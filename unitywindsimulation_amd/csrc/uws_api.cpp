@@ -516,6 +516,19 @@ int uws_set_bnd_field_f64(uws_handle h, int32_t face, const double* xyz, int64_t
     });
 }
 
+int uws_mean_reset(uws_handle h) {
+    return with_solver(h, [&](SolverBase& s) { return s.mean_reset(); });
+}
+
+int uws_mean_accumulate(uws_handle h) {
+    return with_solver(h, [&](SolverBase& s) { return s.mean_accumulate(); });
+}
+
+int uws_get_mean_velocity(uws_handle h, double* dst, int64_t n, int64_t* count) {
+    if (!dst) return UWS_ERR_INVALID_ARG;
+    return with_solver(h, [&](SolverBase& s) { return s.get_mean(dst, n, count); });
+}
+
 int uws_set_smagorinsky_constant(uws_handle h, double cs) {
     if (!std::isfinite(cs) || cs < 0.0) return UWS_ERR_INVALID_ARG;
     return with_solver(h, [&](SolverBase& s) { return s.set_cs(cs, 0.0); });

@@ -70,6 +70,9 @@ struct SolverBase {
     virtual int step_n(int n) = 0;
     virtual int time_kernel(int kernel, int reps, double* avg_ms) = 0;
     virtual int set_cs(double re, double im) = 0;
+    virtual int mean_reset() = 0;
+    virtual int mean_accumulate() = 0;
+    virtual int get_mean(double* dst, int64_t n, int64_t* count) = 0;
     virtual int cstep_imag(bool set, int what, int face, double* buf, int64_t n) = 0;
 };
 // runs whose host-side profiles are evaluated in double

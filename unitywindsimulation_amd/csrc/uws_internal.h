@@ -116,6 +116,10 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
 template <class T>
 void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot = 0);
 
+// sum[c][i] += u[c][i] in FP64 (time averaging of the velocity)
+template <class T>
+void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s);
+
 inline int tiles_of(int X, int Y, int Z, int ty) { return ((X + 63) / 64) * ((Y + ty - 1) / ty) * Z; }
 
 }  // namespace uws

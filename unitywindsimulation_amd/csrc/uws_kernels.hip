@@ -18,6 +18,8 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "uws_internal.h"
 
 namespace uws {
@@ -1138,6 +1140,28 @@ void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, 
 #undef GOB
 #undef GORB
 }
+
+namespace {
+template <class T>
+__global__ __launch_bounds__(256) void k_mean_accumulate(const T* u0, const T* u1, const T* u2, double* s0,
+                                                        double* s1, double* s2, int n) {
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int)(gridDim.x * blockDim.x)) {
+        s0[i] += real_of(u0[i]);
+        s1[i] += real_of(u1[i]);
+        s2[i] += real_of(u2[i]);
+    }
+}
+}  // namespace
+
+template <class T>
+void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s) {
+    const int grid = std::min((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_mean_accumulate<T>, dim3(grid), dim3(256), 0, s, u[0], u[1], u[2], sum[0], sum[1], sum[2],
+                       n);
+}
+template void mean_accumulate<float>(float* const[3], double* const[3], int, hipStream_t);
+template void mean_accumulate<double>(double* const[3], double* const[3], int, hipStream_t);
+template void mean_accumulate<CStep>(CStep* const[3], double* const[3], int, hipStream_t);
 
 template void launch<float>(Kernel, const ParamsT<float>&, const FieldsT<float>&, const Roles&, hipStream_t, int);
 template void launch<double>(Kernel, const ParamsT<double>&, const FieldsT<double>&, const Roles&, hipStream_t,

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <type_traits>
 #include <vector>
@@ -26,7 +27,13 @@ struct Solver : SolverBase {
     FieldsT<T> f{};
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // one per starting state buffer
 
-    ~Solver() override { drop_graphs(); }
+    double* mean_sum[3] = {nullptr, nullptr, nullptr};  // FP64 time sum of u (allocated on first use)
+    int64_t mean_count = 0;
+
+    ~Solver() override {
+        drop_graphs();
+        if (mean_sum[0]) (void)hipFree(mean_sum[0]);
+    }
     void drop_graphs() {
         for (auto& g : graph)
             if (g) {
@@ -269,6 +276,42 @@ struct Solver : SolverBase {
             }
             return UWS_OK;
         }
+    }
+
+    // ---- time mean of the velocity ----
+    int mean_reset() override {
+        if (!mean_sum[0]) {
+            void* m = nullptr;
+            if (hipMalloc(&m, 3 * sizeof(double) * (size_t)p.N) != hipSuccess) return UWS_ERR_ALLOC;
+            for (int c = 0; c < 3; ++c) mean_sum[c] = (double*)m + (size_t)c * p.N;
+        }
+        HIP_OK(hipMemsetAsync(mean_sum[0], 0, 3 * sizeof(double) * (size_t)p.N, stream));
+        mean_count = 0;
+        return UWS_OK;
+    }
+    int mean_accumulate() override {
+        if (rs.cur >= 0) return UWS_ERR_INVALID_ARG;
+        if (!mean_sum[0])
+            if (int rc = mean_reset()) return rc;
+        uws::mean_accumulate<T>(f.vel[rs.state], mean_sum, p.N, stream);
+        mean_count += 1;
+        return check_launch();
+    }
+    int get_mean(double* dst, int64_t n, int64_t* count) override {
+        if (n != 3LL * p.N) return UWS_ERR_INVALID_ARG;
+        if (count) *count = mean_count;
+        std::vector<double> tmp((size_t)p.N);
+        for (int c = 0; c < 3; ++c) {
+            if (mean_sum[c]) {
+                HIP_OK(hipMemcpyAsync(tmp.data(), mean_sum[c], sizeof(double) * p.N, hipMemcpyDeviceToHost, stream));
+                if (int rc = sync()) return rc;
+            } else {
+                std::fill(tmp.begin(), tmp.end(), 0.0);
+            }
+            const double inv = mean_count > 0 ? 1.0 / (double)mean_count : 0.0;
+            for (int i = 0; i < p.N; ++i) dst[(size_t)i * 3 + c] = tmp[i] * inv;
+        }
+        return UWS_OK;
     }
 
     // ---- stepping ----

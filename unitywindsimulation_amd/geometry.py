@@ -75,11 +75,11 @@ class Box:
     h: int
 
 
-def rasterise(grid, boxes, angle_deg: float = 0.0) -> np.ndarray:
+def rasterise(grid, boxes, angle_deg: float = 0.0, centre=None) -> np.ndarray:
     """int32 mask [z][y][x] flattened; with angle != 0 every footprint is
-    rotated about the domain centre by -angle (the inflow direction rotated
-    into the solver frame, whose inflow is always at X0) and a cell is solid
-    when its centre lies inside a rotated footprint."""
+    rotated by -angle about `centre` ((x, z), default the domain centre): the
+    inflow direction rotated into the solver frame, whose inflow is always at
+    X0.  A cell is solid when its centre lies inside a rotated footprint."""
     X, Y, Z = grid
     m = np.zeros((Z, Y, X), dtype=np.int32)
     if angle_deg % 360.0 == 0.0:
@@ -91,7 +91,7 @@ def rasterise(grid, boxes, angle_deg: float = 0.0) -> np.ndarray:
         return m.reshape(-1)
     th = math.radians(angle_deg)
     c, s = math.cos(th), math.sin(th)
-    cx, cz = X / 2.0, Z / 2.0
+    cx, cz = (X / 2.0, Z / 2.0) if centre is None else (float(centre[0]), float(centre[1]))
     xc = np.arange(X) + 0.5
     zc = np.arange(Z) + 0.5
     XX, ZZ = np.meshgrid(xc, zc)  # [z][x]

@@ -194,6 +194,20 @@ class FvmSolverHip(IFvmSolver):
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float32).reshape(-1, 3))
         N.check(self._lib.uws_set_bnd_field(self._h, face, N.ptr(a), a.shape[0]), "uws_set_bnd_field")
 
+    def mean_reset(self) -> None:
+        N.check(self._lib.uws_mean_reset(self._h), "uws_mean_reset")
+
+    def mean_accumulate(self) -> None:
+        """Add the current velocity to the FP64 time sum on the device."""
+        N.check(self._lib.uws_mean_accumulate(self._h), "uws_mean_accumulate")
+
+    def mean_velocity(self):
+        """(time mean of u as (N, 3) float64, number of accumulated steps)."""
+        out = np.empty((self.n, 3), dtype=np.float64)
+        cnt = c_int64()
+        N.check(self._lib.uws_get_mean_velocity(self._h, N.ptr(out), out.size, byref(cnt)), "uws_get_mean_velocity")
+        return out, int(cnt.value)
+
     def bnd_size(self, face: int) -> int:
         X, Y, Z = self.cf.gridRes
         return (Y if face < 2 else X) * (Z if face < 4 else Y)

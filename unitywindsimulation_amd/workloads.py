@@ -12,7 +12,13 @@ gradient elsewhere (the boundary set of the reference's AijBuilding0 case,
 Code/Config/AijBuilding0.asset:35-46).  Smagorinsky LES with Cs = 0.1, the
 rough-wall log law (z0 = 0.01 m) on the ground and all building faces (all
 three wall-viscosity slots), PISO with 2 correctors, 50 momentum and 50
-pressure Jacobi sweeps, dt from CFL 0.5 on the largest inlet speed.  The
+pressure Jacobi sweeps, dt from CFL 0.5 on the peak cell speed, taken as
+PEAK_TO_INLET = 4 times the largest inlet speed (the flow between and over
+the buildings reaches 20-31 m/s for a 7-8.4 m/s inlet peak).  The reference's
+step applies the full SIMPLE pressure correction without under-relaxation to
+a centrally differenced momentum equation, which is only stable for small
+convective Courant numbers: with CFL 0.5 on the inlet speed (dt four times
+larger) the urban case diverges near step 140, with dt/2 near step 820.  The
 initial velocity is U(-0.05, 0.05) m/s noise per component in fluid cells.
 """
 from __future__ import annotations
@@ -27,6 +33,7 @@ from .config import (FluidSimConfig, FvmSolverType, InflowType, PresBndCond, Tur
 from .geometry import Box, SplitMix64, lattice_layout, rasterise
 
 U_REF, H_REF, ALPHA = 5.0, 16.0, 0.25
+PEAK_TO_INLET = 4.0
 
 
 @dataclass
@@ -56,7 +63,7 @@ def power_law_umax(ny: int, dx: float = 1.0) -> float:
 
 def base_config(grid, vel_iters=50, pres_iters=50, correctors=2) -> FluidSimConfig:
     X, Y, Z = grid
-    dt = 0.5 * 1.0 / power_law_umax(Y)
+    dt = 0.5 * 1.0 / (PEAK_TO_INLET * power_law_umax(Y))
     cf = FluidSimConfig(
         physDomainSize=(float(X), float(Y), float(Z)), gridResX=X, dt=dt, mu=1.5e-5, den=1.0,
         velBndCondX0=VelBndCond.FixedValue, velBndCondXn=VelBndCond.ZeroGrad, velBndCondY0=VelBndCond.FixedValue,
@@ -113,12 +120,17 @@ def tiny(grid=(16, 8, 12), seed=7, les=True, piso=True, vel_iters=6, pres_iters=
 ENSEMBLE_GRIDS = [(64, 64, 64), (128, 64, 64), (160, 64, 96), (192, 64, 96)]
 
 
-def ensemble(n_cases=24, seed=2, grids=ENSEMBLE_GRIDS, buildings=(5, 30), steps=500) -> list:
+def ensemble(n_cases=24, seed=2, grids=ENSEMBLE_GRIDS, buildings=(5, 30), steps=500,
+             region=((0.2, 0.5), (0.2, 0.8)), centre_x=0.35) -> list:
     """The uneven ensemble: each case draws its grid from `grids` (x, height,
     span: 64^3, 128 x 64 span x 64 high, 160 x 96 x 64, 192 x 96 x 64), a
     building count in `buildings` and an inflow direction 15 deg * U{0..23};
     the inflow always enters at X0, so the building layout is rotated into the
-    solver frame by the negative inflow angle about the domain centre."""
+    solver frame by the negative inflow angle about (centre_x * X, Z / 2).
+    The layout lattice covers `region` (fractions of x and z) around that
+    centre, so every rotation keeps the buildings in the upstream two thirds
+    of the domain: a wake that reaches the zero-gradient outlet draws
+    backflow through it, which the reference's outlet condition lets grow."""
     rng = SplitMix64(seed)
     out = []
     for c in range(n_cases):
@@ -128,14 +140,28 @@ def ensemble(n_cases=24, seed=2, grids=ENSEMBLE_GRIDS, buildings=(5, 30), steps=
         bseed = rng.next()
         X, Y, Z = grid
         lat = (6, 5)
-        boxes = lattice_layout(grid, nb, lat, bseed, foot=(4, 12), height=(4, 24),
-                               region=((0.2, 0.8), (0.2, 0.8)))
+        boxes = lattice_layout(grid, nb, lat, bseed, foot=(4, 12), height=(4, 24), region=region)
         cf = base_config(grid)
-        w = Workload(f"ens{c:02d}_{X}x{Y}x{Z}_{nb}bldg_{int(angle)}deg", cf, rasterise(grid, boxes, angle),
+        w = Workload(f"ens{c:02d}_{X}x{Y}x{Z}_{nb}bldg_{int(angle)}deg", cf,
+                     rasterise(grid, boxes, angle, centre=(centre_x * X, Z / 2.0)),
                      noise_seed=bseed & 0xFFFFFFFF, steps=steps, boxes=boxes)
         w.angle = angle
         out.append(w)
     return out
+
+
+def precision(grid=(384, 128, 256), n=120, lattice=(15, 10), seed=3, height=(4, 40), steps=500, mean_last=250,
+              rounds=3, sweeps=20) -> Workload:
+    """The precision case: 384 x 256 span x 128 high with n seeded buildings
+    of heights U{4..40}; every pressure solve is rounds x sweeps Jacobi
+    sweeps (presMaxNumIter = rounds * sweeps for the plain FP32/FP64 runs,
+    refineRounds / refineSweeps for the low-precision modes)."""
+    boxes = lattice_layout(grid, n, lattice, seed, foot=(4, 12), height=height)
+    cf = base_config(grid, pres_iters=rounds * sweeps)
+    cf.refineRounds, cf.refineSweeps = rounds, sweeps
+    X, Y, Z = grid
+    return Workload(f"precision_{n}bldg_{X}x{Y}x{Z}", cf, rasterise(grid, boxes), noise_seed=seed, steps=steps,
+                    mean_last=mean_last, boxes=boxes)
 
 
 @dataclass
