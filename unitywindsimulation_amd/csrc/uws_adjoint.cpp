@@ -52,6 +52,7 @@ ParamsT<Dual<T>> dual_params(const ParamsT<T>& p) {
     for (int i = 0; i < 6; ++i) { d.velBnd[i] = p.velBnd[i]; d.presBnd[i] = p.presBnd[i]; }
     d.useLES = p.useLES; d.wallSlots = p.wallSlots; d.wallModel = p.wallModel; d.wallZ0 = p.wallZ0;
     d.normRef = p.normRef;
+    d.zchunk = p.zchunk;
     return d;
 }
 

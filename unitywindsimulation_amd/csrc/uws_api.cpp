@@ -102,6 +102,12 @@ void fill_params(const uws_config& c, ParamsT<T>& p) {
     // CSPresNormalization runs only without a Dirichlet pressure face
     // (FvmSolverGpu.cs:1425-1436); its cell (2, Y/2, 2) outside the grid reads 0
     p.normRef = (!anyFixed && 2 < p.X && 2 < p.Z) ? (2 * p.Y + p.Y / 2) * p.X + 2 : -1;
+    // z planes per workgroup of the two-sweep kernels (UWS_ZCHUNK overrides, for tuning)
+    p.zchunk = 8;
+    if (const char* e = std::getenv("UWS_ZCHUNK")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= 1024) p.zchunk = v;
+    }
 }
 template void fill_params<float>(const uws_config&, ParamsT<float>&);
 template void fill_params<double>(const uws_config&, ParamsT<double>&);
@@ -665,7 +671,7 @@ int uws_get_norms(uws_handle h, double* out, int32_t n) {
 }
 
 int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms) {
-    if (!avg_ms || reps <= 0 || (kernel != UWS_KERNEL_VEL_SWEEP && kernel != UWS_KERNEL_PRES_SWEEP))
+    if (!avg_ms || reps <= 0 || kernel < UWS_KERNEL_VEL_SWEEP || kernel > UWS_KERNEL_PRES_SWEEP2)
         return UWS_ERR_INVALID_ARG;
     return with_solver(h, [&](SolverBase& s) { return s.time_kernel(kernel, reps, avg_ms); });
 }

@@ -141,6 +141,7 @@ struct uws_batch {
         }
         int read_norm(int, double*) { return UWS_ERR_UNSUPPORTED; }
         double ncells() const { return 1.0; }
+        bool fused() const { return false; }  // the ensemble runs one-sweep kernels over its tile table
     };
 
     void sync_case_roles() {

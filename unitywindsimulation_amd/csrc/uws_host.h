@@ -99,7 +99,8 @@ int normalise_norms(const double* raw, double ncells, double* out, int n);
 // Step driver.  L is a launcher: l(kernel, roles, slot) enqueues one kernel
 // (for one case, or for every case of an ensemble), l.copy_vel(dst, src)
 // copies a velocity buffer, l.read_norm(slot, &v) synchronises and reads a
-// norm (used only by the residual-based early exit), l.ncells() is N.
+// norm (used only by the residual-based early exit), l.ncells() is N,
+// l.fused() tells whether the two-sweep kernels may be used.
 // ---------------------------------------------------------------------------
 
 // VelPredict() (FvmSolverGpu.cs:1238-1282): V Jacobi sweeps from u^n
@@ -173,12 +174,22 @@ int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest) {
     }
     int in = -1, last_in = -1;
     bool broke = false;
-    for (int k = 0; k < c.P; ++k) {
-        const int out = k & 1;
+    // pairs of sweeps run as one two-sweep launch where the launcher has it;
+    // the residual-based exit and the last sweep's residual need single sweeps
+    const bool residual_exit = coef == 0 && c.cal_residual;
+    const bool last_single = coef == 0 && c.norms;
+    for (int k = 0; k < c.P;) {
+        const int out = in == 0 ? 1 : 0;
         Roles r{};
         r.coef = coef; r.pin = in; r.pout = out;
+        if (l.fused() && !residual_exit && k + 2 <= c.P && !(last_single && k + 2 == c.P)) {
+            l(KN_PRES_SWEEP2, r);
+            in = out;
+            k += 2;
+            continue;
+        }
         l(KN_PRES_SWEEP, r);
-        if (coef == 0 && c.cal_residual && k % c.pci == 0) {
+        if (residual_exit && k % c.pci == 0) {
             l(KN_RES_PRES, r);
             l(KN_SUM_PARTIALS, r, UWS_NORM_PRES_RESIDUAL);
             double sum = 0.0;
@@ -187,6 +198,7 @@ int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest) {
         }
         last_in = in;
         in = out;
+        ++k;
     }
     if (coef == 0 && c.norms && c.P >= 1 && !broke) {
         Roles r{};

@@ -27,6 +27,7 @@ struct ParamsT {
     int wallModel;        // UWS_WALL_SMOOTH / UWS_WALL_ROUGH
     T wallZ0;
     int normRef;          // flat index of the pressure normalisation cell, -1 when it is inactive
+    int zchunk;           // z planes per workgroup of the two-sweep kernels
 };
 
 // Device fields, structure-of-arrays: every vector field is three planes of
@@ -104,7 +105,8 @@ enum Kernel {
     KN_APPLY_PRES, KN_PRES_NORMALISE, KN_VEL_CORRECT, KN_PISO_AODU, KN_PISO_PRECOMP, KN_PISO_CORRECT,
     KN_ZERO_SOLID_VEL, KN_FILL_PC,
     KN_DEFECT_RESIDUAL, KN_LOWP_SWEEP,
-    KN_RES_VEL, KN_RES_PRES, KN_DIVERGENCE, KN_NONFINITE, KN_SUM_PARTIALS, KN_DEFECT_SCALE
+    KN_RES_VEL, KN_RES_PRES, KN_DIVERGENCE, KN_NONFINITE, KN_SUM_PARTIALS, KN_DEFECT_SCALE,
+    KN_PRES_SWEEP2   // two pressure Jacobi sweeps in one launch (single-case only)
 };
 
 // Single-case launch of kernel k on stream s.  `slot` selects the norms

@@ -973,6 +973,135 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// Two pressure Jacobi sweeps in one launch (temporal blocking).  A workgroup
+// owns a 64 x (PS2_BY - 2) x zchunk block of output cells and marches along
+// z: at plane zz every thread of the 66 x PS2_BY tile (the block plus a
+// one-cell halo in x and y) computes the first sweep's value x1, which goes
+// to LDS (three rotating planes, one barrier per plane), and the block's
+// threads compute the second sweep at plane zz - 1 from x1 of the planes
+// zz - 2 .. zz.  The halo's x1 is recomputed by each neighbouring block, so
+// the coefficients are read once per two sweeps.  Both sweeps use the
+// expressions and operation order of body_pres_sweep, so the result equals
+// two single sweeps bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int PS2_W = 66;                   // threads per tile row: 64 outputs + 2 halo
+constexpr int PS2_BY = 15;                  // tile rows: 13 outputs + 2 halo
+constexpr int PS2_THREADS = PS2_W * PS2_BY;  // 990
+
+// one plane's inputs of the first sweep at a cell: its coefficients and the
+// previous iterate at its six neighbours (clamped at the domain boundary,
+// where the coefficient is zero)
+template <class T>
+struct PS2Plane {
+    T cE, cW, cN, cS, cU, cD, inv, rhs;
+    T vE, vW, vN, vS, vU, vD;
+    bool fl;
+};
+
+template <class T, bool ZIN>
+__global__ __launch_bounds__(1024) void k_pres_sweep2(ParamsT<T> p, FieldsT<T> f, Roles r) {
+    __shared__ __align__(16) unsigned char lds_raw[3 * PS2_THREADS * sizeof(T)];
+    T* lds = reinterpret_cast<T*>(lds_raw);
+    const int tid = (int)threadIdx.x;
+    const int lx = tid % PS2_W, ly = tid / PS2_W;
+    const int ntx = (p.X + 63) / 64, nty = (p.Y + PS2_BY - 3) / (PS2_BY - 2);
+    const int ntz = (p.Z + p.zchunk - 1) / p.zchunk;
+    int b = xcd_tile((int)blockIdx.x, ntx * nty * ntz);
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int x = bx * 64 - 1 + lx, y = by * (PS2_BY - 2) - 1 + ly;
+    const int z0 = bz * p.zchunk, z1 = min(z0 + p.zchunk, p.Z);
+    const bool cell = x >= 0 && x < p.X && y >= 0 && y < p.Y;
+    const bool inner = cell && lx >= 1 && lx <= 64 && ly >= 1 && ly <= PS2_BY - 2;
+    const PresCoef<T> k = pres_coef(f, r.coef);
+    const T* __restrict__ x0 = ZIN ? nullptr : f.pc[r.pin];
+    T* __restrict__ out = f.pc[r.pout];
+    const int sy = p.X, sz = p.X * p.Y;
+    const bool hW = x > 0, hE = x + 1 < p.X, hS = y > 0, hN = y + 1 < p.Y;
+    // loads of plane zz, issued one plane ahead of their use so that their
+    // latency overlaps the barrier and the second sweep of the current plane
+    auto load = [&](int zz) {
+        PS2Plane<T> q{};
+        q.fl = false;
+        if (cell && zz >= 0 && zz < p.Z) {
+            const int P = (zz * p.Y + y) * p.X + x;
+            q.fl = f.flag[P] == UWS_CELL_FLUID;
+            if (q.fl) {
+                const bool hD = zz > 0, hU = zz + 1 < p.Z;
+                q.cE = k.cE[P]; q.cN = k.cN[P]; q.cU = k.cU[P];
+                q.cW = hW ? k.cE[P - 1] : (T)0;
+                q.cS = hS ? k.cN[P - sy] : (T)0;
+                q.cD = hD ? k.cU[P - sz] : (T)0;
+                q.inv = k.invDp[P];
+                q.rhs = k.rhs[P];
+                if (!ZIN) {
+                    q.vE = x0[hE ? P + 1 : P];
+                    q.vW = x0[hW ? P - 1 : P];
+                    q.vN = x0[hN ? P + sy : P];
+                    q.vS = x0[hS ? P - sy : P];
+                    q.vU = x0[hU ? P + sz : P];
+                    q.vD = x0[hD ? P - sz : P];
+                }
+            }
+        }
+        return q;
+    };
+    // own-column state: first-sweep values of the two previous planes and the
+    // coefficients of the previous plane (used by the second sweep)
+    T x1m1 = 0, x1m2 = 0;
+    PS2Plane<T> prev{};
+    prev.fl = false;
+    PS2Plane<T> cur = load(z0 - 1);
+    for (int zz = z0 - 1; zz <= z1; ++zz) {
+        const PS2Plane<T> nxt = zz + 1 <= z1 ? load(zz + 1) : PS2Plane<T>{};
+        T x1 = 0;
+        if (cur.fl) {
+            T term = 0;
+            if (!ZIN) {
+                term -= cur.cE * cur.vE;
+                term -= cur.cW * cur.vW;
+                term -= cur.cN * cur.vN;
+                term -= cur.cS * cur.vS;
+                term -= cur.cU * cur.vU;
+                term -= cur.cD * cur.vD;
+            }
+            x1 = cur.inv * (term + cur.rhs);
+        }
+        lds[((zz + 3) % 3) * PS2_THREADS + tid] = x1;
+        __syncthreads();
+        const int zo = zz - 1;
+        if (inner && zo >= z0 && zo < z1) {
+            const int P = (zo * p.Y + y) * p.X + x;
+            T v = 0;
+            if (prev.fl) {
+                const T* L = lds + ((zo + 3) % 3) * PS2_THREADS;
+                const T vE = hE ? L[tid + 1] : x1m1, vW = hW ? L[tid - 1] : x1m1;
+                const T vN = hN ? L[tid + PS2_W] : x1m1, vS = hS ? L[tid - PS2_W] : x1m1;
+                const T vU = zo + 1 < p.Z ? x1 : x1m1, vD = zo > 0 ? x1m2 : x1m1;
+                T term = 0;
+                term -= prev.cE * vE;
+                term -= prev.cW * vW;
+                term -= prev.cN * vN;
+                term -= prev.cS * vS;
+                term -= prev.cU * vU;
+                term -= prev.cD * vD;
+                v = prev.inv * (term + prev.rhs);
+            }
+            out[P] = v;
+        }
+        x1m2 = x1m1;
+        x1m1 = x1;
+        prev = cur;
+        cur = nxt;
+    }
+}
+
+inline int pres_sweep2_blocks(int X, int Y, int Z, int zchunk) {
+    return ((X + 63) / 64) * ((Y + PS2_BY - 3) / (PS2_BY - 2)) * ((Z + zchunk - 1) / zchunk);
+}
+
 // the case owning tile t: the last c with first_tile[c] <= t (binary search)
 template <class T>
 __device__ __forceinline__ int case_of(const BatchTable<T>& tab, int t) {
@@ -1115,6 +1244,12 @@ void go_batch_reduce(const BatchTable<T>& tab, int grid, const Roles& r, hipStre
 
 template <class T>
 void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s, int slot) {
+    if (k == KN_PRES_SWEEP2) {
+        const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+        if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep2<T, true>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+        else hipLaunchKernelGGL((k_pres_sweep2<T, false>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+        return;
+    }
     if (k == KN_SUM_PARTIALS || k == KN_DEFECT_SCALE) {
         hipLaunchKernelGGL(k_single_sum<T>, dim3(1), dim3(TX, 4, 1), 0, s, p, f, slot,
                            k == KN_DEFECT_SCALE ? 1 : 0);

@@ -62,7 +62,14 @@ struct Solver : SolverBase {
             return UWS_OK;
         }
         double ncells() const { return (double)s->p.N; }
+        bool fused() const { return s->fuse_sweeps; }
     };
+    // two-sweep kernels in the step (UWS_SWEEP_FUSION=0 in the environment
+    // selects the one-sweep kernels, for comparisons)
+    bool fuse_sweeps = [] {
+        const char* e = std::getenv("UWS_SWEEP_FUSION");
+        return !(e && e[0] == '0');
+    }();
 
     int init(const uws_config& c) {
         fill_params(c, p);
@@ -369,6 +376,9 @@ struct Solver : SolverBase {
             if (kernel == UWS_KERNEL_VEL_SWEEP) {
                 r.state = rs.state; r.vin = rs.state; r.vout = w0;
                 launch<T>(KN_VEL_SWEEP, p, f, r, stream);
+            } else if (kernel == UWS_KERNEL_PRES_SWEEP2) {
+                r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
+                launch<T>(KN_PRES_SWEEP2, p, f, r, stream);
             } else {
                 r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
                 launch<T>(KN_PRES_SWEEP, p, f, r, stream);
