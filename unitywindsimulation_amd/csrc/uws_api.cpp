@@ -134,6 +134,7 @@ size_t carve(const ParamsT<T>& p, FieldsT<T>& f, char* base, bool lowp) {
     f.p = fl(N);
     f.pc[0] = fl(N); f.pc[1] = fl(N);
     f.nut = fl(N);
+    for (int c = 0; c < 9; ++c) f.grad[c] = fl(N);
     for (int c = 0; c < 3; ++c) f.nuw[c] = fl(N);
     for (int c = 0; c < 3; ++c) f.D[c] = fl(N);
     for (int c = 0; c < 3; ++c) f.invD[c] = fl(N);

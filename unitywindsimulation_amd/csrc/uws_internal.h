@@ -44,6 +44,7 @@ struct FieldsT {
     T* p;                 // p^n (presFieldLastTime)
     T* pc[2];             // p' ping-pong (presCorrectField / LastIter)
     T* nut;               // eddyVisField
+    T* grad[9];           // cell velocity gradient du_i/dx_j (index 3i + j) of fluid cells, from the eddy-viscosity kernel
     T* nuw[3];            // faceEddyVisField slots W(.x) S(.y) D(.z)
     T* D[3];              // DField
     T* invD[3];           // 1/DField (fluid cells; 0 elsewhere)

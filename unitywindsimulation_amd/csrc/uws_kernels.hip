@@ -257,6 +257,11 @@ __device__ void body_wall_eddy(const ParamsT<T>& p, const FieldsT<T>& f, const C
     if ((r.flags & K_EDDY) && isF) {
         T g[3][3];
         cell_grad(p, f, un, x, y, z, g);
+        // kept for the LES term, which needs it at this cell and its neighbours
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) f.grad[3 * i + j][P] = g[i][j];
         const T Sxx = g[0][0], Syy = g[1][1], Szz = g[2][2];
         const T Sxy = (T)0.5f * (g[0][1] + g[1][0]);
         const T Syz = (T)0.5f * (g[1][2] + g[2][1]);
@@ -267,14 +272,25 @@ __device__ void body_wall_eddy(const ParamsT<T>& p, const FieldsT<T>& f, const C
     }
 }
 
-// LES transposed-stress term of one fluid cell (LES.compute:326-683)
+// the cell gradient stored by the eddy-viscosity kernel
+template <class T>
+__device__ __forceinline__ void load_grad(const FieldsT<T>& f, int P, T g[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) g[i][j] = f.grad[3 * i + j][P];
+}
+
+// LES transposed-stress term of one fluid cell (LES.compute:326-683).  The
+// cell gradients of this cell and its fluid neighbours are those the
+// eddy-viscosity kernel computed from the same u^n (cell_grad), read back.
 template <class T>
 __device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], int x, int y,
                                           int z, int P) {
     const V3<T> uP = ld3(un, P);
     const T hdx = (T)0.5f * p.dx;
     T g[3][3];
-    cell_grad(p, f, un, x, y, z, g);
+    load_grad(f, P, g);
     T fg[6][3];
 #pragma unroll
     for (int d = 0; d < 6; ++d) { fg[d][0] = 0; fg[d][1] = 0; fg[d][2] = 0; }
@@ -288,7 +304,7 @@ __device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>&
             const int Q = lin(p, nx, ny, nz);
             if (fluid(f, Q)) {
                 T gn[3][3];
-                cell_grad(p, f, un, nx, ny, nz, gn);
+                load_grad(f, Q, gn);
 #pragma unroll
                 for (int j = 0; j < 3; ++j) fg[d][j] = (T)0.5f * (g[cc][j] + gn[cc][j]);
             } else {
