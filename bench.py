@@ -57,9 +57,11 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
 
 
 def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
-    """The 24-case uneven ensemble: cases/s of one ensemble graph per step
-    against the same cases stepped one after another (each with its own
-    graph), and the FP32 ensemble's velocity error against FP64 runs."""
+    """The 24-case uneven ensemble: cases/s of the batched ensemble (one graph
+    per step over all cases) against the same cases stepped one after another,
+    each with its own graph per step ("plain") and with one host launch per
+    kernel ("eager_plain", how the reference dispatches), and the FP32
+    ensemble's velocity error against FP64 runs."""
     import numpy as np
 
     from unitywindsimulation_amd import FvmSolverHip, workloads
@@ -68,17 +70,30 @@ def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
 
     ws = workloads.ensemble()
     full_steps = ws[0].steps
-    t_plain = 0.0
-    for w in ws:
-        with FvmSolverHip(w.cf) as s:
-            s.InitFlags(w.flags)
-            s.init_noise(w.noise_seed, w.noise_amp)
-            s.step(2)
-            s.synchronize()
-            t0 = time.perf_counter()
-            s.step(steps)
-            s.synchronize()
-            t_plain += (time.perf_counter() - t0) / steps
+
+    def sequential(eager: bool) -> float:
+        """seconds per step of all cases stepped one after another; eager: one host launch per
+        kernel (the reference's dispatch pattern), else one captured graph per step and case"""
+        if eager:
+            os.environ["UWS_NO_GRAPH"] = "1"
+        try:
+            total = 0.0
+            for w in ws:
+                with FvmSolverHip(w.cf) as s:
+                    s.InitFlags(w.flags)
+                    s.init_noise(w.noise_seed, w.noise_amp)
+                    s.step(2)
+                    s.synchronize()
+                    t0 = time.perf_counter()
+                    s.step(steps)
+                    s.synchronize()
+                    total += (time.perf_counter() - t0) / steps
+            return total
+        finally:
+            os.environ.pop("UWS_NO_GRAPH", None)
+
+    t_plain = sequential(eager=False)
+    t_eager = sequential(eager=True)
     with FvmEnsembleHip([w.cf for w in ws]) as e:
         for c, w in enumerate(ws):
             e.init_flags(c, w.flags)
@@ -113,7 +128,10 @@ def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
             "plain_cases_per_s": round(len(ws) / (t_plain * full_steps), 4),
             "batched_cases_per_s": round(len(ws) / (t_batch * full_steps), 4),
             "speedup": round(t_plain / t_batch, 3),
+            "eager_plain_cases_per_s": round(len(ws) / (t_eager * full_steps), 4),
+            "speedup_vs_eager_plain": round(t_eager / t_batch, 3),
             "plain_ms_per_step_all_cases": round(t_plain * 1e3, 3), "batched_ms_per_step": round(t_batch * 1e3, 3),
+            "eager_plain_ms_per_step_all_cases": round(t_eager * 1e3, 3),
             "launch": {"tile_rows": ty, "persistent": pers, "group_MiB": gbytes >> 20, "groups": ngroups},
             "max_rel_l2_u_vs_fp64": max(errs), "compared_after_steps": check_steps}
 
