@@ -67,3 +67,21 @@ def test_ensemble_rejects_mixed_schedules(gpu):
     b.cf.presMaxNumIter = 9
     with pytest.raises(Exception):
         FvmEnsembleHip([a.cf, b.cf])
+
+
+@pytest.mark.parametrize("group_bytes", [0, 1])
+def test_ensemble_two_sweep_kernel_matches_single_cases(gpu, monkeypatch, group_bytes):
+    """With the two-sweep pressure kernel forced at every size, the ensemble's
+    launch over every case's z-marching blocks equals the single cases."""
+    monkeypatch.setenv("UWS_SWEEP_FUSION", "2")
+    ws = small_cases()
+    ref = singles(ws, 3)
+    got = ensemble_run(ws, 3, 4, False, group_bytes=group_bytes)
+    for (ug, pg), (ur, pr) in zip(got, ref):
+        np.testing.assert_array_equal(ug, ur)
+        np.testing.assert_array_equal(pg, pr)
+    # and the one-sweep kernels agree with both
+    monkeypatch.setenv("UWS_SWEEP_FUSION", "0")
+    for (ug, pg), (ur, pr) in zip(singles(ws, 3), ref):
+        np.testing.assert_array_equal(ug, ur)
+        np.testing.assert_array_equal(pg, pr)

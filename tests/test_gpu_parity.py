@@ -143,7 +143,7 @@ def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, pres_iter
     w.cf.computeNorms = norms
     w.cf.Init()
     res = []
-    for fuse in ("1", "0"):
+    for fuse in ("2", "0"):  # 2: the two-sweep kernel at any grid size, 0: one-sweep kernels only
         monkeypatch.setenv("UWS_SWEEP_FUSION", fuse)
         with solver_for(w, p0=seeded_pressure(w)) as s:
             s.step(2)

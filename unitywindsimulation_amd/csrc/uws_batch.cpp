@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -20,10 +21,13 @@ struct BatchGroup {
     BatchTable<float> tab{};
     int grid = 0;
     int* d_first = nullptr;
+    int* d_first2 = nullptr;
     ParamsT<float>* d_params = nullptr;
     FieldsT<float>* d_fields = nullptr;
     void release() {
         if (d_first) (void)hipFree(d_first);
+        if (d_first2) (void)hipFree(d_first2);
+        d_first2 = nullptr;
         if (d_params) (void)hipFree(d_params);
         if (d_fields) (void)hipFree(d_fields);
         d_first = nullptr;
@@ -41,6 +45,7 @@ struct uws_batch {
     int ty = 4;
     int persistent = 0;
     int64_t group_bytes = 0;  // <= 0: one group for the whole ensemble
+    int fuse_mode = fusion_mode();  // two-sweep pressure kernel: see fusion_mode()
     std::vector<BatchGroup> groups;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
 
@@ -84,19 +89,23 @@ struct uws_batch {
         }
         for (auto& g : groups) {
             const int m = (int)g.cases.size();
-            std::vector<int> first(m + 1, 0);
+            std::vector<int> first(m + 1, 0), first2(m + 1, 0);
             std::vector<ParamsT<float>> ps(m);
             std::vector<FieldsT<float>> fs(m);
             for (int i = 0; i < m; ++i) {
                 ps[i] = params(g.cases[i]);
                 fs[i] = fields(g.cases[i]);
                 first[i + 1] = first[i] + ps[i].tiles;
+                first2[i + 1] = first2[i] + pres_sweep2_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk);
             }
             if (hipMalloc(&g.d_first, sizeof(int) * (m + 1)) != hipSuccess ||
+                hipMalloc(&g.d_first2, sizeof(int) * (m + 1)) != hipSuccess ||
                 hipMalloc(&g.d_params, sizeof(ParamsT<float>) * m) != hipSuccess ||
                 hipMalloc(&g.d_fields, sizeof(FieldsT<float>) * m) != hipSuccess)
                 return UWS_ERR_ALLOC;
             if (hipMemcpyAsync(g.d_first, first.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
+                    hipSuccess ||
+                hipMemcpyAsync(g.d_first2, first2.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
                 hipMemcpyAsync(g.d_params, ps.data(), sizeof(ParamsT<float>) * m, hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
@@ -108,6 +117,8 @@ struct uws_batch {
             g.tab.ty = ty;
             g.tab.persistent = persistent;
             g.tab.first_tile = g.d_first;
+            g.tab.total_ps2 = first2[m];
+            g.tab.first_ps2 = g.d_first2;
             g.tab.params = g.d_params;
             g.tab.fields = g.d_fields;
             if (persistent) {
@@ -141,7 +152,7 @@ struct uws_batch {
         }
         int read_norm(int, double*) { return UWS_ERR_UNSUPPORTED; }
         double ncells() const { return 1.0; }
-        bool fused() const { return false; }  // the ensemble runs one-sweep kernels over its tile table
+        bool fused() const { return use_fusion(b->fuse_mode, g->tab.total_ps2); }
     };
 
     void sync_case_roles() {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -75,6 +76,20 @@ struct SolverBase {
     virtual int get_mean(double* dst, int64_t n, int64_t* count) = 0;
     virtual int cstep_imag(bool set, int what, int face, double* buf, int64_t n) = 0;
 };
+// The two-sweep pressure kernel marches z-chunks of 64 x 13-cell tiles; on a
+// grid with fewer workgroups than two per CU (256 CUs) it leaves the chip
+// idle and two one-sweep launches are faster.
+constexpr int FUSE_MIN_BLOCKS = 512;
+// UWS_SWEEP_FUSION in the environment: 0 = one-sweep kernels only, 2 = the
+// two-sweep kernel at any grid size (for tests), otherwise by grid size
+inline int fusion_mode() {
+    const char* e = std::getenv("UWS_SWEEP_FUSION");
+    if (e && e[0] == '0') return 0;
+    if (e && e[0] == '2') return 2;
+    return 1;
+}
+inline bool use_fusion(int mode, int blocks) { return mode == 2 || (mode == 1 && blocks >= FUSE_MIN_BLOCKS); }
+
 // runs whose host-side profiles are evaluated in double
 inline bool wide_host(int precision) { return precision == UWS_PREC_FP64 || precision == UWS_PREC_CSTEP; }
 

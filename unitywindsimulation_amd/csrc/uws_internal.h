@@ -77,6 +77,8 @@ struct BatchTable {
     int ty;                     // workgroup rows (every case uses the same)
     int persistent;             // 1: grid-stride launch of a fixed number of workgroups
     const int* first_tile;      // [ncases + 1]
+    int total_ps2;              // workgroups of the two-sweep pressure kernel over all cases
+    const int* first_ps2;       // [ncases + 1]: first two-sweep workgroup of each case
     const ParamsT<T>* params;   // [ncases]
     const FieldsT<T>* fields;   // [ncases]
 };
@@ -122,6 +124,9 @@ void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, 
 // sum[c][i] += u[c][i] in FP64 (time averaging of the velocity)
 template <class T>
 void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s);
+
+// workgroups of the two-sweep pressure kernel for one grid
+int pres_sweep2_grid(int X, int Y, int Z, int zchunk);
 
 inline int tiles_of(int X, int Y, int Z, int ty) { return ((X + 63) / 64) * ((Y + ty - 1) / ty) * Z; }
 

@@ -1015,15 +1015,17 @@ struct PS2Plane {
     bool fl;
 };
 
+inline __host__ __device__ int pres_sweep2_blocks(int X, int Y, int Z, int zchunk) {
+    return ((X + 63) / 64) * ((Y + PS2_BY - 3) / (PS2_BY - 2)) * ((Z + zchunk - 1) / zchunk);
+}
+
+// the two-sweep block b of one grid (b in [0, pres_sweep2_blocks))
 template <class T, bool ZIN>
-__global__ __launch_bounds__(1024) void k_pres_sweep2(ParamsT<T> p, FieldsT<T> f, Roles r) {
-    __shared__ __align__(16) unsigned char lds_raw[3 * PS2_THREADS * sizeof(T)];
-    T* lds = reinterpret_cast<T*>(lds_raw);
+__device__ __forceinline__ void pres_sweep2_block(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, int b,
+                                                  T* lds) {
     const int tid = (int)threadIdx.x;
     const int lx = tid % PS2_W, ly = tid / PS2_W;
     const int ntx = (p.X + 63) / 64, nty = (p.Y + PS2_BY - 3) / (PS2_BY - 2);
-    const int ntz = (p.Z + p.zchunk - 1) / p.zchunk;
-    int b = xcd_tile((int)blockIdx.x, ntx * nty * ntz);
     const int bx = b % ntx;
     b /= ntx;
     const int by = b % nty, bz = b / nty;
@@ -1114,9 +1116,13 @@ __global__ __launch_bounds__(1024) void k_pres_sweep2(ParamsT<T> p, FieldsT<T> f
     }
 }
 
-inline int pres_sweep2_blocks(int X, int Y, int Z, int zchunk) {
-    return ((X + 63) / 64) * ((Y + PS2_BY - 3) / (PS2_BY - 2)) * ((Z + zchunk - 1) / zchunk);
+template <class T, bool ZIN>
+__global__ __launch_bounds__(1024) void k_pres_sweep2(ParamsT<T> p, FieldsT<T> f, Roles r) {
+    __shared__ __align__(16) unsigned char lds_raw[3 * PS2_THREADS * sizeof(T)];
+    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+    pres_sweep2_block<T, ZIN>(p, f, r, xcd_tile((int)blockIdx.x, nb), reinterpret_cast<T*>(lds_raw));
 }
+
 
 // the case owning tile t: the last c with first_tile[c] <= t (binary search)
 template <class T>
@@ -1127,6 +1133,20 @@ __device__ __forceinline__ int case_of(const BatchTable<T>& tab, int t) {
         if (tab.first_tile[mid] <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
+}
+
+// two-sweep pressure kernel over every case of an ensemble: workgroup t
+// belongs to the case c with first_ps2[c] <= t < first_ps2[c + 1]
+template <class T, bool ZIN>
+__global__ __launch_bounds__(1024) void k_batch_pres_sweep2(BatchTable<T> tab, Roles r) {
+    __shared__ __align__(16) unsigned char lds_raw[3 * PS2_THREADS * sizeof(T)];
+    const int t = xcd_tile((int)blockIdx.x, tab.total_ps2);
+    int lo = 0, hi = tab.ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab.first_ps2[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    pres_sweep2_block<T, ZIN>(tab.params[lo], tab.fields[lo], r, t - tab.first_ps2[lo], reinterpret_cast<T*>(lds_raw));
 }
 
 // Persistent ensemble launch: the grid is a multiple of 8 workgroups, XCD j
@@ -1280,6 +1300,13 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
 
 template <class T>
 void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot) {
+    if (k == KN_PRES_SWEEP2) {
+        if (r.pin < 0)
+            hipLaunchKernelGGL((k_batch_pres_sweep2<T, true>), dim3(tab.total_ps2), dim3(PS2_THREADS), 0, s, tab, r);
+        else
+            hipLaunchKernelGGL((k_batch_pres_sweep2<T, false>), dim3(tab.total_ps2), dim3(PS2_THREADS), 0, s, tab, r);
+        return;
+    }
     if (k == KN_SUM_PARTIALS || k == KN_DEFECT_SCALE) {
         hipLaunchKernelGGL(k_batch_sum<T>, dim3(tab.ncases), dim3(TX, 4, 1), 0, s, tab, slot,
                            k == KN_DEFECT_SCALE ? 1 : 0);
@@ -1303,6 +1330,8 @@ __global__ __launch_bounds__(256) void k_mean_accumulate(const T* u0, const T* u
     }
 }
 }  // namespace
+
+int pres_sweep2_grid(int X, int Y, int Z, int zchunk) { return pres_sweep2_blocks(X, Y, Z, zchunk); }
 
 template <class T>
 void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s) {

@@ -62,14 +62,11 @@ struct Solver : SolverBase {
             return UWS_OK;
         }
         double ncells() const { return (double)s->p.N; }
-        bool fused() const { return s->fuse_sweeps; }
+        bool fused() const {
+            return use_fusion(s->fuse_mode, pres_sweep2_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk));
+        }
     };
-    // two-sweep kernels in the step (UWS_SWEEP_FUSION=0 in the environment
-    // selects the one-sweep kernels, for comparisons)
-    bool fuse_sweeps = [] {
-        const char* e = std::getenv("UWS_SWEEP_FUSION");
-        return !(e && e[0] == '0');
-    }();
+    int fuse_mode = fusion_mode();  // two-sweep pressure kernel: see fusion_mode()
 
     int init(const uws_config& c) {
         fill_params(c, p);
