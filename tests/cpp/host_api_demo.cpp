@@ -45,6 +45,38 @@ int main() {
         std::printf("steps %d t %.3f solid cells %ld max|u| %.4f max|p| %.4f\n", cf.currSimStep, cf.currPhyTime,
                     solid, umax, pmax);
         if (solid != 4 * 8 * 4 || !(umax > 0 && umax < 100) || !std::isfinite(pmax)) return 3;
+
+        // forward/backward: dJ/dCs of J = sum of u_x after 3 steps (FP64 run),
+        // checked against a central difference
+        FluidSimConfig cd = cf;
+        cd.precision = UWS_PREC_FP64;
+        RuntimeConfig rd(cd);
+        auto J = [&](double csv, double* grad_cs) {
+            FvmSolverHip sd(cd, rd);
+            sd.SetSmagorinskyConstant(csv);
+            double j = 0;
+            if (grad_cs) {
+                FvmAdjointHip adj(sd, 3, 2);
+                adj.Forward();
+                const auto u = sd.GetVelFieldF64();
+                std::vector<double> lam(u.size(), 0.0);
+                for (size_t i = 0; i < u.size(); i += 3) { lam[i] = 1.0; j += u[i]; }
+                adj.Backward(lam);
+                *grad_cs = adj.GradCs();
+                std::printf("adjoint device memory %.1f MB\n", adj.DeviceBytes() / 1e6);
+            } else {
+                for (int i = 0; i < 3; ++i) sd.Step();
+                const auto u = sd.GetVelFieldF64();
+                for (size_t i = 0; i < u.size(); i += 3) j += u[i];
+            }
+            return j;
+        };
+        double g = 0;
+        J(0.18, &g);
+        const double h = 1e-5;
+        const double fd = (J(0.18 + h, nullptr) - J(0.18 - h, nullptr)) / (2 * h);
+        std::printf("dJ/dCs adjoint %.10e central difference %.10e\n", g, fd);
+        if (!(std::fabs(g - fd) <= 1e-4 * std::fabs(fd) + 1e-4)) return 4;
     } catch (const Error& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

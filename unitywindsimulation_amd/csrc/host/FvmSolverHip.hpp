@@ -229,11 +229,75 @@ public:
     uws_handle handle() const { return h_; }
     int64_t numCells() const { return n_; }
 
+    // ---- extensions used by the differentiation API ----
+    int64_t BndSize(int face) const {
+        const int32_t X = cf_.gridRes.x, Y = cf_.gridRes.y, Z = cf_.gridRes.z;
+        return (int64_t)(face < 2 ? Y : X) * (face < 4 ? Z : Y);
+    }
+    // one boundary texture, [j][i][3] doubles (exact for FP64 runs)
+    void SetBndField(int face, const std::vector<double>& xyz) {
+        check(uws_set_bnd_field_f64(h_, face, xyz.data(), (int64_t)xyz.size() / 3), "uws_set_bnd_field_f64");
+    }
+    void SetSmagorinskyConstant(double cs) {
+        check(uws_set_smagorinsky_constant(h_, cs), "uws_set_smagorinsky_constant");
+        cf_.smagorinskyConstant = (float)cs;
+    }
+    // u^n as [cell][3] doubles
+    std::vector<double> GetVelFieldF64() {
+        std::vector<double> v((size_t)n_ * 3);
+        check(uws_get_field_f64(h_, UWS_FIELD_VEL, v.data(), (int64_t)v.size()), "uws_get_field_f64");
+        return v;
+    }
+
 private:
     FluidSimConfig& cf_;
     RuntimeConfig& rcf_;
     uws_handle h_ = nullptr;
     int64_t n_ = 0;
+};
+
+// Discrete adjoint of nsteps steps of a FvmSolverHip (uws_adjoint_*): the
+// forward/backward API a host exposes for gradient-based design.  Forward()
+// steps the solver (which ends at the final state) keeping a checkpoint every
+// ckptEvery steps; Backward(dJ/du_T, dJ/dp_T) reverses them; the gradients
+// w.r.t. the initial state, any boundary texture and Cs are then read.  The
+// solver must outlive the adjoint.
+class FvmAdjointHip {
+public:
+    FvmAdjointHip(FvmSolverHip& s, int nsteps, int ckptEvery = 8) : s_(s) {
+        check(uws_adjoint_create(s.handle(), nsteps, ckptEvery, &a_), "uws_adjoint_create");
+    }
+    ~FvmAdjointHip() {
+        if (a_) uws_adjoint_destroy(a_);
+    }
+    FvmAdjointHip(const FvmAdjointHip&) = delete;
+    FvmAdjointHip& operator=(const FvmAdjointHip&) = delete;
+
+    void Forward() { check(uws_adjoint_forward(a_), "uws_adjoint_forward"); }
+    // dJdu: [cell][3]; dJdp: [cell] or empty for zero
+    void Backward(const std::vector<double>& dJdu, const std::vector<double>& dJdp = {}) {
+        check(uws_adjoint_backward(a_, dJdu.data(), (int64_t)dJdu.size(), dJdp.empty() ? nullptr : dJdp.data(),
+                                   (int64_t)dJdp.size()),
+              "uws_adjoint_backward");
+    }
+    std::vector<double> GradVelocity0() { return grad(UWS_GRAD_VEL0, 0, 3 * s_.numCells()); }
+    std::vector<double> GradPressure0() { return grad(UWS_GRAD_PRES0, 0, s_.numCells()); }
+    std::vector<double> GradBnd(int face) { return grad(UWS_GRAD_BND, face, 3 * s_.BndSize(face)); }
+    double GradCs() { return grad(UWS_GRAD_CS, 0, 1)[0]; }
+    int64_t DeviceBytes() {
+        int64_t b = 0;
+        check(uws_adjoint_memory(a_, &b), "uws_adjoint_memory");
+        return b;
+    }
+
+private:
+    std::vector<double> grad(int what, int face, int64_t n) {
+        std::vector<double> g((size_t)n);
+        check(uws_adjoint_get_grad(a_, what, face, g.data(), n), "uws_adjoint_get_grad");
+        return g;
+    }
+    FvmSolverHip& s_;
+    uws_adjoint_handle a_ = nullptr;
 };
 
 }  // namespace uws
