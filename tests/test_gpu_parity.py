@@ -128,18 +128,20 @@ def test_deterministic(gpu):
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("name,pres_iters,norms", [("tiny", 8, False), ("tiny", 7, False), ("tiny", 8, True),
-                                                    ("isolated_building", 5, False)])
-def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, pres_iters, norms):
+@pytest.mark.parametrize("name,vel_iters,pres_iters,norms", [("tiny", 6, 8, False), ("tiny", 7, 7, False),
+                                                              ("tiny", 6, 8, True), ("tiny", 5, 8, True),
+                                                              ("tiny", 2, 2, False), ("isolated_building", 4, 5, False)])
+def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters, pres_iters, norms):
     """The two-sweep pressure kernel (temporal blocking over z-marching tiles)
     gives bit-identical fields to the one-sweep kernel: odd sweep counts end
-    with a single sweep, and with norms the last sweep stays single."""
+    with a single sweep and with norms the last sweep stays single; momentum
+    sweep counts vary alongside."""
     if name == "tiny":
-        w = workloads.tiny(pres_iters=pres_iters)
+        w = workloads.tiny(vel_iters=vel_iters, pres_iters=pres_iters)
     else:
         w = workloads.isolated_building()
         w.cf.presMaxNumIter = pres_iters
-        w.cf.velMaxNumIter = 4
+        w.cf.velMaxNumIter = vel_iters
     w.cf.computeNorms = norms
     w.cf.Init()
     res = []
