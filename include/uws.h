@@ -162,6 +162,27 @@ int uws_init_vel_pres_fields(uws_handle h, int32_t inflow_type, float wind_speed
 int uws_set_fixed_value_vel_bnd_cond(uws_handle h, int32_t inflow_type, float wind_speed,
                                      const float vel_zn[3], float ref_height, float alpha);
 
+/* IFvmSolver.InitVelPresFieldsFromBackGroundFlow (FvmSolverGpu.cs:898-933,
+ * kernel CSInitVelPresFieldsFromBGFlowRotate, InitShaderColl.compute:184-226):
+ * warm start from a background flow on another grid.  Every cell's offset
+ * from this grid's centre, (i - gridRes/2) dx, is rotated by field_rot_deg
+ * (RuntimeConfig.flowFieldOrientation) about y into world coordinates around
+ * phys_field_pos (FluidSimConfig.physFieldPos), rotated back into the
+ * background grid's frame around bg->phys_field_pos and sampled there with
+ * trilinear filtering and clamped addressing.  All velocity buffers get the
+ * sampled velocity (not rotated, as in the reference), p^n the sampled
+ * pressure, p' and u' zero; flags are not looked at.  bg_vel is the
+ * background's u in [z][y][x][ncomp] (ncomp 3 or 4, as uws_get_velocity
+ * returns it), bg_pres its p in [z][y][x] or NULL for zero. */
+typedef struct uws_background {
+    int32_t grid_res[3];
+    float dx;
+    float phys_field_pos[3];
+} uws_background;
+int uws_init_vel_pres_fields_from_background(uws_handle h, const float phys_field_pos[3], float field_rot_deg,
+                                             const uws_background* bg, const float* bg_vel, int32_t ncomp,
+                                             const float* bg_pres);
+
 /* Upload one boundary texture (face UWS_X0..UWS_ZN) as [j][i][3] float where
  * (i, j) are the reference texture coordinates: X faces (y, z), Y faces (x, z),
  * Z faces (x, y) (FvmSolverGpu.cs:295-300).  Download mirrors it. */

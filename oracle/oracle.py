@@ -157,3 +157,64 @@ class Oracle:
         s, t = c_int64(), c_double()
         self.L.uwso_time(self.h, ctypes.byref(s), ctypes.byref(t))
         return s.value, t.value
+
+
+def background_sample(fg_grid, fg_dx, fg_pos, rot_deg, bg_grid, bg_dx, bg_pos, bg_vel, bg_pres=None,
+                      dtype=np.float32):
+    """CPU restatement of CSInitVelPresFieldsFromBGFlowRotate
+    (InitShaderColl.compute:184-226; angle formed as in FvmSolverGpu.cs:904,
+    flowFieldOrientation * (float)(3.1415926 / 180)): for every foreground
+    cell, rotate its offset from the grid centre (integer gridRes / 2) by the
+    angle about y, translate to world coordinates, rotate back into the
+    background frame and sample the background fields with trilinear
+    filtering and clamped addressing (texel centres at (i + 1/2) / N, lerp
+    along x, then y, then z).  Arithmetic in `dtype`.  bg_vel: [z][y][x][3|4],
+    bg_pres: [z][y][x] or None.  Returns (u [N, 3], p [N])."""
+    t = np.dtype(dtype).type
+    X, Y, Z = fg_grid
+    bX, bY, bZ = bg_grid
+    ang = t(np.float32(rot_deg) * np.float32(3.1415926 / 180))
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    dxf = t(fg_dx)
+    lx = (xx - X // 2).astype(t) * dxf
+    ly = (yy - Y // 2).astype(t) * dxf
+    lz = (zz - Z // 2).astype(t) * dxf
+    s, c = t(np.sin(ang)), t(np.cos(ang))
+    gx = t(fg_pos[0]) + (lx * c + lz * s)
+    gy = t(fg_pos[1]) + ly
+    gz = t(fg_pos[2]) + (lz * c - lx * s)
+    s, c = t(np.sin(-ang)), t(np.cos(-ang))
+    rx, ry, rz = gx - t(bg_pos[0]), gy - t(bg_pos[1]), gz - t(bg_pos[2])
+    bxp, byp, bzp = rx * c + rz * s, ry, rz * c - rx * s
+    half = t(0.5)
+    u = bxp / (t(bX) * t(bg_dx)) + half
+    v = byp / (t(bY) * t(bg_dx)) + half
+    w = bzp / (t(bZ) * t(bg_dx)) + half
+
+    def axis(q, n):
+        tq = q * t(n) - half
+        f0 = np.floor(tq)
+        a = (tq - f0).astype(t)
+        i0 = f0.astype(np.int64)
+        return np.clip(i0, 0, n - 1), np.clip(i0 + 1, 0, n - 1), a
+
+    x0, x1, ax = axis(u, bX)
+    y0, y1, ay = axis(v, bY)
+    z0, z1, az = axis(w, bZ)
+    one = t(1)
+
+    def samp(a3):
+        a3 = np.asarray(a3, dtype=t).reshape(bZ, bY, bX)
+        bx_, cx_, by_, cy_, bz_, cz_ = one - ax, ax, one - ay, ay, one - az, az
+        c00 = a3[z0, y0, x0] * bx_ + a3[z0, y0, x1] * cx_
+        c10 = a3[z0, y1, x0] * bx_ + a3[z0, y1, x1] * cx_
+        c01 = a3[z1, y0, x0] * bx_ + a3[z1, y0, x1] * cx_
+        c11 = a3[z1, y1, x0] * bx_ + a3[z1, y1, x1] * cx_
+        c0 = c00 * by_ + c10 * cy_
+        c1 = c01 * by_ + c11 * cy_
+        return (c0 * bz_ + c1 * cz_).reshape(-1)
+
+    bv = np.asarray(bg_vel).reshape(bZ * bY * bX, -1)
+    uo = np.stack([samp(bv[:, k]) for k in range(3)], axis=1)
+    po = samp(bg_pres) if bg_pres is not None else np.zeros(X * Y * Z, dtype=t)
+    return uo, po

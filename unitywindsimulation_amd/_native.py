@@ -38,6 +38,11 @@ NORM_VEL_RESIDUAL, NORM_PRES_RESIDUAL, NORM_DIVERGENCE, NORM_NONFINITE, NORM_REF
  STAGE_PRES_SWEEPS, STAGE_APPLY_PRES, STAGE_VEL_CORRECT, STAGE_PISO, STAGE_FINISH) = range(11)
 
 
+class uws_background(ctypes.Structure):
+    """Mirror of ``struct uws_background`` (include/uws.h)."""
+    _fields_ = [("grid_res", c_int32 * 3), ("dx", c_float), ("phys_field_pos", c_float * 3)]
+
+
 class uws_config(ctypes.Structure):
     """Mirror of ``struct uws_config`` (include/uws.h)."""
 
@@ -125,6 +130,8 @@ SIGNATURES = {
     "uws_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int64]),
     "uws_batch_num_groups": (c_int, [c_void_p, POINTER(c_int32)]),
     "uws_batch_autotune": (c_int, [c_void_p, c_int32, c_void_p, POINTER(c_double)]),
+    "uws_init_vel_pres_fields_from_background": (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_int32,
+                                                         c_void_p]),
     "uws_mean_reset": (c_int, [c_void_p]),
     "uws_mean_accumulate": (c_int, [c_void_p]),
     "uws_get_mean_velocity": (c_int, [c_void_p, c_void_p, c_int64, POINTER(c_int64)]),

@@ -95,6 +95,7 @@ class FluidSimConfig:
     presBndCondZn: PresBndCond = PresBndCond.ZeroGrad
     velX0: tuple = (0.0, 0.0, 0.0)
     velZn: tuple = (0.0, 0.0, 0.0)
+    physFieldPos: tuple = (0.0, 0.0, 0.0)   # world position of the grid centre (FluidSimConfig.cs:256)
     solidType: SolidType = SolidType.NoSolid
     fvmSolverType: FvmSolverType = FvmSolverType.SIMPLE
     PISONumCorrectors: int = 2
@@ -216,7 +217,7 @@ _ENUM_FIELDS = {
     "solidType": SolidType, "fvmSolverType": FvmSolverType, "turbulenceModel": TurbulenceModel,
     "inflowType": InflowType, "gridType": GridType,
 }
-_VEC_FIELDS = ("physDomainSize", "externalForce", "velX0", "velZn")
+_VEC_FIELDS = ("physDomainSize", "externalForce", "velX0", "velZn", "physFieldPos")
 _INT_FIELDS = ("gridResX", "PISONumCorrectors", "velResidualCheckInterval", "velMaxNumIter",
                "presResidualCheckInterval", "presMaxNumIter")
 _FLOAT_FIELDS = ("dt", "mu", "den", "smagorinskyConstant", "velTolerance", "presTolerance")

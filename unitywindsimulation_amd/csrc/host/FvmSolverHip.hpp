@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -56,6 +57,7 @@ struct FluidSimConfig {
                 presBndCondZ0 = PresBndCond::ZeroGrad, presBndCondZn = PresBndCond::ZeroGrad;
     float3 velX0{0.f, 0.f, 0.f};
     float3 velZn{0.f, 0.f, 0.f};
+    float3 physFieldPos{0.f, 0.f, 0.f};  // world position of the grid centre (FluidSimConfig.cs:256)
     SolidType solidType = SolidType::NoSolid;
     FvmSolverType fvmSolverType = FvmSolverType::SIMPLE;
     int32_t PISONumCorrectors = 2;
@@ -196,9 +198,25 @@ public:
         else
             check(uws_init_vel_pres_fields(h_, (int32_t)cf_.inflowType, rcf_.windSpeed), "uws_init_vel_pres_fields");
     }
-    void InitVelPresFieldsFromBackGroundFlow(const FluidSimConfig&, const std::vector<float>&,
-                                             const std::vector<float>&) override {
-        throw Error(UWS_ERR_UNSUPPORTED, "InitVelPresFieldsFromBackGroundFlow");
+    // velTexBackGround: the background's velocity [z][y][x][4] (its GetVelField()),
+    // presTexBackGround: its pressure [z][y][x] (or empty)
+    void InitVelPresFieldsFromBackGroundFlow(const FluidSimConfig& cfBackGround, const std::vector<float>& velTexBackGround,
+                                             const std::vector<float>& presTexBackGround) override {
+        uws_background bg{};
+        bg.grid_res[0] = cfBackGround.gridRes.x; bg.grid_res[1] = cfBackGround.gridRes.y;
+        bg.grid_res[2] = cfBackGround.gridRes.z;
+        bg.dx = cfBackGround.dx;
+        bg.phys_field_pos[0] = cfBackGround.physFieldPos.x; bg.phys_field_pos[1] = cfBackGround.physFieldPos.y;
+        bg.phys_field_pos[2] = cfBackGround.physFieldPos.z;
+        const int64_t nb = (int64_t)bg.grid_res[0] * bg.grid_res[1] * bg.grid_res[2];
+        if ((int64_t)velTexBackGround.size() != 4 * nb ||
+            (!presTexBackGround.empty() && (int64_t)presTexBackGround.size() != nb))
+            throw Error(UWS_ERR_INVALID_ARG, "InitVelPresFieldsFromBackGroundFlow");
+        const float pos[3] = {cf_.physFieldPos.x, cf_.physFieldPos.y, cf_.physFieldPos.z};
+        check(uws_init_vel_pres_fields_from_background(h_, pos, rcf_.flowFieldOrientation, &bg,
+                                                       velTexBackGround.data(), 4,
+                                                       presTexBackGround.empty() ? nullptr : presTexBackGround.data()),
+              "uws_init_vel_pres_fields_from_background");
     }
     void SetFixedValueVelBndCond() override {
         const float vz[3] = {cf_.velZn.x, cf_.velZn.y, cf_.velZn.z};
@@ -255,6 +273,25 @@ private:
     uws_handle h_ = nullptr;
     int64_t n_ = 0;
 };
+
+// A mask written by the reference's FlagExporter (FlagExporter.cs:13-34): the
+// flat index z*X*Y + y*X + x of every solid cell, one per line; returns the
+// flag array InitFlags takes.
+inline std::vector<int32_t> ReadFlagFile(const std::string& path, int64_t ncells) {
+    std::vector<int32_t> flags((size_t)ncells, 0);
+    FILE* fh = std::fopen(path.c_str(), "r");
+    if (!fh) throw Error(UWS_ERR_INVALID_ARG, "ReadFlagFile: cannot open " + path);
+    long long i = 0;
+    while (std::fscanf(fh, "%lld", &i) == 1) {
+        if (i < 0 || i >= ncells) {
+            std::fclose(fh);
+            throw Error(UWS_ERR_INVALID_ARG, "ReadFlagFile: index outside the grid");
+        }
+        flags[(size_t)i] = 1;
+    }
+    std::fclose(fh);
+    return flags;
+}
 
 // Discrete adjoint of nsteps steps of a FvmSolverHip (uws_adjoint_*): the
 // forward/backward API a host exposes for gradient-based design.  Forward()

@@ -523,6 +523,21 @@ int uws_set_bnd_field_f64(uws_handle h, int32_t face, const double* xyz, int64_t
     });
 }
 
+int uws_init_vel_pres_fields_from_background(uws_handle h, const float phys_field_pos[3], float field_rot_deg,
+                                             const uws_background* bg, const float* bg_vel, int32_t ncomp,
+                                             const float* bg_pres) {
+    if (!phys_field_pos || !bg || !bg_vel || (ncomp != 3 && ncomp != 4) || !std::isfinite(field_rot_deg))
+        return UWS_ERR_INVALID_ARG;
+    for (int i = 0; i < 3; ++i)
+        if (bg->grid_res[i] < 1 || !std::isfinite(bg->phys_field_pos[i]) || !std::isfinite(phys_field_pos[i]))
+            return UWS_ERR_INVALID_ARG;
+    if (!(bg->dx > 0.f) || (long long)bg->grid_res[0] * bg->grid_res[1] * bg->grid_res[2] >= (1LL << 29))
+        return UWS_ERR_INVALID_ARG;
+    return with_solver(h, [&](SolverBase& s) {
+        return s.init_from_background(phys_field_pos, field_rot_deg, *bg, bg_vel, ncomp, bg_pres);
+    });
+}
+
 int uws_mean_reset(uws_handle h) {
     return with_solver(h, [&](SolverBase& s) { return s.mean_reset(); });
 }

@@ -121,6 +121,21 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
 template <class T>
 void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot = 0);
 
+// A background flow field resident on the device, for the warm start of
+// InitVelPresFieldsFromBackGroundFlow (geometry in R, values in T).
+template <class T, class R>
+struct BgField {
+    int X, Y, Z;
+    R dx;
+    R pos[3];        // physFieldPos of the background grid (its centre)
+    R fgPos[3];      // physFieldPos of this grid
+    R angle;         // flow-field orientation [rad]
+    const T* vel[3];
+    const T* pres;   // may be null (zero)
+};
+template <class T, class R>
+void init_from_background(const ParamsT<T>& p, const FieldsT<T>& f, const BgField<T, R>& bg, hipStream_t s);
+
 // sum[c][i] += u[c][i] in FP64 (time averaging of the velocity)
 template <class T>
 void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s);

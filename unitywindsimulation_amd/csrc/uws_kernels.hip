@@ -1333,6 +1333,79 @@ __global__ __launch_bounds__(256) void k_mean_accumulate(const T* u0, const T* u
 
 int pres_sweep2_grid(int X, int Y, int Z, int zchunk) { return pres_sweep2_blocks(X, Y, Z, zchunk); }
 
+// ---------------------------------------------------------------------------
+// CSInitVelPresFieldsFromBGFlowRotate (InitShaderColl.compute:184-226), the
+// kernel FvmSolverGpu binds for InitVelPresFieldsFromBackGroundFlow
+// (FvmSolverGpu.cs:335, 898-933): the cell's offset from the grid centre is
+// rotated by the flow orientation into world coordinates, rotated back by
+// the same angle into the background frame and sampled there with trilinear
+// filtering and clamped addressing (texel centres at (i + 1/2) / N).  The
+// sampled velocity vector itself is not rotated, as in the reference.  All
+// velocity buffers get the sample, p^n the sampled pressure, p' and u' zero;
+// flags are not looked at.
+// ---------------------------------------------------------------------------
+namespace {
+template <class T, class R>
+__device__ __forceinline__ T sample_trilinear(const T* a, int X, int Y, int Z, R u, R v, R w) {
+    const R tx = u * (R)X - (R)0.5f, ty = v * (R)Y - (R)0.5f, tz = w * (R)Z - (R)0.5f;
+    const R fx0 = floor(tx), fy0 = floor(ty), fz0 = floor(tz);
+    const R ax = tx - fx0, ay = ty - fy0, az = tz - fz0;
+    const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
+    const int x0 = min(max(ix, 0), X - 1), x1 = min(max(ix + 1, 0), X - 1);
+    const int y0 = min(max(iy, 0), Y - 1), y1 = min(max(iy + 1, 0), Y - 1);
+    const int z0 = min(max(iz, 0), Z - 1), z1 = min(max(iz + 1, 0), Z - 1);
+    auto at = [&](int x, int y, int z) { return a[((size_t)z * Y + y) * X + x]; };
+    const T bx = (T)((R)1 - ax), cx = (T)ax, by = (T)((R)1 - ay), cy = (T)ay, bz = (T)((R)1 - az), cz = (T)az;
+    const T c00 = at(x0, y0, z0) * bx + at(x1, y0, z0) * cx;
+    const T c10 = at(x0, y1, z0) * bx + at(x1, y1, z0) * cx;
+    const T c01 = at(x0, y0, z1) * bx + at(x1, y0, z1) * cx;
+    const T c11 = at(x0, y1, z1) * bx + at(x1, y1, z1) * cx;
+    const T c0 = c00 * by + c10 * cy, c1 = c01 * by + c11 * cy;
+    return c0 * bz + c1 * cz;
+}
+
+template <class T, class R>
+__global__ __launch_bounds__(256) void k_init_from_bg(ParamsT<T> p, FieldsT<T> f, BgField<T, R> bg) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= p.N) return;
+    const int x = i % p.X, t = i / p.X, y = t % p.Y, z = t / p.Y;
+    const R dxFG = (R)p.dx;  // the grid's dx in the geometry arithmetic
+    R s = sin(bg.angle), c = cos(bg.angle);
+    const R lx = (R)(x - p.X / 2) * dxFG, ly = (R)(y - p.Y / 2) * dxFG, lz = (R)(z - p.Z / 2) * dxFG;
+    const R gx = bg.fgPos[0] + (lx * c + lz * s), gy = bg.fgPos[1] + ly, gz = bg.fgPos[2] + (lz * c - lx * s);
+    s = sin(-bg.angle);
+    c = cos(-bg.angle);
+    const R rx = gx - bg.pos[0], ry = gy - bg.pos[1], rz = gz - bg.pos[2];
+    const R bxp = rx * c + rz * s, byp = ry, bzp = rz * c - rx * s;
+    const R u = bxp / ((R)bg.X * bg.dx) + (R)0.5f, v = byp / ((R)bg.Y * bg.dx) + (R)0.5f,
+            w = bzp / ((R)bg.Z * bg.dx) + (R)0.5f;
+    T vel[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vel[a] = sample_trilinear<T, R>(bg.vel[a], bg.X, bg.Y, bg.Z, u, v, w);
+    const T pr = bg.pres ? sample_trilinear<T, R>(bg.pres, bg.X, bg.Y, bg.Z, u, v, w) : (T)0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) f.vel[b][a][i] = vel[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f.ucorr[a][i] = 0;
+    f.p[i] = pr;
+    f.pc[0][i] = 0;
+    f.pc[1][i] = 0;
+}
+}  // namespace
+
+template <class T, class R>
+void init_from_background(const ParamsT<T>& p, const FieldsT<T>& f, const BgField<T, R>& bg, hipStream_t s) {
+    hipLaunchKernelGGL((k_init_from_bg<T, R>), dim3((p.N + 255) / 256), dim3(256), 0, s, p, f, bg);
+}
+template void init_from_background<float, float>(const ParamsT<float>&, const FieldsT<float>&,
+                                                 const BgField<float, float>&, hipStream_t);
+template void init_from_background<double, double>(const ParamsT<double>&, const FieldsT<double>&,
+                                                   const BgField<double, double>&, hipStream_t);
+template void init_from_background<CStep, double>(const ParamsT<CStep>&, const FieldsT<CStep>&,
+                                                  const BgField<CStep, double>&, hipStream_t);
+
 template <class T>
 void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s) {
     const int grid = std::min((n + 255) / 256, 8192);

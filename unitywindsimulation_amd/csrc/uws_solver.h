@@ -282,6 +282,45 @@ struct Solver : SolverBase {
         }
     }
 
+    // ---- warm start from a background flow (InitVelPresFieldsFromBackGroundFlow) ----
+    int init_from_background(const float fg_pos[3], float rot_deg, const uws_background& bgc, const float* vel,
+                             int ncomp, const float* pres) override {
+        using R = typename std::conditional<std::is_same<T, float>::value, float, double>::type;
+        const int64_t nb = (int64_t)bgc.grid_res[0] * bgc.grid_res[1] * bgc.grid_res[2];
+        void* tmp = nullptr;
+        if (hipMalloc(&tmp, sizeof(T) * (size_t)nb * 4) != hipSuccess) return UWS_ERR_ALLOC;
+        T* planes = static_cast<T*>(tmp);
+        int rc = UWS_OK;
+        {
+            std::vector<float> plane((size_t)nb);
+            for (int c = 0; c < 3 && rc == UWS_OK; ++c) {
+                for (int64_t i = 0; i < nb; ++i) plane[i] = vel[i * ncomp + c];
+                rc = up(planes + (size_t)c * nb, plane.data(), (size_t)nb);
+            }
+            if (rc == UWS_OK && pres) rc = up(planes + 3 * (size_t)nb, pres, (size_t)nb);
+        }
+        if (rc == UWS_OK) {
+            BgField<T, R> bg{};
+            bg.X = bgc.grid_res[0]; bg.Y = bgc.grid_res[1]; bg.Z = bgc.grid_res[2];
+            bg.dx = (R)bgc.dx;
+            for (int a = 0; a < 3; ++a) {
+                bg.pos[a] = (R)bgc.phys_field_pos[a];
+                bg.fgPos[a] = (R)fg_pos[a];
+                bg.vel[a] = planes + (size_t)a * nb;
+            }
+            bg.pres = pres ? planes + 3 * (size_t)nb : nullptr;
+            // the reference forms the angle in float: flowFieldOrientation * (float)(3.1415926 / 180)
+            bg.angle = (R)(rot_deg * (float)(3.1415926 / 180));
+            uws::init_from_background<T, R>(p, f, bg, stream);
+            rc = check_launch();
+            if (rc == UWS_OK) rc = sync();
+        }
+        (void)hipFree(tmp);
+        rs.cur = -1;
+        rs.pc_latest = -1;
+        return rc;
+    }
+
     // ---- time mean of the velocity ----
     int mean_reset() override {
         if (!mean_sum[0]) {

@@ -145,3 +145,27 @@ def lattice_layout(grid, n_buildings: int, lattice, seed: int, foot=(4, 12), hei
         z0 = cz0 + rng.randint(0, (cz1 - cz0 - 1) - wz)
         boxes.append(Box(x0, x0 + wx, z0, z0 + wz, h))
     return boxes
+
+
+def read_flag_file(path, ncells: int) -> np.ndarray:
+    """A mask written by the reference's FlagExporter (FlagExporter.cs:13-34):
+    the flat index z*X*Y + y*X + x of every solid cell, one per line.  Returns
+    the int32 flag array InitFlags takes (1 solid, 0 fluid)."""
+    flags = np.zeros(int(ncells), dtype=np.int32)
+    with open(path) as fh:
+        for line in fh:
+            line = line.strip()
+            if not line:
+                continue
+            i = int(line)
+            if i < 0 or i >= ncells:
+                raise ValueError(f"solid index {i} outside a grid of {ncells} cells")
+            flags[i] = 1
+    return flags
+
+
+def write_flag_file(path, flags) -> None:
+    """The FlagExporter format: one line per non-zero flag with its flat index."""
+    idx = np.flatnonzero(np.asarray(flags).reshape(-1))
+    with open(path, "w") as fh:
+        fh.write("".join(f"{int(i)}\n" for i in idx))

@@ -117,7 +117,29 @@ class FvmSolverHip(IFvmSolver):
                     "uws_init_vel_pres_fields")
 
     def InitVelPresFieldsFromBackGroundFlow(self, cfBackGround, velTexBackGround, presTexBackGround) -> None:
-        raise NotImplementedError("nested background-flow initialisation is outside the solver-step path")
+        """Warm start from a background flow (FvmSolverGpu.cs:898-933): cfBackGround
+        gives its grid, dx and physFieldPos, velTexBackGround its velocity as
+        [z][y][x][3|4] (what GetVelField returns), presTexBackGround its pressure
+        [z][y][x] (or None); this solver's physFieldPos and the runtime flow-field
+        orientation place the grid (uws_init_vel_pres_fields_from_background)."""
+        cfBackGround.Init()
+        bX, bY, bZ = cfBackGround.gridRes
+        nb = bX * bY * bZ
+        v = np.ascontiguousarray(np.asarray(velTexBackGround, dtype=np.float32).reshape(nb, -1))
+        if v.shape[1] not in (3, 4):
+            raise ValueError("background velocity must have 3 or 4 components per cell")
+        bg = N.uws_background()
+        bg.grid_res[:] = (bX, bY, bZ)
+        bg.dx = float(cfBackGround.dx)
+        bg.phys_field_pos[:] = tuple(float(x) for x in cfBackGround.physFieldPos)
+        pos = (ctypes.c_float * 3)(*[float(x) for x in self.cf.physFieldPos])
+        pp = None
+        if presTexBackGround is not None:
+            pr = np.ascontiguousarray(np.asarray(presTexBackGround, dtype=np.float32).reshape(nb))
+            pp = N.ptr(pr)
+        N.check(self._lib.uws_init_vel_pres_fields_from_background(
+            self._h, ctypes.cast(pos, c_void_p), float(self.rcf.flowFieldOrientation), byref(bg), N.ptr(v),
+            v.shape[1], pp), "uws_init_vel_pres_fields_from_background")
 
     def SetFixedValueVelBndCond(self) -> None:
         vz = (ctypes.c_float * 3)(*[float(v) for v in self.cf.velZn])
