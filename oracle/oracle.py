@@ -18,6 +18,7 @@ ORACLE_DIR = Path(__file__).resolve().parent
 LIB = ORACLE_DIR / "liboracle_uws.so"
 
 _SIG = {
+    "uwso_num_threads": (c_int, []),
     "uwso_create": (c_void_p, [POINTER(N.uws_config), c_int]),
     "uwso_destroy": (None, [c_void_p]),
     "uwso_init_flags": (None, [c_void_p, c_void_p]),

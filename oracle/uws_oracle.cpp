@@ -29,10 +29,20 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <algorithm>
 #include <type_traits>
 
 #include "../include/uws.h"
+
+
+// Cell loops of the step's kernels run in parallel over (z, y) when built
+// with OpenMP: every iteration writes only its own cell (gather form), so the
+// result does not depend on the thread count (small grids stay serial).
+#define UWSO_PRAGMA(x) _Pragma(#x)
+#define UWSO_PARALLEL_CELLS UWSO_PRAGMA(omp parallel for collapse(2) schedule(static) if (N >= 65536))
 
 namespace {
 
@@ -199,6 +209,7 @@ template <class T> struct Oracle {
     // CSCalLogLawWallFunc (WallFunc.compute:125-190)
     void CalLogLawWallFunc() {
         const V3<T>* u = velFieldLastTime;
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             V3<T> fev{0, 0, 0};
             if (IsFluid(x, y, z)) {
@@ -271,6 +282,7 @@ template <class T> struct Oracle {
     }
     // CSCalEddyVis (LES.compute:267-324)
     void CalEddyVis() {
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!IsFluid(x, y, z)) continue;
             T g[3][3];
@@ -307,6 +319,7 @@ template <class T> struct Oracle {
     void CalLesDeferCorrectTerm() {
         const V3<T>* u = velFieldLastTime;
         const T hdx = (T)0.5f * dx;
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!IsFluid(x, y, z)) continue;
             const V3<T>& uP = u[idx(x, y, z)];
@@ -365,6 +378,7 @@ template <class T> struct Oracle {
         const T* p = presLastTime.data();
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
         static const int bface[6] = {UWS_XN, UWS_X0, UWS_YN, UWS_Y0, UWS_ZN, UWS_Z0};
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!IsFluid(x, y, z)) continue;
             const int64_t P = idx(x, y, z);
@@ -460,6 +474,7 @@ template <class T> struct Oracle {
         const V3<T>* un = velFieldLastTime;
         const V3<T>* uk = velFieldLastIter;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!IsFluid(x, y, z)) continue;
             const int64_t P = idx(x, y, z);
@@ -530,6 +545,7 @@ template <class T> struct Oracle {
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
         static const int bface[6] = {UWS_XN, UWS_X0, UWS_YN, UWS_Y0, UWS_ZN, UWS_Z0};
         const T two_dx = (T)2 * dx;
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!IsFluid(x, y, z)) continue;
             const int64_t P = idx(x, y, z);
@@ -611,6 +627,7 @@ template <class T> struct Oracle {
     void PresCorrectSweep() {
         const T* pk = presCorrectFieldLastIter;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!IsFluid(x, y, z)) continue;
             const int64_t P = idx(x, y, z);
@@ -666,6 +683,7 @@ template <class T> struct Oracle {
         const T* pc = presCorrectField;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
         static const int bface[6] = {UWS_XN, UWS_X0, UWS_YN, UWS_Y0, UWS_ZN, UWS_Z0};
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!IsFluid(x, y, z)) continue;
             const int64_t P = idx(x, y, z);
@@ -699,6 +717,7 @@ template <class T> struct Oracle {
     void PisoCalAodUCorrect() {
         const V3<T>* un = velFieldLastTime;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!InRegionAndIsFluid(x, y, z)) continue;
             const int64_t P = idx(x, y, z);
@@ -733,6 +752,7 @@ template <class T> struct Oracle {
     void PisoPresCorrection() {
         const T* pk = presCorrectFieldLastIter;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!InRegionAndIsFluid(x, y, z)) continue;
             const int64_t P = idx(x, y, z);
@@ -767,6 +787,7 @@ template <class T> struct Oracle {
     void PisoVelCorrection() {
         const T* pc = presCorrectField;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        UWSO_PARALLEL_CELLS
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
             if (!IsFluid(x, y, z)) continue;
             const int64_t P = idx(x, y, z);
@@ -866,6 +887,15 @@ template <class F> static void dispatch(Handle* h, F&& fn) {
 }
 
 extern "C" {
+
+int uwso_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 
 /* precision: 0 = FP32 (the oracle proper), 1 = FP64 reference run */
 void* uwso_create(const uws_config* cfg, int precision) {
