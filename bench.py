@@ -339,8 +339,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation leg (adjoint vs complex step and central differences)")
     ap.add_argument("--adjoint-dirs", type=int, default=16)
-    ap.add_argument("--precision", action="store_true",
-                    help="add the precision leg (FP32 / FP16 / BF16 correction storage vs an FP64 run, 500 steps)")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the precision leg (FP32 / FP16 / BF16 correction storage vs an FP64 run, 500 steps)")
     ap.add_argument("--precision-steps", type=int, default=500)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM counters)")
@@ -428,7 +428,7 @@ def main() -> int:
     }
     if not args.no_ensemble:
         out["ensemble"] = ensemble_report()
-    if args.precision:
+    if not args.no_precision:
         out["precision"] = precision_report(args.precision_steps, args.precision_steps // 2)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_report(args.adjoint_dirs)
