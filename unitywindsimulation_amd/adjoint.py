@@ -185,3 +185,43 @@ def power_law_bnd(grid, u_ref, alpha, delta, ref_height=16.0, dx=1.0, xp=np):
     ux = col.repeat(Z)
     zero = torch.zeros_like(ux)
     return torch.stack([ux, zero, zero], dim=1)
+
+
+def _main(argv=None) -> int:
+    """Time forward and backward of the differentiation case:
+    ``python -m unitywindsimulation_amd.adjoint --steps 8 --ckpt 8 --precision 0``."""
+    import argparse
+    import copy
+    import time
+
+    from . import workloads
+    from .solver import FvmSolverHip
+
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--ckpt", type=int, default=8)
+    ap.add_argument("--precision", type=int, default=N.PREC_FP32, choices=(N.PREC_FP32, N.PREC_FP64))
+    args = ap.parse_args(argv)
+    d = workloads.differentiation()
+    w = d.base
+    cf = copy.deepcopy(w.cf)
+    cf.precision = args.precision
+    cf.Init()
+    with FvmSolverHip(cf) as s:
+        s.InitFlags(w.flags)
+        s.init_noise(w.noise_seed, w.noise_amp)
+        with FvmAdjoint(s, args.steps, args.ckpt) as adj:
+            t0 = time.perf_counter()
+            adj.forward()
+            t1 = time.perf_counter()
+            J, g = layer_objective(s.field64(N.FIELD_VEL), w.grid, d.region)
+            adj.backward(g)
+            t2 = time.perf_counter()
+            print(f"{w.name}: {args.steps} steps, checkpoint every {args.ckpt}, precision {args.precision}: "
+                  f"J = {J:.6e}, forward {t1 - t0:.3f} s, backward {t2 - t1:.3f} s, "
+                  f"dJ/dCs = {adj.grad_cs():.6e}, device memory {adj.memory_bytes / 2 ** 30:.2f} GiB", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())
