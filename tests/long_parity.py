@@ -32,12 +32,14 @@ def main() -> int:
     first = args.steps - args.mean_last
     t0 = time.perf_counter()
     o = oracle_for(w)
-    o.step(first)
     acc = None
-    for _ in range(args.mean_last):
+    for k in range(args.steps):
         o.step(1)
-        u = o.velocity().astype(np.float64)
-        acc = u if acc is None else acc + u
+        if k >= first:
+            u = o.velocity().astype(np.float64)
+            acc = u if acc is None else acc + u
+        if (k + 1) % 50 == 0:
+            print(f"  oracle step {k + 1} ({time.perf_counter() - t0:.0f} s)", flush=True)
     o_mean, o_u, o_p = acc / args.mean_last, o.velocity(), o.pressure()
     t1 = time.perf_counter()
     with solver_for(w) as s:
