@@ -63,6 +63,9 @@ enum { UWS_CELL_FLUID = 0, UWS_CELL_SOLID = 1 };
  * (WallFunc.compute:77-123); 1 = rough-wall log law with roughness z0
  * (extension). */
 enum { UWS_WALL_SMOOTH = 0, UWS_WALL_ROUGH = 1 };
+/* GridType (FluidSimConfig.cs:9-14): COLL_NONUNIFORM is the stretched
+ * collocated grid of FvmCollNonUniform.compute; STAGGERED is not built */
+enum { UWS_GRID_COLLOCATED = 0, UWS_GRID_COLL_NONUNIFORM = 1, UWS_GRID_STAGGERED = 2 };
 /* boundary faces, the order of the reference's six boundary textures
  * (FvmSolverGpu.cs:295-300) */
 enum { UWS_X0 = 0, UWS_XN = 1, UWS_Y0 = 2, UWS_YN = 3, UWS_Z0 = 4, UWS_ZN = 5 };
@@ -104,7 +107,9 @@ typedef struct uws_config {
     int32_t refine_rounds;        /* low-precision modes: defect-correction rounds per pressure solve */
     int32_t refine_sweeps;        /* low-precision modes: Jacobi sweeps per round */
     int32_t compute_norms;        /* 1: every step also computes the FP64 norms (uws_get_norms) */
-    int32_t reserved[4];          /* must be zero */
+    int32_t grid_type;            /* gridType (UWS_GRID_*); COLL_NONUNIFORM runs SIMPLE only, as the
+                                     reference (InitPisoShader, FvmSolverGpu.cs:490-520) */
+    int32_t reserved[3];          /* must be zero */
 } uws_config;
 
 /* precision modes (extension).  FP32 is the reference's arithmetic; FP64 is
@@ -146,6 +151,30 @@ int uws_init_flags(uws_handle h, const int32_t* flags, int64_t n);
 /* SolidType.Box (FvmSolverGpu.cs:963-970, InitShaderColl.compute:302-343):
  * the reference's hard-coded box x in [20,24), y in [0,8), z in [18,22). */
 int uws_init_box(uws_handle h);
+
+/* Stretched grid (grid_type UWS_GRID_COLL_NONUNIFORM).  The reference keeps
+ * the face coordinates of each axis in facePosX/Y/Z buffers of gridRes+1
+ * floats (InitMeshBuffers, FvmSolverGpu.cs:682-788); every stretched-grid
+ * kernel derives its spacings from them.  uws_create starts from uniform
+ * faces i*dx; uws_set_face_positions replaces them (pos_x has X+1 entries,
+ * pos_y Y+1, pos_z Z+1, strictly increasing).  Set them before the inflow and
+ * field initialisation, which place the log profile at face midpoints
+ * (CSInitLogVelPresFieldsNonUniform, CSSetLogVelBndCondNonUniform,
+ * InitShaderColl.compute:134-154, 287-300).  Returns UWS_ERR_CONFIG on a
+ * collocated grid. */
+int uws_set_face_positions(uws_handle h, const float* pos_x, const float* pos_y, const float* pos_z);
+int uws_get_face_positions(uws_handle h, float* pos_x, float* pos_y, float* pos_z);
+
+/* InitMeshBuffers' generator for one axis (FvmSolverGpu.cs:682-788), host
+ * only: n cells, a block of `uniform` cells of size dx preceded by
+ * `one_side` cells growing away from it, dx*ratio^k for the k-th cell from
+ * the block, and followed by the remaining n - uniform - one_side cells
+ * growing the same way; faces accumulate from 0 in float.  out has n+1
+ * entries.  The reference hard-codes uniform = (50, 50, 50),
+ * one_side = (25, 0, 25), ratio = 1.08.  Returns UWS_ERR_CONFIG when the
+ * block does not fit or the trailing side is longer than the leading one
+ * (the reference indexes the leading side's size table for both). */
+int uws_stretched_face_positions(int32_t n, float dx, int32_t uniform, int32_t one_side, float ratio, float* out);
 
 /* IFvmSolver.InitVelPresFields() (FvmSolverGpu.cs:790-896): inflow_type
  * UWS_INFLOW_CONSTANT sets u = (wind_speed, 0, 0) everywhere
@@ -225,7 +254,9 @@ enum {
     UWS_FIELD_DP = 9,         /* DFieldPresCorrect (1) */
     UWS_FIELD_BP = 10,        /* bFieldPresCorrect (1) */
     UWS_FIELD_AODU = 11,      /* AodUCorrectField (3) */
-    UWS_FIELD_COUNT = 12
+    UWS_FIELD_FACE_VEL = 12,  /* stretched grid: velFaceFluxField at the cell, the Rhie-Chow
+                                 velocity of its W, S, D faces (3) */
+    UWS_FIELD_COUNT = 13
 };
 int uws_field_ncomp(int32_t field);
 int uws_get_field(uws_handle h, int32_t field, float* dst, int64_t n_floats);

@@ -35,6 +35,9 @@ _SIG = {
     "uwso_log_law_wall_func": (c_double, [c_void_p, c_double]),
     "uwso_residuals": (None, [c_void_p, c_void_p]),
     "uwso_time": (None, [c_void_p, POINTER(c_int64), POINTER(c_double)]),
+    "uwso_set_face_positions": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "uwso_get_face_flux": (c_int, [c_void_p, c_void_p]),
+    "uwso_set_face_flux": (c_int, [c_void_p, c_void_p]),
 }
 _lib = None
 
@@ -83,8 +86,10 @@ class Oracle:
             pass
 
     # reference-style initialisation, mirroring FvmSolverHip's constructor
-    def init_like_solver(self, wind_speed=None):
+    def init_like_solver(self, wind_speed=None, face_positions=None):
         cf = self.cf
+        if face_positions is not None:
+            self.set_face_positions(*face_positions)
         ws = float(cf.velX0[0] if wind_speed is None else wind_speed)
         it = int(cf.inflowType)
         if it == 2:
@@ -96,6 +101,27 @@ class Oracle:
         vz = (ctypes.c_float * 3)(*[float(v) for v in cf.velZn])
         self.L.uwso_set_fixed_value_vel_bnd_cond(self.h, it, ws, ctypes.cast(vz, c_void_p),
                                                  float(cf.inflowRefHeight), float(cf.inflowAlpha))
+
+    def set_face_positions(self, fx, fy, fz):
+        """Stretched grid: face coordinates of each axis (X+1, Y+1, Z+1 values)."""
+        X, Y, Z = self.grid
+        a = [np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(-1)) for v in (fx, fy, fz)]
+        assert [v.size for v in a] == [X + 1, Y + 1, Z + 1]
+        self._face_pos = a
+        assert self.L.uwso_set_face_positions(self.h, *[v.ctypes.data_as(c_void_p) for v in a]) == 0
+
+    def face_flux(self):
+        """velFaceFluxField as [Z+1][Y+1][X+1][3]."""
+        X, Y, Z = self.grid
+        out = np.empty((Z + 1) * (Y + 1) * (X + 1) * 3, dtype=np.float64)
+        assert self.L.uwso_get_face_flux(self.h, out.ctypes.data_as(c_void_p)) == 0
+        return out.reshape(Z + 1, Y + 1, X + 1, 3)
+
+    def set_face_flux(self, a):
+        X, Y, Z = self.grid
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+        assert a.size == (Z + 1) * (Y + 1) * (X + 1) * 3
+        assert self.L.uwso_set_face_flux(self.h, a.ctypes.data_as(c_void_p)) == 0
 
     def init_flags(self, flags):
         a = np.ascontiguousarray(np.asarray(flags, dtype=np.int32).reshape(-1))

@@ -24,6 +24,13 @@
  *    it (FvmColl.compute:2864-2880); here the reference value is read once,
  *    then subtracted (the race-free meaning).
  *  - the 30 ms time slicing of Step() is not modelled (whole steps only).
+ *  - stretched grid (FvmCollNonUniform.compute): the momentum kernels read
+ *    velFaceFluxField, which the pressure precompute writes, although the
+ *    host's bindings of it for those two kernels are commented out
+ *    (FvmSolverGpu.cs:1160, 1196); here they read it, as the shader text
+ *    says.  The texture starts at zero and is not reset by the field
+ *    initialisation.  The LES and wall-function kernels stay the uniform ones
+ *    with the uniform dx, as the host binds them (FvmSolverGpu.cs:522-552).
  */
 #include <cmath>
 #include <cstdint>
@@ -59,6 +66,12 @@ template <class T> struct Oracle {
     T fext[3];
     int velBnd[6], presBnd[6];
     bool useLES, piso;
+    // stretched grid (GridType.CollNonUniform): face coordinates of each axis
+    // (the facePosX/Y/Z float buffers, FvmSolverGpu.cs:682-788) and the
+    // velFaceFluxField texture of (X+1)(Y+1)(Z+1) texels (FvmSolverGpu.cs:275)
+    bool nonuniform = false;
+    std::vector<float> facePos[3];
+    std::vector<V3<T>> faceFlux;
 
     std::vector<int32_t> flag;
     // texture storage; the pointers below are swapped like the C# references
@@ -97,6 +110,13 @@ template <class T> struct Oracle {
         for (int i = 0; i < 6; ++i) { velBnd[i] = c.vel_bnd[i]; presBnd[i] = c.pres_bnd[i]; }
         useLES = c.turbulence_model == UWS_TURB_SMAGORINSKY;
         piso = c.fvm_solver_type == UWS_PISO;
+        nonuniform = c.grid_type == UWS_GRID_COLL_NONUNIFORM;
+        const int res[3] = {X, Y, Z};
+        for (int a = 0; a < 3; ++a) {
+            facePos[a].resize(res[a] + 1);
+            for (int i = 0; i <= res[a]; ++i) facePos[a][i] = (float)i * c.dx;
+        }
+        if (nonuniform) faceFlux.assign((int64_t)(X + 1) * (Y + 1) * (Z + 1), V3<T>{0, 0, 0});
         flag.assign(N, UWS_CELL_FLUID);
         V3<T> z0{0, 0, 0};
         for (auto& t : tex_vel) t.assign(N, z0);
@@ -132,6 +152,8 @@ template <class T> struct Oracle {
                 velField[i] = velFieldLastIter[i] = velFieldLastTime[i] = V3<T>{0, 0, 0};
             }
     }
+    // 0.5 * (facePosY[y] + facePosY[y + 1]) (InitShaderColl.compute:141, 295)
+    T YMid(int y) const { return (T)0.5f * ((T)facePos[1][y] + (T)facePos[1][y + 1]); }
     // CSInitVelPresFields (:94-110) / CSInitLogVelPresFields (:112-132)
     void InitVelPresFields(int inflow, T wind) {
         for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
@@ -139,7 +161,8 @@ template <class T> struct Oracle {
             V3<T> v{wind, 0, 0};
             if (inflow == UWS_INFLOW_LOG) {
                 const T karman = (T)0.4f, rough = (T)1.0f, refH = (T)10.0f;
-                T yPhys = (T)y * dx;
+                // stretched grid: the cell-centre height (CSInitLogVelPresFieldsNonUniform, :134-154)
+                T yPhys = nonuniform ? YMid(y) : (T)y * dx;
                 T fricVel = karman * wind / std::log((refH + rough) / rough);
                 T vel = fricVel / karman * std::log((yPhys + rough) / rough);
                 v = V3<T>{vel, 0, 0};
@@ -161,14 +184,15 @@ template <class T> struct Oracle {
         } else if (inflow == UWS_INFLOW_LOG) {
             const T karman = (T)0.4f, rough = (T)1.0f, refH = (T)10.0f;
             for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) {
-                T yPhys = ((T)y + (T)0.5f) * dx;
+                // stretched grid: CSSetLogVelBndCondNonUniform (:287-300)
+                T yPhys = nonuniform ? YMid(y) : ((T)y + (T)0.5f) * dx;
                 T fricVel = karman * wind / std::log((refH + rough) / rough);
                 T vel = fricVel / karman * std::log((yPhys + rough) / rough);
                 bnd[UWS_X0][(int64_t)z * Y + y] = V3<T>{vel, 0, 0};
             }
         } else {
             for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) {
-                T yPhys = ((T)y + (T)0.5f) * dx;
+                T yPhys = nonuniform ? YMid(y) : ((T)y + (T)0.5f) * dx;
                 T vel = wind * (T)std::pow((double)(yPhys / refHeight), (double)alpha);
                 bnd[UWS_X0][(int64_t)z * Y + y] = V3<T>{vel, 0, 0};
             }
@@ -374,6 +398,7 @@ template <class T> struct Oracle {
     // ------------------------------------------------------------------
     // CSVelPredictPreCompute (FvmColl.compute:1291-1859)
     void VelPredictPreCompute() {
+        if (nonuniform) { NuVelPredictPreCompute(); return; }
         const V3<T>* un = velFieldLastTime;
         const T* p = presLastTime.data();
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
@@ -471,6 +496,7 @@ template <class T> struct Oracle {
     }
     // CSVelPredict (FvmColl.compute:1862-2069): reads velFieldLastIter, writes velField
     void VelPredict() {
+        if (nonuniform) { NuVelPredict(); return; }
         const V3<T>* un = velFieldLastTime;
         const V3<T>* uk = velFieldLastIter;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
@@ -540,6 +566,7 @@ template <class T> struct Oracle {
     T Dc(int64_t i, int c) const { return c == 0 ? Dfield[i].x : (c == 1 ? Dfield[i].y : Dfield[i].z); }
     // CSPresCorrectRhieChowPreCompute (FvmColl.compute:2480-2787)
     void PresCorrectPreCompute() {
+        if (nonuniform) { NuPresCorrectPreCompute(); return; }
         const V3<T>* u = velField;
         const T* p = presLastTime.data();
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
@@ -625,6 +652,7 @@ template <class T> struct Oracle {
     }
     // CSPresCorrectRhieChow (FvmColl.compute:2789-2848): reads LastIter, writes presCorrectField
     void PresCorrectSweep() {
+        if (nonuniform) { NuPresCorrectSweep(); return; }
         const T* pk = presCorrectFieldLastIter;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
         UWSO_PARALLEL_CELLS
@@ -680,6 +708,11 @@ template <class T> struct Oracle {
     }
     // CSApplyVelCorrectionDivergenceTheorem (FvmColl.compute:2950-3121) + SetPresCorrectFieldtoZero
     void ApplyVelCorrectionStage() {
+        if (nonuniform) {
+            NuApplyVelCorrection();
+            SetPresCorrectFieldToZero();
+            return;
+        }
         const T* pc = presCorrectField;
         static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
         static const int bface[6] = {UWS_XN, UWS_X0, UWS_YN, UWS_Y0, UWS_ZN, UWS_Z0};
@@ -818,6 +851,358 @@ template <class T> struct Oracle {
         PisoVelCorrection();
         SetPresCorrectFieldToZero();
     }
+    // ------------------------------------------------------------------
+    // Stretched collocated grid (FvmCollNonUniform.compute).  Same step as
+    // the uniform grid (SIMPLE only); spacings come from the face
+    // coordinates, face values are linear interpolations with the weights
+    // of the two cell sizes, and the momentum convection uses the Rhie-Chow
+    // face velocities that the previous pressure precompute left in
+    // velFaceFluxField.
+    // ------------------------------------------------------------------
+    int64_t fidx(int x, int y, int z) const { return ((int64_t)z * (Y + 1) + y) * (X + 1) + x; }
+    T Sp(int a, int i) const { return (T)facePos[a][i + 1] - (T)facePos[a][i]; }
+    // cell and neighbour spacings; a neighbour outside the grid takes the
+    // cell's own spacing (FvmCollNonUniform.compute:130-157, 943-983)
+    struct Geo {
+        T d[3];    // dx dy dz
+        T ds[3];   // dsx = dy*dz, dsy = dx*dz, dsz = dx*dy
+        T dv;      // dx*dy*dz
+        T dn[6];   // dxE dxW dyN dyS dzU dzD
+        T dnn[6];  // dxEE dxWW dyNN dySS dzUU dzDD
+    };
+    Geo Geometry(int x, int y, int z) const {
+        Geo g;
+        const int c[3] = {x, y, z}, n[3] = {X, Y, Z};
+        for (int a = 0; a < 3; ++a) g.d[a] = Sp(a, c[a]);
+        g.ds[0] = g.d[1] * g.d[2];
+        g.ds[1] = g.d[0] * g.d[2];
+        g.ds[2] = g.d[0] * g.d[1];
+        g.dv = g.d[0] * g.d[1] * g.d[2];
+        for (int a = 0; a < 3; ++a) {
+            g.dn[2 * a] = c[a] + 1 < n[a] ? Sp(a, c[a] + 1) : g.d[a];
+            g.dn[2 * a + 1] = c[a] - 1 >= 0 ? Sp(a, c[a] - 1) : g.d[a];
+            g.dnn[2 * a] = c[a] + 2 < n[a] ? Sp(a, c[a] + 2) : g.d[a];
+            g.dnn[2 * a + 1] = c[a] - 2 >= 0 ? Sp(a, c[a] - 2) : g.d[a];
+        }
+        return g;
+    }
+    // face velocity of face f (E W N S U D) of cell (x, y, z): the + faces are
+    // stored at the neighbour's texel, the - faces at the cell's own
+    T FaceFlux(int f, int x, int y, int z) const {
+        switch (f) {
+        case 0: return faceFlux[fidx(x + 1, y, z)].x;
+        case 1: return faceFlux[fidx(x, y, z)].x;
+        case 2: return faceFlux[fidx(x, y + 1, z)].y;
+        case 3: return faceFlux[fidx(x, y, z)].y;
+        case 4: return faceFlux[fidx(x, y, z + 1)].z;
+        default: return faceFlux[fidx(x, y, z)].z;
+        }
+    }
+    // face viscosities: nu, plus the interpolated eddy viscosity with LES
+    // (FvmCollNonUniform.compute:482-503)
+    void NuFaceVis(int x, int y, int z, const Geo& g, T vis[6]) const {
+        static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        const int64_t P = idx(x, y, z);
+        for (int f = 0; f < 6; ++f) vis[f] = nu;
+        if (!useLES) return;
+        for (int f = 0; f < 6; ++f) vis[f] = nu + eddyVis[P];
+        for (int f = 0; f < 6; ++f) {
+            int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
+            if (!InRegionAndIsFluid(nx, ny, nz)) continue;
+            const int a = f / 2;
+            T wP = g.dn[f] / (g.d[a] + g.dn[f]);
+            T wN = (T)1.0f - wP;
+            vis[f] = nu + wP * eddyVis[P] + wN * eddyVis[idx(nx, ny, nz)];
+        }
+    }
+    // CSVelPredictPreCompute (FvmCollNonUniform.compute:111-721)
+    void NuVelPredictPreCompute() {
+        const V3<T>* un = velFieldLastTime;
+        const T* p = presLastTime.data();
+        static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        static const int bface[6] = {UWS_XN, UWS_X0, UWS_YN, UWS_Y0, UWS_ZN, UWS_Z0};
+        UWSO_PARALLEL_CELLS
+        for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
+            if (!IsFluid(x, y, z)) continue;
+            const int64_t P = idx(x, y, z);
+            const Geo g = Geometry(x, y, z);
+            T Dx = g.dv / dt, Dy = g.dv / dt, Dz = g.dv / dt;
+            T bU = 0, bV = 0, bW = 0;
+            const V3<T>& uP = un[P];
+            // advection with the stored face velocities (:185-325)
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                const bool plus = (f % 2) == 0;
+                int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
+                const T F = FaceFlux(f, x, y, z);
+                const T dsa = g.ds[a];
+                if (InRegionAndIsFluid(nx, ny, nz)) {
+                    T wP = g.dn[f] / (g.d[a] + g.dn[f]);
+                    T t = dsa * wP * F;
+                    if (plus) { Dx += t; Dy += t; Dz += t; } else { Dx -= t; Dy -= t; Dz -= t; }
+                } else if (!InRegion(nx, ny, nz)) {
+                    int bf = bface[f];
+                    if (velBnd[bf] == UWS_VEL_FIXED_VALUE) {
+                        int i, j;
+                        if (bf < 2) { i = y; j = z; } else if (bf < 4) { i = x; j = z; } else { i = x; j = y; }
+                        const V3<T>& ub = B(bf, i, j);
+                        if (plus) { bU -= dsa * ub.x * F; bV -= dsa * ub.y * F; bW -= dsa * ub.z * F; }
+                        else { bU += dsa * ub.x * F; bV += dsa * ub.y * F; bW += dsa * ub.z * F; }
+                    } else if (velBnd[bf] == UWS_VEL_ZERO_GRAD) {
+                        T t = dsa * F;
+                        if (plus) { Dx += t; Dy += t; Dz += t; } else { Dx -= t; Dy -= t; Dz -= t; }
+                    }
+                }
+            }
+            // pressure face values (:327-462)
+            T pf[6];
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
+                if (InRegion(nx, ny, nz)) {
+                    if (IsFluid(nx, ny, nz)) {
+                        T wP = g.dn[f] / (g.d[a] + g.dn[f]);
+                        T wN = (T)1.0f - wP;
+                        pf[f] = wP * p[P] + wN * p[idx(nx, ny, nz)];
+                    } else {
+                        pf[f] = p[P];
+                    }
+                } else {
+                    pf[f] = presBnd[bface[f]] == UWS_PRES_FIXED_VALUE ? (T)0 : p[P];
+                }
+            }
+            bU += -((T)1.0f / den) * g.ds[0] * (pf[0] - pf[1]);
+            bV += -((T)1.0f / den) * g.ds[1] * (pf[2] - pf[3]);
+            bW += -((T)1.0f / den) * g.ds[2] * (pf[4] - pf[5]);
+            // diffusion (:464-697)
+            T vis[6];
+            NuFaceVis(x, y, z, g, vis);
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                const T dsa = g.ds[a];
+                int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
+                if (InRegion(nx, ny, nz)) {
+                    T t = IsFluid(nx, ny, nz) ? vis[f] * dsa / ((T)0.5f * g.d[a] + (T)0.5f * g.dn[f])
+                                              : vis[f] * dsa / ((T)0.5f * g.d[a]);
+                    Dx += t; Dy += t; Dz += t;
+                } else {
+                    int bf = bface[f];
+                    if (velBnd[bf] == UWS_VEL_FIXED_VALUE) {
+                        int i, j;
+                        if (bf < 2) { i = y; j = z; } else if (bf < 4) { i = x; j = z; } else { i = x; j = y; }
+                        const V3<T>& ub = B(bf, i, j);
+                        T k = vis[f] * dsa / ((T)0.5f * g.d[a]);
+                        bU += k * ub.x; bV += k * ub.y; bW += k * ub.z;
+                        Dx += k; Dy += k; Dz += k;
+                    } else if (velBnd[bf] == UWS_VEL_SYMMETRY) {
+                        T k = vis[f] * dsa / ((T)0.5f * g.d[a]);
+                        if (a == 0) Dx += k; else if (a == 1) Dy += k; else Dz += k;
+                    }
+                }
+            }
+            // time derivative, force, LES (:699-715)
+            bU += g.dv / dt * uP.x; bV += g.dv / dt * uP.y; bW += g.dv / dt * uP.z;
+            bU += g.dv * fext[0]; bV += g.dv * fext[1]; bW += g.dv * fext[2];
+            if (useLES) { bU += bfield[P].x; bV += bfield[P].y; bW += bfield[P].z; }
+            Dfield[P] = V3<T>{Dx, Dy, Dz};
+            bfield[P] = V3<T>{bU, bV, bW};
+        }
+    }
+    // CSVelPredict (FvmCollNonUniform.compute:725-918)
+    void NuVelPredict() {
+        const V3<T>* uk = velFieldLastIter;
+        static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        UWSO_PARALLEL_CELLS
+        for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
+            if (!IsFluid(x, y, z)) continue;
+            const int64_t P = idx(x, y, z);
+            const Geo g = Geometry(x, y, z);
+            T UA = 0, VA = 0, WA = 0, UD = 0, VD = 0, WD = 0;
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                const bool plus = (f % 2) == 0;
+                int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
+                if (!InRegionAndIsFluid(nx, ny, nz)) continue;
+                const int64_t Q = idx(nx, ny, nz);
+                T wP = g.dn[f] / (g.d[a] + g.dn[f]);
+                T wN = (T)1.0f - wP;
+                const T F = FaceFlux(f, x, y, z);
+                T tU = g.ds[a] * wN * uk[Q].x * F;
+                T tV = g.ds[a] * wN * uk[Q].y * F;
+                T tW = g.ds[a] * wN * uk[Q].z * F;
+                if (plus) { UA -= tU; VA -= tV; WA -= tW; } else { UA += tU; VA += tV; WA += tW; }
+            }
+            T vis[6];
+            NuFaceVis(x, y, z, g, vis);
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
+                if (!InRegionAndIsFluid(nx, ny, nz)) continue;
+                const int64_t Q = idx(nx, ny, nz);
+                T k = vis[f] * g.ds[a] / ((T)0.5f * g.d[a] + (T)0.5f * g.dn[f]);
+                UD += k * uk[Q].x;
+                VD += k * uk[Q].y;
+                WD += k * uk[Q].z;
+            }
+            velField[P].x = (T)1.0f / Dfield[P].x * (UA + UD + bfield[P].x);
+            velField[P].y = (T)1.0f / Dfield[P].y * (VA + VD + bfield[P].y);
+            velField[P].z = (T)1.0f / Dfield[P].z * (WA + WD + bfield[P].z);
+        }
+    }
+    // CSPresCorrectRhieChowPreCompute (FvmCollNonUniform.compute:920-1344):
+    // Dp, bp and the face velocities left in velFaceFluxField
+    void NuPresCorrectPreCompute() {
+        const V3<T>* u = velField;
+        const T* p = presLastTime.data();
+        static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        static const int bface[6] = {UWS_XN, UWS_X0, UWS_YN, UWS_Y0, UWS_ZN, UWS_Z0};
+        UWSO_PARALLEL_CELLS
+        for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
+            if (!IsFluid(x, y, z)) continue;
+            const int64_t P = idx(x, y, z);
+            const Geo g = Geometry(x, y, z);
+            T Dp = 0, bp = 0;
+            T uf[6] = {0, 0, 0, 0, 0, 0};
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                const bool plus = (f % 2) == 0;
+                const int ox = off[f][0], oy = off[f][1], oz = off[f][2];
+                const int nx = x + ox, ny = y + oy, nz = z + oz;
+                const T uPc = a == 0 ? u[P].x : (a == 1 ? u[P].y : u[P].z);
+                if (InRegion(nx, ny, nz)) {
+                    if (!IsFluid(nx, ny, nz)) continue;  // a solid neighbour leaves the face at 0
+                    const int64_t Q = idx(nx, ny, nz);
+                    const T d = g.d[a], dQ = g.dn[f], dF = g.dnn[f], dB = g.dn[f ^ 1];
+                    T wP = dQ / (d + dQ);
+                    T wN = (T)1.0f - wP;
+                    T DInv = wP * ((T)1 / Dc(P, a)) + wN * ((T)1 / Dc(Q, a));
+                    Dp -= (g.dv / den) * DInv / ((T)0.5f * d + (T)0.5f * dQ);
+                    const T uQc = a == 0 ? u[Q].x : (a == 1 ? u[Q].y : u[Q].z);
+                    T face = wP * uPc + wN * uQc;
+                    // Rhie-Chow correction with one-sided averages next to solids and walls
+                    const int fx = x + 2 * ox, fy = y + 2 * oy, fz = z + 2 * oz;
+                    const int bx = x - ox, by = y - oy, bz = z - oz;
+                    const bool farF = InRegionAndIsFluid(fx, fy, fz), backF = InRegionAndIsFluid(bx, by, bz);
+                    const T pP = p[P], pQ = p[Q];
+                    const T h = (T)0.5f; const T h3 = (T)1.5f;
+                    T gr, ga;
+                    if (plus) {
+                        gr = (pQ - pP) / (h * d + h * dQ);
+                        if (farF && backF) {
+                            const T pF = p[idx(fx, fy, fz)], pB = p[idx(bx, by, bz)];
+                            ga = h * ((pF - pP) / (h * dF + dQ + h * d) + (pQ - pB) / (h * dQ + d + h * dB));
+                        } else if (farF) {
+                            const T pF = p[idx(fx, fy, fz)];
+                            ga = h * ((pF - pP) / (h * dF + dQ + h * d) + (pQ - pP) / (h * dQ + h3 * d));
+                        } else if (backF) {
+                            const T pB = p[idx(bx, by, bz)];
+                            ga = h * ((pQ - pP) / (h3 * dQ + h * d) + (pQ - pB) / (h * dQ + d + h * dB));
+                        } else {
+                            ga = gr;
+                        }
+                    } else {
+                        gr = (pP - pQ) / (h * d + h * dQ);
+                        if (farF && backF) {
+                            const T pF = p[idx(fx, fy, fz)], pB = p[idx(bx, by, bz)];
+                            ga = h * ((pB - pQ) / (h * dB + d + h * dQ) + (pP - pF) / (h * d + dQ + h * dF));
+                        } else if (farF) {
+                            const T pF = p[idx(fx, fy, fz)];
+                            ga = h * ((pP - pQ) / (h3 * d + h * dQ) + (pP - pF) / (h * d + dQ + h * dF));
+                        } else if (backF) {
+                            const T pB = p[idx(bx, by, bz)];
+                            ga = h * ((pB - pQ) / (h * dB + d + h * dQ) + (pP - pQ) / (h * d + h3 * dQ));
+                        } else {
+                            ga = gr;
+                        }
+                    }
+                    face -= (g.dv * DInv) / den * (gr - ga);
+                    if (plus) bp += face; else bp -= face;
+                    uf[f] = face;
+                } else {
+                    const int bf = bface[f];
+                    if (velBnd[bf] == UWS_VEL_FIXED_VALUE) {
+                        int i, j;
+                        if (bf < 2) { i = y; j = z; } else if (bf < 4) { i = x; j = z; } else { i = x; j = y; }
+                        const V3<T>& ub = B(bf, i, j);
+                        const T ubc = a == 0 ? ub.x : (a == 1 ? ub.y : ub.z);
+                        uf[f] = ubc;
+                        if (plus) bp += ubc; else bp -= ubc;
+                    } else if (velBnd[bf] == UWS_VEL_ZERO_GRAD) {
+                        uf[f] = uPc;
+                        if (plus) bp += uPc; else bp -= uPc;
+                    }
+                    if (presBnd[bf] == UWS_PRES_FIXED_VALUE) {
+                        T DInv = (T)1 / Dc(P, a);
+                        Dp -= (g.dv / den) * DInv / ((T)0.5f * g.d[a]);
+                    }
+                }
+            }
+            // velFaceFluxField: the cell's W/S/D faces, and the + faces on the domain boundary (:1335-1342)
+            faceFlux[fidx(x, y, z)] = V3<T>{uf[1], uf[3], uf[5]};
+            if (x + 1 >= X) faceFlux[fidx(x + 1, y, z)] = V3<T>{uf[0], 0, 0};
+            if (y + 1 >= Y) faceFlux[fidx(x, y + 1, z)] = V3<T>{0, uf[2], 0};
+            if (z + 1 >= Z) faceFlux[fidx(x, y, z + 1)] = V3<T>{0, 0, uf[4]};
+            DpField[P] = Dp;
+            bpField[P] = bp;
+        }
+    }
+    // CSPresCorrectRhieChow (FvmCollNonUniform.compute:1346-1453)
+    void NuPresCorrectSweep() {
+        const T* pk = presCorrectFieldLastIter;
+        static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        UWSO_PARALLEL_CELLS
+        for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
+            if (!IsFluid(x, y, z)) continue;
+            const int64_t P = idx(x, y, z);
+            const Geo g = Geometry(x, y, z);
+            T term = 0;
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
+                if (!InRegionAndIsFluid(nx, ny, nz)) continue;
+                const int64_t Q = idx(nx, ny, nz);
+                T wP = g.dn[f] / (g.d[a] + g.dn[f]);
+                T wN = (T)1.0f - wP;
+                T DInv = wP * ((T)1 / Dc(P, a)) + wN * ((T)1 / Dc(Q, a));
+                term -= (g.dv / den) * DInv / ((T)0.5f * g.d[a] + (T)0.5f * g.dn[f]) * pk[Q];
+            }
+            presCorrectField[P] = ((T)1 / DpField[P]) * (term + bpField[P]);
+        }
+    }
+    // CSApplyVelCorrectionDivergenceTheorem (FvmCollNonUniform.compute:1487-1732)
+    void NuApplyVelCorrection() {
+        const T* pc = presCorrectField;
+        static const int off[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        static const int bface[6] = {UWS_XN, UWS_X0, UWS_YN, UWS_Y0, UWS_ZN, UWS_Z0};
+        UWSO_PARALLEL_CELLS
+        for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
+            if (!IsFluid(x, y, z)) continue;
+            const int64_t P = idx(x, y, z);
+            const Geo g = Geometry(x, y, z);
+            T pf[6];
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
+                if (InRegion(nx, ny, nz)) {
+                    if (IsFluid(nx, ny, nz)) {
+                        T wP = g.dn[f] / (g.d[a] + g.dn[f]);
+                        T wN = (T)1.0f - wP;
+                        pf[f] = wP * pc[P] + wN * pc[idx(nx, ny, nz)];
+                    } else {
+                        pf[f] = pc[P];
+                    }
+                } else {
+                    pf[f] = presBnd[bface[f]] == UWS_PRES_FIXED_VALUE ? (T)0 : pc[P];
+                }
+            }
+            T cx = -(g.dv / den) * ((T)1 / Dfield[P].x) * (pf[0] - pf[1]) / g.d[0];
+            T cy = -(g.dv / den) * ((T)1 / Dfield[P].y) * (pf[2] - pf[3]) / g.d[1];
+            T cz = -(g.dv / den) * ((T)1 / Dfield[P].z) * (pf[4] - pf[5]) / g.d[2];
+            velCorrect[P] = V3<T>{cx, cy, cz};
+            velField[P].x += cx; velField[P].y += cy; velField[P].z += cz;
+        }
+    }
+
     // end of Step() (FvmSolverGpu.cs:1103-1110)
     void FinishStage() {
         std::swap(velFieldLastTime, velField);
@@ -927,7 +1312,15 @@ void uwso_run_stage(void* hv, int s) { dispatch((Handle*)hv, [&](auto& o) { o.Ru
 int uwso_get_field(void* hv, int f, double* dst) {
     int rc = 0;
     dispatch((Handle*)hv, [&](auto& o) {
-        if (auto* v = o.vfield(f)) {
+        if (f == UWS_FIELD_FACE_VEL) {
+            // velFaceFluxField at the cells: their W, S, D face velocities
+            if (!o.nonuniform) { rc = -1; return; }
+            for (int z = 0; z < o.Z; ++z) for (int y = 0; y < o.Y; ++y) for (int x = 0; x < o.X; ++x) {
+                const int64_t i = o.idx(x, y, z);
+                const auto& v = o.faceFlux[o.fidx(x, y, z)];
+                dst[3*i] = v.x; dst[3*i+1] = v.y; dst[3*i+2] = v.z;
+            }
+        } else if (auto* v = o.vfield(f)) {
             for (int64_t i = 0; i < o.N; ++i) { dst[3*i] = v[i].x; dst[3*i+1] = v[i].y; dst[3*i+2] = v[i].z; }
         } else if (auto* s = o.sfield(f)) {
             for (int64_t i = 0; i < o.N; ++i) dst[i] = s[i];
@@ -949,7 +1342,40 @@ int uwso_set_field(void* hv, int f, const double* src) {
             for (int64_t i = 0; i < o.N; ++i) v[i] = V3<TT>{(TT)src[3*i], (TT)src[3*i+1], (TT)src[3*i+2]};
         } else if (auto* s = o.sfield(f)) {
             for (int64_t i = 0; i < o.N; ++i) s[i] = (TT)src[i];
+            // p' is also the next sweep's input (presCorrectFieldLastIter)
+            if (f == UWS_FIELD_PRES_CORR) std::copy(s, s + o.N, o.presCorrectFieldLastIter);
         } else rc = -1;
+    });
+    return rc;
+}
+/* stretched grid: face coordinates (X+1, Y+1, Z+1 floats) */
+int uwso_set_face_positions(void* hv, const float* fx, const float* fy, const float* fz) {
+    int rc = 0;
+    dispatch((Handle*)hv, [&](auto& o) {
+        if (!o.nonuniform) { rc = -1; return; }
+        const float* src[3] = {fx, fy, fz};
+        for (int a = 0; a < 3; ++a) std::copy(src[a], src[a] + o.facePos[a].size(), o.facePos[a].begin());
+    });
+    return rc;
+}
+/* the whole velFaceFluxField texture, [(z*(Y+1) + y)*(X+1) + x][3] */
+int uwso_get_face_flux(void* hv, double* dst) {
+    int rc = 0;
+    dispatch((Handle*)hv, [&](auto& o) {
+        if (!o.nonuniform) { rc = -1; return; }
+        for (size_t i = 0; i < o.faceFlux.size(); ++i) {
+            dst[3*i] = o.faceFlux[i].x; dst[3*i+1] = o.faceFlux[i].y; dst[3*i+2] = o.faceFlux[i].z;
+        }
+    });
+    return rc;
+}
+int uwso_set_face_flux(void* hv, const double* src) {
+    int rc = 0;
+    dispatch((Handle*)hv, [&](auto& o) {
+        using TT = std::remove_reference_t<decltype(o.dx)>;
+        if (!o.nonuniform) { rc = -1; return; }
+        for (size_t i = 0; i < o.faceFlux.size(); ++i)
+            o.faceFlux[i] = V3<TT>{(TT)src[3*i], (TT)src[3*i+1], (TT)src[3*i+2]};
     });
     return rc;
 }

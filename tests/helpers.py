@@ -18,7 +18,7 @@ def rel_l2(a, b) -> float:
 
 def oracle_for(w, precision=0, u0=None, p0=None):
     o = Oracle(w.cf, precision)
-    o.init_like_solver()
+    o.init_like_solver(face_positions=getattr(w, "face_positions", None))
     o.init_flags(w.flags)
     o.set_velocity(noise_velocity(w.flags, w.noise_seed, w.noise_amp) if u0 is None else u0)
     if p0 is not None:
@@ -29,7 +29,7 @@ def oracle_for(w, precision=0, u0=None, p0=None):
 def solver_for(w, u0=None, p0=None):
     from unitywindsimulation_amd import FvmSolverHip
 
-    s = FvmSolverHip(w.cf)
+    s = FvmSolverHip(w.cf, face_positions=getattr(w, "face_positions", None))
     s.InitFlags(w.flags)
     s.set_velocity(noise_velocity(w.flags, w.noise_seed, w.noise_amp) if u0 is None else u0)
     if p0 is not None:

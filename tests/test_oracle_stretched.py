@@ -1,0 +1,371 @@
+"""Pins of the oracle's stretched-grid restatement (FvmCollNonUniform.compute).
+
+Every check holds for the discretisation as the reference writes it and fails
+for a plausible slip: a swapped interpolation weight (the cell's own weight
+used for its neighbour), a spacing taken from the wrong cell, a wrong face
+distance, a sign error in a boundary term.  Spacings are deliberately uneven
+in every axis.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from unitywindsimulation_amd import _native as N
+from unitywindsimulation_amd.config import (FluidSimConfig, FvmSolverType, GridType, InflowType, PresBndCond,
+                                            TurbulenceModel, VelBndCond)
+from unitywindsimulation_amd.geometry import Box, rasterise
+
+DEN = float(np.float32(1.3))  # the configuration's density as the library stores it (float)
+
+
+def stretched_config(grid=(9, 7, 8), les=False, vel=1, pres=1, dt=0.05, mu=0.0, grid_type=GridType.CollNonUniform):
+    X, Y, Z = grid
+    cf = FluidSimConfig(physDomainSize=(float(X), float(Y), float(Z)), gridResX=X, dt=dt, mu=mu, den=DEN,
+                        velMaxNumIter=vel, presMaxNumIter=pres,
+                        turbulenceModel=TurbulenceModel.Smagorinsky if les else TurbulenceModel.None_,
+                        fvmSolverType=FvmSolverType.SIMPLE, smagorinskyConstant=0.1,
+                        inflowType=InflowType.Constant, gridType=grid_type)
+    cf.Init()
+    return cf
+
+
+def set_bc(cf, vel, pres):
+    for n, v, p in zip(["X0", "Xn", "Y0", "Yn", "Z0", "Zn"], vel, pres):
+        setattr(cf, "velBndCond" + n, VelBndCond(v))
+        setattr(cf, "presBndCond" + n, PresBndCond(p))
+    cf.Init()
+
+
+def uneven_faces(n, seed):
+    """Face coordinates of n cells with sizes in [0.55, 1.7] (float32)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.uniform(0.55, 1.7, size=n).astype(np.float32)
+    return np.concatenate([[0.0], np.cumsum(sizes)]).astype(np.float32)
+
+
+def faces_for(grid, seed=3):
+    return [uneven_faces(n, seed + a) for a, n in enumerate(grid)]
+
+
+def centres(faces):
+    f = [np.asarray(v, dtype=np.float64) for v in faces]
+    return [0.5 * (v[1:] + v[:-1]) for v in f], [np.diff(v) for v in f]
+
+
+def cell_coords(grid):
+    X, Y, Z = grid
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    return x.reshape(-1), y.reshape(-1), z.reshape(-1)
+
+
+def make(cf, faces, precision=1, flags=None):
+    o = Oracle(cf, precision=precision)
+    o.set_face_positions(*faces)
+    if flags is not None:
+        o.init_flags(flags)
+    return o
+
+
+def test_uniform_faces_reduce_to_the_uniform_pressure_path():
+    """With faces i*dx every weight is 1/2 and every distance dx: the
+    stretched-grid pressure precompute, Jacobi sweeps and velocity correction
+    equal the (separately pinned) uniform-grid kernels on a random state with
+    solids, every velocity BC type and a Dirichlet outlet."""
+    grid = (9, 7, 6)
+    X, Y, Z = grid
+    out = {}
+    for gt in (GridType.Collocated, GridType.CollNonUniform):
+        cf = stretched_config(grid, pres=7, grid_type=gt)
+        set_bc(cf, [0, 1, 2, 0, 1, 2], [1, 0, 1, 2, 2, 1])
+        flags = rasterise(cf.gridRes, [Box(3, 5, 0, 2, 2), Box(6, 7, 3, 4, 1)])
+        o = Oracle(cf, precision=1)
+        if gt == GridType.CollNonUniform:
+            o.set_face_positions(*[np.arange(n + 1, dtype=np.float32) for n in grid])
+        o.init_flags(flags)
+        rng = np.random.default_rng(4)
+        u = rng.normal(size=(o.n, 3))
+        u[flags != 0] = 0
+        p = rng.normal(size=o.n)
+        D = rng.uniform(4.0, 9.0, size=(o.n, 3))
+        for face in range(6):
+            nb = (Y * Z, Y * Z, X * Z, X * Z, X * Y, X * Y)[face]
+            o.set_bnd_field(face, rng.normal(size=(nb, 3)))
+        o.set_field(N.FIELD_VEL, u)
+        o.set_pressure(p)
+        o.set_field(N.FIELD_D, D)
+        o.run_stage(N.STAGE_PRES_PRECOMP)
+        r = {"Dp": o.field(N.FIELD_DP), "bp": o.field(N.FIELD_BP)}
+        o.run_stage(N.STAGE_PRES_SWEEPS)
+        r["pc"] = o.field(N.FIELD_PRES_CORR)
+        o.run_stage(N.STAGE_VEL_CORRECT)
+        r["uc"] = o.field(N.FIELD_VEL_CORR)
+        r["u"] = o.field(N.FIELD_VEL_WORK)
+        out[gt] = r
+    for k in out[GridType.Collocated]:
+        a, b = out[GridType.Collocated][k], out[GridType.CollNonUniform][k]
+        np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-12 * np.abs(a).max(), err_msg=k)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_convection_of_a_linear_field_is_exact(axis):
+    """nu = 0, a constant face velocity F on every face normal to `axis`, and
+    u_c = a + b x_c along that axis: linear interpolation of a linear field is
+    exact at the faces, so the convective balance of an interior cell is
+    ds F b h (h its own size).  It is read off one Jacobi sweep from u^n
+    itself: D (u^n - u_new) = sum of the convective face fluxes."""
+    grid = (8, 7, 9)
+    cf = stretched_config(grid, mu=0.0, vel=1)
+    set_bc(cf, [2] * 6, [1] * 6)
+    faces = faces_for(grid, seed=11)
+    o = make(cf, faces)
+    (cx, cy, cz), (hx, hy, hz) = centres(faces)
+    x, y, z = cell_coords(grid)
+    c = (cx[x], cy[y], cz[z])[axis]
+    h = (hx[x], hy[y], hz[z])
+    a, b, F = 0.7, 1.9, 2.3
+    u = np.zeros((o.n, 3))
+    u[:, 0] = a + b * c  # the x component is the transported one; y, z stay 0
+    o.set_velocity(u)
+    o.set_pressure(np.zeros(o.n))
+    flux = np.zeros((grid[2] + 1, grid[1] + 1, grid[0] + 1, 3))
+    flux[..., axis] = F
+    o.set_face_flux(flux)
+    o.run_stage(N.STAGE_VEL_PRECOMP)
+    D = o.field(N.FIELD_D)
+    o.run_stage(N.STAGE_VEL_SWEEPS)
+    un = o.field(N.FIELD_VEL_WORK)
+    R = D * (u - un)
+    area = {0: h[1] * h[2], 1: h[0] * h[2], 2: h[0] * h[1]}[axis]
+    lim = grid[axis]
+    ci = (x, y, z)[axis]
+    inner = (ci > 0) & (ci < lim - 1)
+    expect = area * F * b * h[axis]
+    np.testing.assert_allclose(R[inner, 0], expect[inner], rtol=1e-11)
+    np.testing.assert_allclose(R[inner, 1:], 0.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_diffusion_of_a_linear_field_with_linear_eddy_viscosity(axis):
+    """No convection (F = 0), LES on with nu_t = c0 + g x_c and u = a + b x_c
+    along `axis`: face viscosities interpolate nu_t exactly and the two-point
+    gradients are exact, so the diffusive balance of an interior cell is
+    -ds b (nu_t(face+) - nu_t(face-)) = -ds b g h."""
+    grid = (8, 9, 7)
+    nu = 0.05
+    cf = stretched_config(grid, les=True, mu=nu * DEN, vel=1)
+    set_bc(cf, [2] * 6, [1] * 6)
+    faces = faces_for(grid, seed=21)
+    o = make(cf, faces)
+    (cx, cy, cz), (hx, hy, hz) = centres(faces)
+    x, y, z = cell_coords(grid)
+    c = (cx[x], cy[y], cz[z])[axis]
+    h = (hx[x], hy[y], hz[z])
+    a, b, c0, g = 0.4, -1.3, 0.2, 0.11
+    u = np.zeros((o.n, 3))
+    u[:, 2] = a + b * c
+    o.set_velocity(u)
+    o.set_pressure(np.zeros(o.n))
+    o.set_field(N.FIELD_EDDY_VIS, c0 + g * c)
+    o.set_field(N.FIELD_B, np.zeros((o.n, 3)))  # the LES source term is pinned with the uniform kernels
+    o.run_stage(N.STAGE_VEL_PRECOMP)
+    D = o.field(N.FIELD_D)
+    o.run_stage(N.STAGE_VEL_SWEEPS)
+    R = D * (u - o.field(N.FIELD_VEL_WORK))
+    area = {0: h[1] * h[2], 1: h[0] * h[2], 2: h[0] * h[1]}[axis]
+    X, Y, Z = grid
+    # away from every boundary (a symmetry face adds its normal component's wall term)
+    inner = (x > 0) & (x < X - 1) & (y > 0) & (y < Y - 1) & (z > 0) & (z < Z - 1)
+    np.testing.assert_allclose(R[inner, 2], (-area * b * g * h[axis])[inner], rtol=1e-10)
+
+
+def test_pressure_precompute_linear_fields_closed_form():
+    """u* = a + b x_c (x component), p linear in x, D constant: in cells two
+    away from the x boundaries the Rhie-Chow correction vanishes (both
+    gradients exact), the stored W face velocity is a + b x_face, bp = b h_x,
+    and Dp = -(dv/den)(1/D) sum over faces of 1/distance between centres."""
+    grid = (10, 6, 7)
+    cf = stretched_config(grid)
+    set_bc(cf, [1] * 6, [1] * 6)
+    faces = faces_for(grid, seed=31)
+    o = make(cf, faces)
+    (cx, cy, cz), (hx, hy, hz) = centres(faces)
+    x, y, z = cell_coords(grid)
+    a, b, e, d = 0.3, 0.8, -2.1, 5.5
+    u = np.zeros((o.n, 3))
+    u[:, 0] = a + b * cx[x]
+    o.set_field(N.FIELD_VEL, u)
+    o.set_pressure(0.4 + e * cx[x])
+    o.set_field(N.FIELD_D, np.full((o.n, 3), d))
+    o.run_stage(N.STAGE_PRES_PRECOMP)
+    X, Y, Z = grid
+    inner = (x >= 2) & (x <= X - 3)
+    fx = np.asarray(faces[0], dtype=np.float64)
+    ff = o.face_flux()
+    uw = ff[z, y, x, 0]
+    np.testing.assert_allclose(uw[inner], (a + b * fx[x])[inner], rtol=1e-12)
+    np.testing.assert_allclose(o.field(N.FIELD_BP)[inner], (b * hx[x])[inner], rtol=1e-11)
+    dv = hx[x] * hy[y] * hz[z]
+
+    def inv_dist(c, i, n):
+        lo = np.where(i > 0, 1.0 / (c[i] - c[np.maximum(i - 1, 0)] + (i == 0)), 0.0)
+        hi = np.where(i < n - 1, 1.0 / (c[np.minimum(i + 1, n - 1)] - c[i] + (i == n - 1)), 0.0)
+        return lo + hi
+
+    s = inv_dist(cx, x, X) + inv_dist(cy, y, Y) + inv_dist(cz, z, Z)
+    np.testing.assert_allclose(o.field(N.FIELD_DP), -(dv / DEN) / d * s, rtol=1e-12)
+
+
+def test_pressure_sweep_linear_fixed_point_and_varying_D():
+    """bp = 0: a linear p' is a fixed point of an interior Jacobi sweep with
+    constant D (face coefficients go as 1/distance); with 1/D linear in x the
+    face value of 1/D is exact and one sweep moves p' by
+    (dv/den) e g h_x / sum_f c_f, with c_f = (dv/den) (1/D)(x_f) / distance."""
+    grid = (9, 6, 7)
+    cf = stretched_config(grid, pres=1)
+    set_bc(cf, [1] * 6, [1] * 6)
+    faces = faces_for(grid, seed=41)
+    (cx, cy, cz), (hx, hy, hz) = centres(faces)
+    x, y, z = cell_coords(grid)
+    X, Y, Z = grid
+    inner = (x > 0) & (x < X - 1) & (y > 0) & (y < Y - 1) & (z > 0) & (z < Z - 1)
+    e = 1.7
+    pc0 = 0.5 + e * cx[x]
+    for varying in (False, True):
+        o = make(cf, faces)
+        g = 0.013 if varying else 0.0
+        invD = 0.2 + g * cx[x]
+        o.set_field(N.FIELD_D, np.repeat((1.0 / invD)[:, None], 3, axis=1))
+        o.set_field(N.FIELD_VEL, np.zeros((o.n, 3)))
+        o.run_stage(N.STAGE_PRES_PRECOMP)
+        o.set_field(N.FIELD_BP, np.zeros(o.n))
+        o.set_field(N.FIELD_PRES_CORR, pc0)
+        o.run_stage(N.STAGE_PRES_SWEEPS)
+        pc = o.field(N.FIELD_PRES_CORR)
+        if not varying:
+            np.testing.assert_allclose(pc[inner], pc0[inner], rtol=1e-12)
+            continue
+        dvden = hx[x] * hy[y] * hz[z] / DEN
+        fxx = np.asarray(faces[0], dtype=np.float64)
+        inv_face = lambda xf: 0.2 + g * xf
+        csum = np.zeros(o.n)
+        # x faces: 1/D interpolated at the face position, distance between centres
+        csum += dvden * inv_face(fxx[x + 1]) / (cx[np.minimum(x + 1, X - 1)] - cx[x] + (x == X - 1))
+        csum += dvden * inv_face(fxx[x]) / (cx[x] - cx[np.maximum(x - 1, 0)] + (x == 0))
+        for c_, i_, n_ in ((cy, y, Y), (cz, z, Z)):
+            csum += dvden * invD / (c_[np.minimum(i_ + 1, n_ - 1)] - c_[i_] + (i_ == n_ - 1))
+            csum += dvden * invD / (c_[i_] - c_[np.maximum(i_ - 1, 0)] + (i_ == 0))
+        expect = pc0 + dvden * e * g * hx[x] / csum
+        np.testing.assert_allclose(pc[inner], expect[inner], rtol=1e-11)
+
+
+def test_velocity_correction_linear_pressure_on_stretched_cells():
+    """p' = e . x_c: the interpolated face values differ by e_a h_a along each
+    axis, so u'_a = -(dv/den) e_a / D_a in interior cells."""
+    grid = (8, 7, 6)
+    cf = stretched_config(grid)
+    set_bc(cf, [1] * 6, [1] * 6)
+    faces = faces_for(grid, seed=51)
+    o = make(cf, faces)
+    (cx, cy, cz), (hx, hy, hz) = centres(faces)
+    x, y, z = cell_coords(grid)
+    e = np.array([0.6, -1.1, 0.9])
+    Dv = np.array([5.0, 6.5, 4.0])
+    o.set_field(N.FIELD_D, np.tile(Dv, (o.n, 1)))
+    o.set_field(N.FIELD_VEL, np.zeros((o.n, 3)))
+    o.set_field(N.FIELD_PRES_CORR, e[0] * cx[x] + e[1] * cy[y] + e[2] * cz[z])
+    o.run_stage(N.STAGE_VEL_CORRECT)
+    uc = o.field(N.FIELD_VEL_CORR)
+    X, Y, Z = grid
+    inner = (x > 0) & (x < X - 1) & (y > 0) & (y < Y - 1) & (z > 0) & (z < Z - 1)
+    dvden = (hx[x] * hy[y] * hz[z] / DEN)[:, None]
+    np.testing.assert_allclose(uc[inner], (-dvden * e / Dv)[inner], rtol=1e-12)
+
+
+@pytest.mark.parametrize("xn_bc", [0, 1, 2])
+def test_outlet_face_velocity_is_stored_by_bc_type(xn_bc):
+    """The + faces on the domain boundary live in the extra texels x = X:
+    fixed value -> the boundary texture's normal component, zero gradient ->
+    the cell's u*, symmetry -> 0; the inlet face (own texel) likewise."""
+    grid = (6, 5, 4)
+    cf = stretched_config(grid)
+    set_bc(cf, [xn_bc, xn_bc, 2, 2, 2, 2], [1, 0, 1, 1, 1, 1])
+    faces = faces_for(grid, seed=61)
+    o = make(cf, faces)
+    X, Y, Z = grid
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=(o.n, 3))
+    bnd = {f: rng.normal(size=(Y * Z, 3)) for f in (0, 1)}
+    for f in (0, 1):
+        o.set_bnd_field(f, bnd[f])
+    o.set_field(N.FIELD_VEL, u)
+    o.set_pressure(np.zeros(o.n))
+    o.set_field(N.FIELD_D, np.full((o.n, 3), 3.0))
+    o.run_stage(N.STAGE_PRES_PRECOMP)
+    ff = o.face_flux()
+    u3 = u.reshape(Z, Y, X, 3)
+    got_out, got_in = ff[:Z, :Y, X, 0], ff[:Z, :Y, 0, 0]
+    if xn_bc == 0:
+        exp_out, exp_in = bnd[1][:, 0].reshape(Z, Y), bnd[0][:, 0].reshape(Z, Y)
+    elif xn_bc == 1:
+        exp_out, exp_in = u3[:, :, X - 1, 0], u3[:, :, 0, 0]
+    else:
+        exp_out = exp_in = np.zeros((Z, Y))
+    np.testing.assert_allclose(got_out, exp_out, rtol=1e-14)
+    np.testing.assert_allclose(got_in, exp_in, rtol=1e-14)
+
+
+@pytest.mark.parametrize("axis", [0, 2])
+def test_uniform_flow_is_a_fixed_point_of_the_stretched_step(axis):
+    """u = U e_axis, p = 0, fixed inlet / zero-gradient outlet with Dirichlet p
+    along `axis`, symmetry elsewhere, uneven spacing on every axis: the first
+    step (no face velocities yet) and the following ones (convection by the
+    stored face velocities, interior weights summing to one on each face)
+    return the same field."""
+    grid = (9, 6, 8)
+    cf = stretched_config(grid, mu=0.02, vel=6, pres=10)
+    vel = [2] * 6
+    pres = [1] * 6
+    vel[2 * axis], vel[2 * axis + 1] = 0, 1
+    pres[2 * axis + 1] = 0
+    set_bc(cf, vel, pres)
+    faces = faces_for(grid, seed=71)
+    o = make(cf, faces)
+    X, Y, Z = grid
+    U = 2.5
+    u = np.zeros((o.n, 3))
+    u[:, axis] = U
+    o.set_velocity(u)
+    o.set_pressure(np.zeros(o.n))
+    nb = (Y * Z, X * Y)[axis // 2]
+    ub = np.zeros((nb, 3))
+    ub[:, axis] = U
+    o.set_bnd_field(2 * axis, ub)
+    o.step(4)
+    np.testing.assert_allclose(o.velocity(), u, atol=1e-12)
+    np.testing.assert_allclose(o.pressure(), 0.0, atol=1e-12)
+    ff = o.face_flux()
+    sl = ff[:Z, :Y, :X, axis]
+    np.testing.assert_allclose(sl, U, rtol=1e-12)
+
+
+def test_log_profile_at_face_midpoints():
+    """CSInitLogVelPresFieldsNonUniform / CSSetLogVelBndCondNonUniform place
+    the log law at y = (f_j + f_j+1)/2 for both the field and the inlet."""
+    grid = (5, 8, 4)
+    cf = stretched_config(grid)
+    cf.inflowType = InflowType.Log
+    cf.Init()
+    faces = faces_for(grid, seed=81)
+    o = Oracle(cf, precision=1)
+    o.init_like_solver(wind_speed=6.0, face_positions=faces)
+    fy = np.asarray(faces[1], dtype=np.float64)
+    yc = 0.5 * (fy[1:] + fy[:-1])
+    fric = 0.4 * 6.0 / np.log(11.0)
+    prof = fric / 0.4 * np.log(yc + 1.0)
+    X, Y, Z = grid
+    u = o.velocity().reshape(Z, Y, X, 3)
+    np.testing.assert_allclose(u[..., 0], np.broadcast_to(prof[None, :, None], (Z, Y, X)), rtol=1e-6)
+    b = o.bnd_field(0).reshape(Z, Y, 3)
+    np.testing.assert_allclose(b[..., 0], np.broadcast_to(prof[None, :], (Z, Y)), rtol=1e-6)

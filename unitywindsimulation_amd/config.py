@@ -125,8 +125,8 @@ class FluidSimConfig:
 
     # ---- FluidSimConfig.Init (FluidSimConfig.cs:322-346) ----
     def Init(self) -> "FluidSimConfig":
-        if self.gridType != GridType.Collocated:
-            raise NotImplementedError("only GridType.Collocated is on the solver-step path")
+        if self.gridType == GridType.Staggered:
+            raise NotImplementedError("GridType.Staggered is not built (collocated and stretched grids only)")
         dx = f32(self.physDomainSize[0]) / f32(self.gridResX)
         ds = dx * dx
         dv = dx * dx * dx
@@ -193,6 +193,7 @@ class FluidSimConfig:
         c.refine_rounds = int(self.refineRounds)
         c.refine_sweeps = int(self.refineSweeps)
         c.compute_norms = int(bool(self.computeNorms))
+        c.grid_type = int(self.gridType)
         return c
 
 

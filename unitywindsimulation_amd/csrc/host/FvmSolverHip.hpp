@@ -32,6 +32,7 @@ enum class FvmSolverType : int32_t { SIMPLE = 0, PISO = 1 };
 enum class SolidType : int32_t { NoSolid = 0, Box = 1, Model = 2 };
 enum class TurbulenceModel : int32_t { None = 0, Smagorinsky = 1 };
 enum class InflowType : int32_t { Constant = 0, Log = 1, PowerLaw = 2 };
+enum class GridType : int32_t { Collocated = 0, CollNonUniform = 1, Staggered = 2 };
 
 struct float3 {
     float x, y, z;
@@ -64,6 +65,7 @@ struct FluidSimConfig {
     TurbulenceModel turbulenceModel = TurbulenceModel::None;
     float smagorinskyConstant = 0.15f;
     InflowType inflowType = InflowType::Constant;
+    GridType gridType = GridType::Collocated;
     bool calResidual = false;
     int32_t velResidualCheckInterval = 10;
     float velTolerance = 1e-15f;
@@ -129,6 +131,7 @@ struct FluidSimConfig {
         c.refine_rounds = refineRounds;
         c.refine_sweeps = refineSweeps;
         c.compute_norms = computeNorms ? 1 : 0;
+        c.grid_type = (int32_t)gridType;
         return c;
     }
 };
@@ -169,6 +172,7 @@ public:
         const uws_config c = cf_.to_c();
         check(uws_create(&c, &h_), "uws_create");
         n_ = (int64_t)cf_.gridRes.x * cf_.gridRes.y * cf_.gridRes.z;
+        if (cf_.gridType == GridType::CollNonUniform) InitMeshBuffers();
         InitVelPresFields();
         if (cf_.solidType == SolidType::Box) check(uws_init_box(h_), "uws_init_box");
         SetFixedValueVelBndCond();
@@ -178,6 +182,28 @@ public:
     }
     FvmSolverHip(const FvmSolverHip&) = delete;
     FvmSolverHip& operator=(const FvmSolverHip&) = delete;
+
+    // InitMeshBuffers (FvmSolverGpu.cs:682-788): the hard-coded stretched mesh,
+    // a uniform block of 50 cells per axis with 25 cells growing by 1.08 on each
+    // side along x and z; throws where the reference's tables would overflow
+    void InitMeshBuffers() {
+        const int32_t res[3] = {cf_.gridRes.x, cf_.gridRes.y, cf_.gridRes.z};
+        const int32_t uniform[3] = {50, 50, 50}, one_side[3] = {25, 0, 25};
+        std::vector<float> pos[3];
+        for (int a = 0; a < 3; ++a) {
+            pos[a].resize((size_t)res[a] + 1);
+            check(uws_stretched_face_positions(res[a], cf_.dx, uniform[a], one_side[a], 1.08f, pos[a].data()),
+                  "InitMeshBuffers");
+        }
+        SetFacePositions(pos[0], pos[1], pos[2]);
+    }
+    // stretched grid: any strictly increasing face coordinates (X+1, Y+1, Z+1)
+    void SetFacePositions(const std::vector<float>& fx, const std::vector<float>& fy, const std::vector<float>& fz) {
+        if (fx.size() != (size_t)cf_.gridRes.x + 1 || fy.size() != (size_t)cf_.gridRes.y + 1 ||
+            fz.size() != (size_t)cf_.gridRes.z + 1)
+            throw Error(UWS_ERR_INVALID_ARG, "SetFacePositions");
+        check(uws_set_face_positions(h_, fx.data(), fy.data(), fz.data()), "SetFacePositions");
+    }
 
     // whole steps only: always ready for rendering afterwards
     bool Step() override {

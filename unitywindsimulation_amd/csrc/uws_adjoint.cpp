@@ -659,7 +659,7 @@ int uws_adjoint_create(uws_handle h, int32_t nsteps, int32_t ckpt_every, uws_adj
     if (hipSetDevice(s->device) != hipSuccess) return UWS_ERR_DEVICE;
     const int prec = s->cfg.precision;
     if (prec != UWS_PREC_FP32 && prec != UWS_PREC_FP64) return UWS_ERR_UNSUPPORTED;
-    if (s->sc.cal_residual || s->sc.lowp || s->sc.normalise) return UWS_ERR_UNSUPPORTED;
+    if (s->sc.cal_residual || s->sc.lowp || s->sc.normalise || s->nonuniform()) return UWS_ERR_UNSUPPORTED;
     std::unique_ptr<AdjointBase> impl;
     const int rc = prec == UWS_PREC_FP64 ? make_adjoint<double>(s, nsteps, ckpt_every, &impl)
                                          : make_adjoint<float>(s, nsteps, ckpt_every, &impl);

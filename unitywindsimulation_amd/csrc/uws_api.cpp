@@ -46,7 +46,11 @@ int validate(const uws_config* c, bool allow_residual_break) {
         if (c->cal_residual) return UWS_ERR_UNSUPPORTED;
     }
     if (c->compute_norms != 0 && c->compute_norms != 1) return UWS_ERR_CONFIG;
-    for (int i = 0; i < 4; ++i)
+    if (c->grid_type == UWS_GRID_STAGGERED) return UWS_ERR_UNSUPPORTED;
+    if (c->grid_type != UWS_GRID_COLLOCATED && c->grid_type != UWS_GRID_COLL_NONUNIFORM) return UWS_ERR_CONFIG;
+    // the stretched grid has no PISO corrector (InitPisoShader throws, FvmSolverGpu.cs:515-519)
+    if (c->grid_type == UWS_GRID_COLL_NONUNIFORM && c->fvm_solver_type == UWS_PISO) return UWS_ERR_UNSUPPORTED;
+    for (int i = 0; i < 3; ++i)
         if (c->reserved[i] != 0) return UWS_ERR_CONFIG;
     return UWS_OK;
 }
@@ -108,6 +112,7 @@ void fill_params(const uws_config& c, ParamsT<T>& p) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 1024) p.zchunk = v;
     }
+    p.nonuniform = c.grid_type == UWS_GRID_COLL_NONUNIFORM ? 1 : 0;
 }
 template void fill_params<float>(const uws_config&, ParamsT<float>&);
 template void fill_params<double>(const uws_config&, ParamsT<double>&);
@@ -148,6 +153,15 @@ size_t carve(const ParamsT<T>& p, FieldsT<T>& f, char* base, bool lowp) {
     f.invDpiso = fl(N); f.bpiso = fl(N);
     for (int face = 0; face < 6; ++face)
         for (int c = 0; c < 3; ++c) f.bnd[face][c] = fl((size_t)bnd_w(p.X, p.Y, face) * bnd_h(p.Y, p.Z, face));
+    if (p.nonuniform) {
+        f.h[0] = fl(p.X); f.h[1] = fl(p.Y); f.h[2] = fl(p.Z);
+        for (int c = 0; c < 3; ++c) f.flux[c] = fl(N);
+        f.fluxB[0] = fl((size_t)p.Y * p.Z);
+        f.fluxB[1] = fl((size_t)p.X * p.Z);
+        f.fluxB[2] = fl((size_t)p.X * p.Y);
+    } else {
+        for (int c = 0; c < 3; ++c) f.h[c] = f.flux[c] = f.fluxB[c] = nullptr;
+    }
     if (lowp) {
         f.rho = fl(N);
         f.e16[0] = take(2 * N);
@@ -235,25 +249,34 @@ int create_solver(const uws_config& cfg, bool allow_residual_break, hipStream_t 
 
 // host-side inflow profiles shared by the single and ensemble entry points,
 // evaluated in the run's arithmetic T with the reference's float constants
+// cell-centre height of row y: (y + 1/2) dx, or on a stretched grid the face
+// midpoint 0.5 * (facePosY[y] + facePosY[y + 1]) (InitShaderColl.compute:141, 295)
+template <class T>
+T centre_height(const float* face_y, T dx, int y) {
+    if (face_y) return (T)0.5f * ((T)face_y[y] + (T)face_y[y + 1]);
+    return ((T)y + (T)0.5f) * dx;
+}
+
 template <class T>
 int inflow_column(int inflow_type, float wind_speed, float dx_f, int Y, float ref_height, float alpha,
-                  std::vector<T>& col) {
+                  std::vector<T>& col, const float* face_y) {
     col.assign((size_t)Y, (T)0);
     const T wind = (T)wind_speed, dx = (T)dx_f;
     if (inflow_type == UWS_INFLOW_CONSTANT) {
         for (int y = 0; y < Y; ++y) col[y] = wind;
     } else if (inflow_type == UWS_INFLOW_LOG) {
-        // CSSetLogVelBndCond (InitShaderColl.compute:272-285): y = (j + 1/2) dx
+        // CSSetLogVelBndCond (InitShaderColl.compute:272-285): y = (j + 1/2) dx;
+        // CSSetLogVelBndCondNonUniform (:287-300): the face midpoint
         const T karman = (T)0.4f, rough = (T)1.0f, refH = (T)10.0f;
         for (int y = 0; y < Y; ++y) {
-            const T yPhys = ((T)y + (T)0.5f) * dx;
+            const T yPhys = centre_height<T>(face_y, dx, y);
             const T fricVel = karman * wind / std::log((refH + rough) / rough);
             col[y] = fricVel / karman * std::log((yPhys + rough) / rough);
         }
     } else if (inflow_type == UWS_INFLOW_POWER_LAW) {
         if (!(ref_height > 0.f)) return UWS_ERR_INVALID_ARG;
         for (int y = 0; y < Y; ++y) {
-            const T yPhys = ((T)y + (T)0.5f) * dx;
+            const T yPhys = centre_height<T>(face_y, dx, y);
             col[y] = wind * (T)std::pow((double)(yPhys / (T)ref_height), (double)alpha);
         }
     } else {
@@ -278,7 +301,8 @@ int apply_fixed_value_bnd_t(SolverBase& s, int32_t inflow_type, float wind_speed
                             float ref_height, float alpha) {
     const int X = s.dims(0), Y = s.dims(1), Z = s.dims(2);
     std::vector<T> col;
-    if (int rc = inflow_column<T>(inflow_type, wind_speed, s.cfg.dx, Y, ref_height, alpha, col)) return rc;
+    const float* face_y = s.nonuniform() ? s.face_pos[1].data() : nullptr;
+    if (int rc = inflow_column<T>(inflow_type, wind_speed, s.cfg.dx, Y, ref_height, alpha, col, face_y)) return rc;
     const size_t nx0 = (size_t)Y * Z;
     std::vector<T> a(nx0), zero(nx0, (T)0);
     for (int z = 0; z < Z; ++z)
@@ -333,9 +357,10 @@ int init_vel_pres_t(SolverBase& s, int32_t inflow_type, float wind_speed) {
     std::vector<T> col((size_t)Y);
     for (int y = 0; y < Y; ++y) {
         if (inflow_type == UWS_INFLOW_LOG) {
-            // CSInitLogVelPresFields (InitShaderColl.compute:112-132): y = j*dx
+            // CSInitLogVelPresFields (InitShaderColl.compute:112-132): y = j*dx;
+            // CSInitLogVelPresFieldsNonUniform (:134-154): the face midpoint
             const T karman = (T)0.4f, rough = (T)1.0f, refH = (T)10.0f;
-            const T yPhys = (T)y * dx;
+            const T yPhys = s.nonuniform() ? centre_height<T>(s.face_pos[1].data(), dx, y) : (T)y * dx;
             const T fricVel = karman * wind / std::log((refH + rough) / rough);
             col[y] = fricVel / karman * std::log((yPhys + rough) / rough);
         } else {
@@ -370,6 +395,25 @@ int flags_from_int(SolverBase& s, const int32_t* flags, int64_t n) {
     std::vector<uint8_t> fl((size_t)n);
     for (int64_t i = 0; i < n; ++i) fl[i] = (double)flags[i] > 0.5 ? UWS_CELL_SOLID : UWS_CELL_FLUID;
     return s.set_flags_u8(fl.data());
+}
+
+// InitMeshBuffers (FvmSolverGpu.cs:682-788) for one axis
+int stretched_face_positions(int n, float dx, int uniform, int one_side, float ratio, float* out) {
+    if (n < 1 || uniform < 0 || one_side < 0 || !(dx > 0.f) || !(ratio > 0.f) || !out) return UWS_ERR_INVALID_ARG;
+    const int trailing = n - uniform - one_side;
+    if (trailing < 0 || trailing > one_side) return UWS_ERR_CONFIG;
+    // cell sizes away from the uniform block: dx, dx*ratio, dx*ratio^2, ... (float products, as the C# loop)
+    std::vector<float> grow((size_t)one_side);
+    if (one_side > 0) {
+        grow[0] = dx;
+        for (int i = 1; i < one_side; ++i) grow[i] = grow[i - 1] * ratio;
+    }
+    out[0] = 0.f;
+    int i = 1;
+    for (; i < one_side + 1; ++i) out[i] = out[i - 1] + grow[(size_t)(one_side - i)];
+    for (; i < one_side + uniform + 1; ++i) out[i] = out[i - 1] + dx;
+    for (; i < n + 1; ++i) out[i] = out[i - 1] + grow[(size_t)(i - (one_side + uniform + 1))];
+    return UWS_OK;
 }
 
 int normalise_norms(const double* raw, double ncells, double* out, int n) {
@@ -481,6 +525,28 @@ int uws_init_box(uws_handle h) {
                         fl[((size_t)z * Y + y) * X + x] = UWS_CELL_SOLID;
         return s.set_flags_u8(fl.data());
     });
+}
+
+int uws_set_face_positions(uws_handle h, const float* pos_x, const float* pos_y, const float* pos_z) {
+    if (!pos_x || !pos_y || !pos_z) return UWS_ERR_INVALID_ARG;
+    return with_solver(h, [&](SolverBase& s) {
+        const float* const pos[3] = {pos_x, pos_y, pos_z};
+        return s.set_face_positions(pos);
+    });
+}
+
+int uws_get_face_positions(uws_handle h, float* pos_x, float* pos_y, float* pos_z) {
+    if (!pos_x || !pos_y || !pos_z) return UWS_ERR_INVALID_ARG;
+    return with_solver(h, [&](SolverBase& s) {
+        if (!s.nonuniform()) return (int)UWS_ERR_CONFIG;
+        float* const dst[3] = {pos_x, pos_y, pos_z};
+        for (int a = 0; a < 3; ++a) std::copy(s.face_pos[a].begin(), s.face_pos[a].end(), dst[a]);
+        return (int)UWS_OK;
+    });
+}
+
+int uws_stretched_face_positions(int32_t n, float dx, int32_t uniform, int32_t one_side, float ratio, float* out) {
+    return stretched_face_positions(n, dx, uniform, one_side, ratio, out);
 }
 
 int uws_init_vel_pres_fields(uws_handle h, int32_t inflow_type, float wind_speed) {
@@ -646,7 +712,7 @@ int uws_get_time(uws_handle h, int64_t* step, double* phys_time) {
 int uws_field_ncomp(int32_t field) {
     switch (field) {
     case UWS_FIELD_VEL: case UWS_FIELD_VEL_WORK: case UWS_FIELD_VEL_CORR: case UWS_FIELD_WALL_VIS:
-    case UWS_FIELD_B: case UWS_FIELD_D: case UWS_FIELD_AODU:
+    case UWS_FIELD_B: case UWS_FIELD_D: case UWS_FIELD_AODU: case UWS_FIELD_FACE_VEL:
         return 3;
     case UWS_FIELD_PRES: case UWS_FIELD_PRES_CORR: case UWS_FIELD_EDDY_VIS: case UWS_FIELD_DP: case UWS_FIELD_BP:
         return 1;

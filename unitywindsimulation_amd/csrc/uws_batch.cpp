@@ -230,6 +230,7 @@ int uws_batch_create(const uws_config* cfgs, int32_t ncases, uws_batch_handle* o
     *out = nullptr;
     for (int c = 0; c < ncases; ++c) {
         if (int rc = validate(&cfgs[c], false)) return rc;
+        if (cfgs[c].grid_type != UWS_GRID_COLLOCATED) return UWS_ERR_UNSUPPORTED;
         if (cfgs[c].precision != UWS_PREC_FP32) return UWS_ERR_UNSUPPORTED;
         if (!same_schedule(cfgs[0], cfgs[c])) return UWS_ERR_UNSUPPORTED;
     }

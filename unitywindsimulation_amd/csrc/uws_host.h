@@ -49,6 +49,8 @@ struct SolverBase {
     double* h_norms = nullptr;
     int64_t step = 0;
     double phys_time = 0.0;
+    // stretched grid: facePosX/Y/Z (gridRes + 1 floats per axis); empty otherwise
+    std::vector<float> face_pos[3];
 
     virtual ~SolverBase();
     int sync();
@@ -77,6 +79,9 @@ struct SolverBase {
     virtual int mean_accumulate() = 0;
     virtual int get_mean(double* dst, int64_t n, int64_t* count) = 0;
     virtual int cstep_imag(bool set, int what, int face, double* buf, int64_t n) = 0;
+    // stretched grid: replace the face coordinates (validated) and upload the cell sizes
+    virtual int set_face_positions(const float* const pos[3]) = 0;
+    bool nonuniform() const { return cfg.grid_type == UWS_GRID_COLL_NONUNIFORM; }
 };
 // The two-sweep pressure kernel marches z-chunks of 64 x 13-cell tiles; on a
 // grid with fewer workgroups than two per CU (256 CUs) it leaves the chip
@@ -111,6 +116,7 @@ int apply_noise(SolverBase& s, uint64_t seed, float amp);
 int init_vel_pres(SolverBase& s, int32_t inflow_type, float wind_speed);
 int flags_from_int(SolverBase& s, const int32_t* flags, int64_t n);
 int normalise_norms(const double* raw, double ncells, double* out, int n);
+int stretched_face_positions(int n, float dx, int uniform, int one_side, float ratio, float* out);
 
 // ---------------------------------------------------------------------------
 // Step driver.  L is a launcher: l(kernel, roles, slot) enqueues one kernel
@@ -296,6 +302,7 @@ int run_stage(Lr& l, const StepCfg& c, RoleState& rs, int stage) {
         if (c.norms) {
             Roles q{};
             q.vout = rs.cur; q.state = rs.cur;
+            q.flags = K_NO_FLUX;  // the stretched grid's face velocities stay those of u* for the next step
             l(KN_PRES_PRECOMP, q);
             l(KN_DIVERGENCE, q);
             l(KN_SUM_PARTIALS, q, UWS_NORM_DIVERGENCE);

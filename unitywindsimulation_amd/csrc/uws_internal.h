@@ -28,6 +28,7 @@ struct ParamsT {
     T wallZ0;
     int normRef;          // flat index of the pressure normalisation cell, -1 when it is inactive
     int zchunk;           // z planes per workgroup of the two-sweep kernels
+    int nonuniform;       // 1: stretched grid (GridType.CollNonUniform), spacings from FieldsT::h
 };
 
 // Device fields, structure-of-arrays: every vector field is three planes of
@@ -60,6 +61,10 @@ struct FieldsT {
     T* invDpiso;          // 1/presCoeff of the PISO corrector equation
     T* bpiso;             // velTerm of the PISO corrector equation
     T* bnd[6][3];         // boundary textures, [face][comp][j*W + i]
+    // stretched grid only (null otherwise)
+    T* h[3];              // cell sizes along x (X values), y (Y), z (Z): facePos[i+1] - facePos[i]
+    T* flux[3];           // velFaceFluxField: Rhie-Chow velocity of each cell's W / S / D face
+    T* fluxB[3];          // ... of the + faces on the domain boundary: x = X [z*Y + y], y = Y [z*X + x], z = Z [y*X + x]
     // defect correction with low-precision correction storage
     T* rho;               // defect right-hand side
     void* e16[2];         // correction iterates, 2-byte storage
@@ -99,7 +104,8 @@ struct Roles {
 enum {
     K_WALL = 1, K_EDDY = 2, K_FUSE_LES = 4, K_LOWP_FIRST = 16, K_LOWP_LAST = 32,
     K_BF16 = 64,
-    K_NO_INPLACE = 128   // corrections write u' only, not u += u' / p += p'
+    K_NO_INPLACE = 128,  // corrections write u' only, not u += u' / p += p'
+    K_NO_FLUX = 256      // stretched-grid pressure precompute: leave velFaceFluxField as it is
 };
 
 // kernel identifiers understood by launch()
@@ -109,7 +115,9 @@ enum Kernel {
     KN_ZERO_SOLID_VEL, KN_FILL_PC,
     KN_DEFECT_RESIDUAL, KN_LOWP_SWEEP,
     KN_RES_VEL, KN_RES_PRES, KN_DIVERGENCE, KN_NONFINITE, KN_SUM_PARTIALS, KN_DEFECT_SCALE,
-    KN_PRES_SWEEP2   // two pressure Jacobi sweeps in one launch (single-case only)
+    KN_PRES_SWEEP2,  // two pressure Jacobi sweeps in one launch (single-case only)
+    // stretched-grid variants, selected by launch() when ParamsT::nonuniform is set
+    KN_NU_VEL_PRECOMP, KN_NU_VEL_SWEEP, KN_NU_PRES_PRECOMP, KN_NU_VEL_CORRECT, KN_NU_DIVERGENCE
 };
 
 // Single-case launch of kernel k on stream s.  `slot` selects the norms

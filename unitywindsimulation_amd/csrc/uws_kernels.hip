@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "uws_internal.h"
 
@@ -467,7 +468,9 @@ __device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, co
 // advection terms E W N S U D, then diffusion terms E W N S U D, then
 // (1/D) * (adv + diff + b).  Non-fluid cells are not written.
 // ---------------------------------------------------------------------------
-template <class T>
+// NU: stretched grid, where the convective coefficient of each axis is the
+// stored face value times the cell's own size along that axis instead of ds/4
+template <class T, bool NU = false>
 __device__ void body_vel_sweep(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
@@ -483,6 +486,12 @@ __device__ void body_vel_sweep(const ParamsT<T>& p, const FieldsT<T>& f, const C
     const T SS = c.y > 0 ? f.S[1][iS] : (T)0, GS = c.y > 0 ? f.G[1][iS] : (T)0;
     const T SD = c.z > 0 ? f.S[2][iD] : (T)0, GD = c.z > 0 ? f.G[2][iD] : (T)0;
     const T q = p.ds * (T)0.25f;
+    T qx = q, qy = q, qz = q;
+    if constexpr (NU) {
+        qx = f.h[0][c.x];
+        qy = f.h[1][c.y];
+        qz = f.h[2][c.z];
+    }
     T* const* kk = f.vel[r.vin];
     T* const* oo = f.vel[r.vout];
 #pragma unroll
@@ -490,12 +499,12 @@ __device__ void body_vel_sweep(const ParamsT<T>& p, const FieldsT<T>& f, const C
         const T* __restrict__ k = kk[a];
         const T vE = k[iE], vW = k[iW], vN = k[iN], vS = k[iS], vU = k[iU], vD = k[iD];
         T A = 0;
-        A -= q * vE * SE;
-        A += q * vW * SW;
-        A -= q * vN * SN;
-        A += q * vS * SS;
-        A -= q * vU * SU;
-        A += q * vD * SD;
+        A -= qx * vE * SE;
+        A += qx * vW * SW;
+        A -= qy * vN * SN;
+        A += qy * vS * SS;
+        A -= qz * vU * SU;
+        A += qz * vD * SD;
         T Df = 0;
         Df += GE * vE;
         Df += GW * vW;
@@ -938,6 +947,8 @@ __device__ double body_nonfinite(const ParamsT<T>& p, const FieldsT<T>& f, const
     return (isfinite(a) && isfinite(b) && isfinite(d) && isfinite(q)) ? 0.0 : 1.0;
 }
 
+#include "uws_nonuniform.h"
+
 template <class T, int K>
 __device__ __forceinline__ void run_body(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if constexpr (K == KN_WALL_EDDY) body_wall_eddy(p, f, c, r);
@@ -954,6 +965,10 @@ __device__ __forceinline__ void run_body(const ParamsT<T>& p, const FieldsT<T>& 
     else if constexpr (K == KN_PISO_CORRECT) body_piso_correct(p, f, c, r);
     else if constexpr (K == KN_ZERO_SOLID_VEL) body_zero_solid_vel(p, f, c, r);
     else if constexpr (K == KN_FILL_PC) body_fill_pc(p, f, c, r);
+    else if constexpr (K == KN_NU_VEL_PRECOMP) body_nu_vel_precompute(p, f, c, r);
+    else if constexpr (K == KN_NU_VEL_SWEEP) body_vel_sweep<T, true>(p, f, c, r);
+    else if constexpr (K == KN_NU_PRES_PRECOMP) body_nu_pres_precompute(p, f, c, r);
+    else if constexpr (K == KN_NU_VEL_CORRECT) body_nu_vel_correct(p, f, c, r);
     else if constexpr (K == KN_LOWP_SWEEP) {
         if constexpr (sizeof(T) == 4) {
             if (r.flags & K_BF16) body_lowp_sweep<T, true>(p, f, c, r);
@@ -969,6 +984,7 @@ __device__ __forceinline__ double run_reduce(const ParamsT<T>& p, const FieldsT<
     else if constexpr (K == KN_DIVERGENCE) return body_divergence(p, f, c, r);
     else if constexpr (K == KN_NONFINITE) return body_nonfinite(p, f, c, r);
     else if constexpr (K == KN_DEFECT_RESIDUAL) return body_defect_residual(p, f, c, r);
+    else if constexpr (K == KN_NU_DIVERGENCE) return body_nu_divergence(p, f, c, r);
     else return 0.0;
 }
 
@@ -1290,6 +1306,19 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
         hipLaunchKernelGGL(k_single_sum<T>, dim3(1), dim3(TX, 4, 1), 0, s, p, f, slot,
                            k == KN_DEFECT_SCALE ? 1 : 0);
         return;
+    }
+    if constexpr (std::is_same_v<T, float> || std::is_same_v<T, double> || std::is_same_v<T, CStep>) {
+        if (p.nonuniform) {
+            // stretched grid: its own precomputes, momentum sweep, correction and divergence
+            switch (k) {
+            case KN_VEL_PRECOMP: go_single<T, KN_NU_VEL_PRECOMP>(p, f, r, s); return;
+            case KN_VEL_SWEEP: go_single<T, KN_NU_VEL_SWEEP>(p, f, r, s); return;
+            case KN_PRES_PRECOMP: go_single<T, KN_NU_PRES_PRECOMP>(p, f, r, s); return;
+            case KN_VEL_CORRECT: go_single<T, KN_NU_VEL_CORRECT>(p, f, r, s); return;
+            case KN_DIVERGENCE: go_single_reduce<T, KN_NU_DIVERGENCE>(p, f, r, s); return;
+            default: break;
+            }
+        }
     }
 #define GO1(K) go_single<T, K>(p, f, r, s)
 #define GOR1(K) go_single_reduce<T, K>(p, f, r, s)

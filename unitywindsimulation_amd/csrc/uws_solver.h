@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <type_traits>
 #include <vector>
@@ -74,7 +75,52 @@ struct Solver : SolverBase {
         if (hipMalloc(&slab, slab_bytes) != hipSuccess) { slab = nullptr; return UWS_ERR_ALLOC; }
         carve(p, f, (char*)slab, step_cfg(c).lowp);
         if (hipMemsetAsync(slab, 0, slab_bytes, stream) != hipSuccess) return UWS_ERR_DEVICE;
-        return sync();
+        if (int rc = sync()) return rc;
+        if (p.nonuniform) {
+            // uniform faces i*dx until the caller sets its own
+            std::vector<float> pos[3];
+            const int n[3] = {p.X, p.Y, p.Z};
+            for (int a = 0; a < 3; ++a) {
+                pos[a].resize((size_t)n[a] + 1);
+                for (int i = 0; i <= n[a]; ++i) pos[a][i] = (float)i * c.dx;
+            }
+            const float* const pp[3] = {pos[0].data(), pos[1].data(), pos[2].data()};
+            return set_face_positions(pp);
+        }
+        return UWS_OK;
+    }
+
+    // stretched grid: facePosX/Y/Z, strictly increasing; the kernels read the
+    // cell sizes facePos[i+1] - facePos[i] formed here in T
+    int set_face_positions(const float* const pos[3]) override {
+        if (!p.nonuniform) return UWS_ERR_CONFIG;
+        const int n[3] = {p.X, p.Y, p.Z};
+        for (int a = 0; a < 3; ++a)
+            for (int i = 0; i < n[a]; ++i)
+                if (!(pos[a][i + 1] > pos[a][i]) || !std::isfinite(pos[a][i + 1])) return UWS_ERR_INVALID_ARG;
+        for (int a = 0; a < 3; ++a) {
+            face_pos[a].assign(pos[a], pos[a] + n[a] + 1);
+            std::vector<T> h((size_t)n[a]);
+            for (int i = 0; i < n[a]; ++i) h[i] = (T)pos[a][i + 1] - (T)pos[a][i];
+            HIP_OK(hipMemcpyAsync(f.h[a], h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, stream));
+            if (int rc = sync()) return rc;
+        }
+        return UWS_OK;
+    }
+    // dv/den of every cell in T (the factor folded into the stretched grid's pressure equation)
+    std::vector<T> dvden_host() const {
+        std::vector<T> r((size_t)p.N);
+        std::vector<T> h[3];
+        const int n[3] = {p.X, p.Y, p.Z};
+        for (int a = 0; a < 3; ++a) {
+            h[a].resize((size_t)n[a]);
+            for (int i = 0; i < n[a]; ++i) h[a][i] = (T)face_pos[a][i + 1] - (T)face_pos[a][i];
+        }
+        for (int z = 0; z < p.Z; ++z)
+            for (int y = 0; y < p.Y; ++y)
+                for (int x = 0; x < p.X; ++x)
+                    r[((size_t)z * p.Y + y) * p.X + x] = h[0][x] * h[1][y] * h[2][z] / p.den;
+        return r;
     }
 
     int ncells() const override { return p.N; }
@@ -183,6 +229,10 @@ struct Solver : SolverBase {
         case UWS_FIELD_EDDY_VIS: planes[0] = f.nut; return 1;
         case UWS_FIELD_DP: planes[0] = f.Dp; return 1;
         case UWS_FIELD_BP: planes[0] = f.bp; return 1;
+        case UWS_FIELD_FACE_VEL:
+            if (!p.nonuniform || for_write) return UWS_ERR_INVALID_ARG;
+            for (int c = 0; c < 3; ++c) planes[c] = f.flux[c];
+            return 3;
         default: return UWS_ERR_INVALID_ARG;
         }
     }
@@ -197,6 +247,15 @@ struct Solver : SolverBase {
                 for (int64_t i = 0; i < n; ++i) dst[i] = 0;
                 return UWS_OK;
             }
+            if (field == UWS_FIELD_BP && p.nonuniform) {
+                // the stretched grid stores bp / (dv/den): unfold to bFieldPresCorrect
+                std::vector<T> tmp((size_t)p.N);
+                HIP_OK(hipMemcpyAsync(tmp.data(), planes[0], sizeof(T) * tmp.size(), hipMemcpyDeviceToHost, stream));
+                if (int rc = sync()) return rc;
+                const std::vector<T> k = dvden_host();
+                for (int64_t i = 0; i < n; ++i) dst[i] = (H)(tmp[(size_t)i] * k[(size_t)i]);
+                return UWS_OK;
+            }
             return down(dst, planes[0], p.N);
         }
         return get_vec(dst, planes, 3, p.N);
@@ -209,6 +268,13 @@ struct Solver : SolverBase {
         const int nc = field_planes(field, planes, true);
         if (nc < 0) return nc;
         if (n != (int64_t)nc * p.N) return UWS_ERR_INVALID_ARG;
+        if (field == UWS_FIELD_BP && p.nonuniform) {
+            const std::vector<T> k = dvden_host();
+            std::vector<T> tmp((size_t)p.N);
+            for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = (T)src[i] / k[(size_t)i];
+            HIP_OK(hipMemcpyAsync(planes[0], tmp.data(), sizeof(T) * tmp.size(), hipMemcpyHostToDevice, stream));
+            return sync();
+        }
         if (nc == 1) return up(planes[0], src, p.N);
         return set_vec(planes, src, 3, p.N);
     }

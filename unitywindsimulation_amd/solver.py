@@ -16,7 +16,34 @@ from ctypes import byref, c_double, c_int64, c_void_p
 import numpy as np
 
 from . import _native as N
-from .config import FluidSimConfig, InflowType, RuntimeConfig
+from .config import FluidSimConfig, GridType, InflowType, RuntimeConfig
+
+# InitMeshBuffers' hard-coded stretched mesh (FvmSolverGpu.cs:692-697): a
+# uniform block of 50 cells per axis, 25 growing cells before it along x and
+# z, none along y, growth ratio 1.08
+MESH_UNIFORM_SIZE = (50, 50, 50)
+MESH_ONE_SIDE = (25, 0, 25)
+MESH_RATIO = 1.08
+
+
+def stretched_face_positions(n: int, dx: float, uniform: int, one_side: int, ratio: float) -> np.ndarray:
+    """One axis of InitMeshBuffers (FvmSolverGpu.cs:682-788): n + 1 face
+    coordinates (float32).  Raises ValueError where the reference's generator
+    indexes outside its tables."""
+    out = np.zeros(n + 1, dtype=np.float32)
+    rc = N.lib().uws_stretched_face_positions(int(n), float(dx), int(uniform), int(one_side), float(ratio),
+                                              N.ptr(out))
+    if rc == N.UWS_ERR_CONFIG:
+        raise ValueError(f"stretched mesh: {n} cells do not fit a uniform block of {uniform} "
+                         f"with {one_side} growing cells on the leading side")
+    N.check(rc, "uws_stretched_face_positions")
+    return out
+
+
+def reference_mesh(cf: FluidSimConfig):
+    """The face coordinates FvmSolverGpu builds for GridType.CollNonUniform."""
+    return [stretched_face_positions(n, float(cf.dx), u, o, MESH_RATIO)
+            for n, u, o in zip(cf.gridRes, MESH_UNIFORM_SIZE, MESH_ONE_SIDE)]
 
 
 class IFvmSolver(abc.ABC):
@@ -57,12 +84,14 @@ class FvmSolverHip(IFvmSolver):
     """FvmSolverGpu's behaviour on MI355X through ``libuws_hip.so``.
 
     Constructor = ``FvmSolverGpu(FluidSimConfig, RuntimeConfig)`` (:173-222):
-    fields allocated, flags all fluid, fields initialised by
-    ``InitVelPresFields``, boundary textures zeroed then filled by
-    ``SetFixedValueVelBndCond``; ``SolidType.Box`` applies the hard-coded box.
+    fields allocated, on a stretched grid the face coordinates set
+    (``InitMeshBuffers``, the reference's hard-coded mesh unless
+    ``face_positions`` gives three arrays), flags all fluid, fields
+    initialised by ``InitVelPresFields``, boundary textures zeroed then filled
+    by ``SetFixedValueVelBndCond``; ``SolidType.Box`` applies the hard-coded box.
     """
 
-    def __init__(self, cf: FluidSimConfig, rcf: RuntimeConfig | None = None):
+    def __init__(self, cf: FluidSimConfig, rcf: RuntimeConfig | None = None, face_positions=None):
         self.cf = cf
         cf.Init()
         self.rcf = rcf or RuntimeConfig.from_config(cf)
@@ -74,6 +103,10 @@ class FvmSolverHip(IFvmSolver):
         X, Y, Z = cf.gridRes
         self.shape = (Z, Y, X)
         self.n = X * Y * Z
+        if int(cf.gridType) == int(GridType.CollNonUniform):
+            self.set_face_positions(*(face_positions if face_positions is not None else reference_mesh(cf)))
+        elif face_positions is not None:
+            raise ValueError("face positions need GridType.CollNonUniform")
         self.InitVelPresFields()
         if int(cf.solidType) == 1:  # SolidType.Box (FvmSolverGpu.cs:216-217)
             N.check(self._lib.uws_init_box(self._h), "uws_init_box")
@@ -96,6 +129,21 @@ class FvmSolverHip(IFvmSolver):
             self.close()
         except Exception:
             pass
+
+    # ---- stretched grid ----
+    def set_face_positions(self, fx, fy, fz) -> None:
+        """facePosX/Y/Z: X+1, Y+1, Z+1 strictly increasing coordinates."""
+        Z, Y, X = self.shape
+        a = [np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(-1)) for v in (fx, fy, fz)]
+        if [v.size for v in a] != [X + 1, Y + 1, Z + 1]:
+            raise ValueError("face positions need X+1, Y+1, Z+1 entries")
+        N.check(self._lib.uws_set_face_positions(self._h, *[N.ptr(v) for v in a]), "uws_set_face_positions")
+
+    def face_positions(self):
+        Z, Y, X = self.shape
+        a = [np.zeros(n + 1, dtype=np.float32) for n in (X, Y, Z)]
+        N.check(self._lib.uws_get_face_positions(self._h, *[N.ptr(v) for v in a]), "uws_get_face_positions")
+        return a
 
     # ---- IFvmSolver ----
     def Step(self) -> bool:

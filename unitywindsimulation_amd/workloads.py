@@ -28,8 +28,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .config import (FluidSimConfig, FvmSolverType, InflowType, PresBndCond, TurbulenceModel, VelBndCond,
-                     WallModel)
+from .config import (FluidSimConfig, FvmSolverType, GridType, InflowType, PresBndCond, SolidType, TurbulenceModel,
+                     VelBndCond, WallModel)
 from .geometry import Box, SplitMix64, lattice_layout, rasterise
 
 U_REF, H_REF, ALPHA = 5.0, 16.0, 0.25
@@ -46,6 +46,7 @@ class Workload:
     steps: int = 200
     mean_last: int = 0
     boxes: list = field(default_factory=list)
+    face_positions: list | None = None  # stretched grid: facePosX/Y/Z (float32)
 
     @property
     def grid(self):
@@ -205,8 +206,73 @@ def differentiation(grid=(128, 64, 96), n=20, lattice=(5, 4), seed=4, nsteps=64,
     return DiffWorkload(w, nsteps, ckpt_every, region, delta)
 
 
+def growing_faces(n: int, dx: float, ratio: float) -> np.ndarray:
+    """n + 1 face coordinates (float32) of cells dx, dx*ratio, dx*ratio^2, ...
+    from 0 upwards: a ground-refined vertical axis."""
+    sizes = np.empty(n, dtype=np.float32)
+    sizes[0] = np.float32(dx)
+    for i in range(1, n):
+        sizes[i] = sizes[i - 1] * np.float32(ratio)
+    out = np.zeros(n + 1, dtype=np.float32)
+    for i in range(n):
+        out[i + 1] = out[i] + sizes[i]
+    return out
+
+
+def tiny_stretched(grid=(16, 10, 12), seed=7, les=True, vel_iters=6, pres_iters=8, ratio=1.12) -> Workload:
+    """tiny() on a stretched grid (GridType.CollNonUniform, SIMPLE: the
+    stretched grid has no PISO corrector).  x and z use the reference's
+    InitMeshBuffers layout scaled down (a uniform core with cells growing by
+    `ratio` on both sides), y grows upward from the ground by `ratio`."""
+    from .solver import stretched_face_positions
+
+    X, Y, Z = grid
+    w = tiny(grid=grid, seed=seed, les=les, piso=False, vel_iters=vel_iters, pres_iters=pres_iters)
+    cf = w.cf
+    cf.gridType = GridType.CollNonUniform
+    cf.Init()
+    fx = stretched_face_positions(X, 1.0, X - 2 * (X // 3), X // 3, ratio)
+    fy = growing_faces(Y, 1.0, ratio)
+    fz = stretched_face_positions(Z, 1.0, Z - 2 * (Z // 3), Z // 3, ratio)
+    w.name = f"tiny_stretched_{X}x{Y}x{Z}"
+    w.face_positions = [fx, fy, fz]
+    return w
+
+
+def wind_tunnel_box(vel_iters=800, pres_iters=2000, steps=20) -> Workload:
+    """The reference's stretched-grid case, Code/Config/WindTunnelBoxConfig.asset:
+    100 x 50 x 100 cells, dx = 1 m (physDomainSize 100 x 50 x 100), dt = 0.02 s,
+    mu = 1, den = 1, uniform inflow 10 m/s at X0, zero-gradient outlet with
+    fixed pressure, symmetry elsewhere, the hard-coded box x in [20, 24),
+    y in [0, 8), z in [18, 22), laminar SIMPLE with 800 momentum and 2000
+    pressure sweeps, on InitMeshBuffers' mesh (a 50-cell uniform core with 25
+    cells growing by 1.08 on both sides along x and z, FvmSolverGpu.cs:682-788).
+    The asset's initial field is the constant inflow velocity."""
+    from .solver import reference_mesh
+
+    cf = FluidSimConfig(
+        physDomainSize=(100.0, 50.0, 100.0), gridResX=100, dt=0.02, mu=1.0, den=1.0,
+        velBndCondX0=VelBndCond.FixedValue, velBndCondXn=VelBndCond.ZeroGrad, velBndCondY0=VelBndCond.Symmetry,
+        velBndCondYn=VelBndCond.Symmetry, velBndCondZ0=VelBndCond.Symmetry, velBndCondZn=VelBndCond.Symmetry,
+        presBndCondX0=PresBndCond.ZeroGrad, presBndCondXn=PresBndCond.FixedValue,
+        presBndCondY0=PresBndCond.Symmetry, presBndCondYn=PresBndCond.Symmetry,
+        presBndCondZ0=PresBndCond.Symmetry, presBndCondZn=PresBndCond.Symmetry,
+        velX0=(10.0, 0.0, 0.0), solidType=SolidType.Box, fvmSolverType=FvmSolverType.SIMPLE,
+        turbulenceModel=TurbulenceModel.None_, inflowType=InflowType.Constant, gridType=GridType.CollNonUniform,
+        velMaxNumIter=vel_iters, presMaxNumIter=pres_iters,
+    )
+    cf.Init()
+    X, Y, Z = cf.gridRes
+    flags = np.zeros((Z, Y, X), dtype=np.int32)
+    flags[18:22, 0:8, 20:24] = 1
+    return Workload(f"wind_tunnel_box_stretched_{X}x{Y}x{Z}", cf, flags.reshape(-1), noise_seed=0, noise_amp=0.0,
+                    steps=steps, boxes=[Box(20, 24, 18, 22, 8)], face_positions=reference_mesh(cf))
+
+
 WORKLOADS = {
     "isolated_building": isolated_building,
     "urban": urban,
     "tiny": tiny,
+    "tiny_stretched": tiny_stretched,
+    "wind_tunnel_box": wind_tunnel_box,
 }
