@@ -22,13 +22,15 @@ METRIC = "MLUPS per PISO step + %HBM peak; batched vs plain cases/s at ≤1e-3 r
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per cell per launch read by rocprofv3 FETCH_SIZE (x2, calibrated on a
 # 256 MiB copy) + WRITE_SIZE on the urban case: profiles/urban_256x96x128_hbm_traffic.md
-MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 75.4, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.4}}
+MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 75.5, "k_pres_sweep": 29.7, "k_pres_sweep2": 36.7}}
 
 
-def bytes_per_cell_step(V: int, P: int, C: int) -> int:
+def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
     """Algorithmic HBM bytes per cell per step: vectors 12 B, scalars 4 B,
-    flag 1 B, each array counted once per read and per write per kernel."""
-    return 263 + 77 * V + 29 * P + C * (123 + 33 * P)
+    flag 1 B, each array counted once per read and per write per kernel.
+    Without LES the wall function (25 B), eddy viscosity (17 B) and LES term
+    (41 B) do not run."""
+    return 263 - (0 if les else 83) + 77 * V + 29 * P + C * (123 + 33 * P)
 
 
 def cpu_baseline(budget_s: float = 15.0) -> dict:
@@ -327,6 +329,43 @@ def adjoint_report(ndir: int = 16, h_fd=(1e-3, 1e-6)) -> dict:
     return out
 
 
+def stretched_report(steps: int = 5, warmup: int = 2) -> dict:
+    """The reference's stretched-grid case: WindTunnelBoxConfig (100 x 50 x 100,
+    laminar SIMPLE, 800 momentum and 2000 pressure sweeps per step) on
+    InitMeshBuffers' mesh (25 cells growing by 1.08 on both sides of a
+    50-cell uniform block along x and z)."""
+    import torch
+
+    from unitywindsimulation_amd import FvmSolverHip, workloads
+    from unitywindsimulation_amd import _native as N
+
+    w = workloads.wind_tunnel_box()
+    cf = w.cf
+    X, Y, Z = w.grid
+    n = X * Y * Z
+    V, P = cf.velMaxNumIter, cf.presMaxNumIter
+    s = FvmSolverHip(cf, face_positions=w.face_positions)
+    s.InitFlags(w.flags)
+    s.step(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    b = bytes_per_cell_step(V, P, 0, les=False)
+    t_vel = s.time_kernel(N.KERNEL_VEL_SWEEP, 20)
+    t_pres = s.time_kernel(N.KERNEL_PRES_SWEEP, 20)
+    u = s.velocity(3)
+    s.close()
+    return {"workload": w.name, "grid_xyz": [X, Y, Z], "cells": n, "vel_sweeps": V, "pres_sweeps": P,
+            "steps": steps, "ms_per_step": round(ms, 3), "mlups": round(n / (ms * 1e-3) / 1e6, 2),
+            "bytes_per_cell_step": b, "hbm_frac": round(b * n / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "kernel_ms": {"k_vel_sweep (stretched)": round(t_vel, 5), "k_pres_sweep": round(t_pres, 5)},
+            "domain_m": [float(v[-1]) for v in w.face_positions],
+            "finite": bool(torch.isfinite(torch.from_numpy(u)).all()),
+            "max_abs_u": float(abs(u).max())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -342,6 +381,8 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the precision leg (FP32 / FP16 / BF16 correction storage vs an FP64 run, 500 steps)")
     ap.add_argument("--precision-steps", type=int, default=500)
+    ap.add_argument("--no-stretched", action="store_true",
+                    help="skip the stretched-grid leg (the reference's wind-tunnel box on its stretched mesh)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM counters)")
     args = ap.parse_args()
@@ -360,8 +401,10 @@ def main() -> int:
     cf = w.cf
     X, Y, Z = w.grid
     ncell = X * Y * Z
-    V, P, C = cf.velMaxNumIter, cf.presMaxNumIter, cf.PISONumCorrectors
-    s = FvmSolverHip(cf)
+    V, P = cf.velMaxNumIter, cf.presMaxNumIter
+    C = cf.PISONumCorrectors if int(cf.fvmSolverType) == 1 else 0
+    les = int(cf.turbulenceModel) == 1
+    s = FvmSolverHip(cf, face_positions=w.face_positions)
     s.InitFlags(w.flags)
     s.init_noise(w.noise_seed, w.noise_amp)
     s.step(args.warmup)
@@ -374,7 +417,7 @@ def main() -> int:
     dt = time.perf_counter() - t0
     ms_step = dt / args.steps * 1e3
     mlups = ncell / (ms_step * 1e-3) / 1e6
-    b_step = bytes_per_cell_step(V, P, C)
+    b_step = bytes_per_cell_step(V, P, C, les)
     hbm_frac = b_step * ncell / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     # dominant kernel: the Jacobi sweep kernel with the larger share of step time.  The step runs the
@@ -415,7 +458,8 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "grid_xyz": [X, Y, Z], "cells": ncell, "vel_sweeps": V,
-                   "pres_sweeps": P, "piso_correctors": C, "les": "smagorinsky", "wall": "rough log law",
+                   "pres_sweeps": P, "piso_correctors": C, "les": "smagorinsky" if les else "none",
+                   "wall": ("rough log law" if int(cf.wallModel) == 1 else "smooth log law") if les else "none",
                    "bytes_per_cell_step": b_step},
         "hbm_frac": round(hbm_frac, 4),
         "algorithmic_GBps": round(b_step * ncell / (ms_step * 1e-3) / 1e9, 1),
@@ -428,6 +472,8 @@ def main() -> int:
     }
     if not args.no_ensemble:
         out["ensemble"] = ensemble_report()
+    if not args.no_stretched:
+        out["stretched"] = stretched_report()
     if not args.no_precision:
         out["precision"] = precision_report(args.precision_steps, args.precision_steps // 2)
     if not args.no_adjoint:
