@@ -1024,11 +1024,11 @@ constexpr int PS2_THREADS = PS2_W * PS2_BY;  // 990
 // one plane's inputs of the first sweep at a cell: its coefficients and the
 // previous iterate at its six neighbours (clamped at the domain boundary,
 // where the coefficient is zero)
+// the raw loads of one plane at a thread's column (masked when used)
 template <class T>
 struct PS2Plane {
-    T cE, cW, cN, cS, cU, cD, inv, rhs;
-    T vE, vW, vN, vS, vU, vD;
-    bool fl;
+    T cE, cW, cN, cS, cU, inv, rhs;
+    T vE, vW, vN, vS, vU;
 };
 
 inline __host__ __device__ int pres_sweep2_blocks(int X, int Y, int Z, int zchunk) {
@@ -1036,6 +1036,19 @@ inline __host__ __device__ int pres_sweep2_blocks(int X, int Y, int Z, int zchun
 }
 
 // the two-sweep block b of one grid (b in [0, pres_sweep2_blocks))
+//
+// Latency: the loads of plane zz + 1 are issued before the first sweep of
+// plane zz and consumed one iteration later.  They are unconditional (indices
+// clamped into the grid, values masked where they are used), and the three
+// plane records rotate through three named variables (the loop is unrolled
+// three times), so no branch and no register copy makes an iteration wait
+// for the loads still in flight.  The D-face values come from the column's
+// own records: the D coefficient of plane zz is the U coefficient of plane
+// zz - 1, and the iterate below it is the U-neighbour value loaded with
+// plane zz - 2.  No flag is read: a non-fluid cell has 1/Dp = bp = 0 and zero
+// coefficients on all its faces, so both sweeps give it 0 (+0 or -0; zeros
+// are only ever multiplied by the zero coefficients of its neighbours), and
+// values of cells outside the grid are never used by an output cell.
 template <class T, bool ZIN>
 __device__ __forceinline__ void pres_sweep2_block(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, int b,
                                                   T* lds) {
@@ -1054,81 +1067,95 @@ __device__ __forceinline__ void pres_sweep2_block(const ParamsT<T>& p, const Fie
     T* __restrict__ out = f.pc[r.pout];
     const int sy = p.X, sz = p.X * p.Y;
     const bool hW = x > 0, hE = x + 1 < p.X, hS = y > 0, hN = y + 1 < p.Y;
-    // loads of plane zz, issued one plane ahead of their use so that their
-    // latency overlaps the barrier and the second sweep of the current plane
-    auto load = [&](int zz) {
-        PS2Plane<T> q{};
-        q.fl = false;
-        if (cell && zz >= 0 && zz < p.Z) {
-            const int P = (zz * p.Y + y) * p.X + x;
-            q.fl = f.flag[P] == UWS_CELL_FLUID;
-            if (q.fl) {
-                const bool hD = zz > 0, hU = zz + 1 < p.Z;
-                q.cE = k.cE[P]; q.cN = k.cN[P]; q.cU = k.cU[P];
-                q.cW = hW ? k.cE[P - 1] : (T)0;
-                q.cS = hS ? k.cN[P - sy] : (T)0;
-                q.cD = hD ? k.cU[P - sz] : (T)0;
-                q.inv = k.invDp[P];
-                q.rhs = k.rhs[P];
-                if (!ZIN) {
-                    q.vE = x0[hE ? P + 1 : P];
-                    q.vW = x0[hW ? P - 1 : P];
-                    q.vN = x0[hN ? P + sy : P];
-                    q.vS = x0[hS ? P - sy : P];
-                    q.vU = x0[hU ? P + sz : P];
-                    q.vD = x0[hD ? P - sz : P];
-                }
-            }
+    const int xc = min(max(x, 0), p.X - 1), yc = min(max(y, 0), p.Y - 1);
+    // unconditional loads of plane zz (clamped into the grid) into q; the U
+    // neighbour is read at plane zz + 1 clamped on its own, so that the record
+    // of plane zz - 2 always holds the D neighbour of plane zz (at the top
+    // face it is the cell itself, multiplied by a zero coefficient)
+    // neighbour flags of the clamped column: the loads of threads outside the
+    // grid stay inside it (their values are never used)
+    const bool gE = xc + 1 < p.X, gW = xc > 0, gN = yc + 1 < p.Y, gS = yc > 0;
+    auto load = [&](PS2Plane<T>& q, int zz) {
+        const int zc = min(max(zz, 0), p.Z - 1);
+        const int P = (zc * p.Y + yc) * p.X + xc;
+        const int PU = (min(max(zz + 1, 0), p.Z - 1) * p.Y + yc) * p.X + xc;
+        q.cE = k.cE[P]; q.cN = k.cN[P]; q.cU = k.cU[P];
+        q.cW = k.cE[max(P - 1, 0)];
+        q.cS = k.cN[max(P - sy, 0)];
+        q.inv = k.invDp[P];
+        q.rhs = k.rhs[P];
+        if (!ZIN) {
+            q.vE = x0[gE ? P + 1 : P];
+            q.vW = x0[gW ? P - 1 : P];
+            q.vN = x0[gN ? P + sy : P];
+            q.vS = x0[gS ? P - sy : P];
+            q.vU = x0[PU];
         }
-        return q;
     };
-    // own-column state: first-sweep values of the two previous planes and the
-    // coefficients of the previous plane (used by the second sweep)
+    // a face on the domain boundary has no coefficient (the loaded neighbour
+    // value is scaled by 0; coefficients are >= +0, so this is the +0 of a select)
+    const T mW = hW ? (T)1 : (T)0, mS = hS ? (T)1 : (T)0;
+    // own-column first-sweep values of the two previous planes
     T x1m1 = 0, x1m2 = 0;
-    PS2Plane<T> prev{};
-    prev.fl = false;
-    PS2Plane<T> cur = load(z0 - 1);
-    for (int zz = z0 - 1; zz <= z1; ++zz) {
-        const PS2Plane<T> nxt = zz + 1 <= z1 ? load(zz + 1) : PS2Plane<T>{};
-        T x1 = 0;
-        if (cur.fl) {
-            T term = 0;
-            if (!ZIN) {
-                term -= cur.cE * cur.vE;
-                term -= cur.cW * cur.vW;
-                term -= cur.cN * cur.vN;
-                term -= cur.cS * cur.vS;
-                term -= cur.cU * cur.vU;
-                term -= cur.cD * cur.vD;
-            }
-            x1 = cur.inv * (term + cur.rhs);
+    // one plane: loads of zz + 1 into nx, first sweep of zz from cu, second
+    // sweep of zz - 1 from pv (its coefficients) and the LDS ring
+    T pdU = 0;  // U coefficient of the plane below the previous one (the previous plane's D coefficient)
+    auto step = [&](const PS2Plane<T>& pv, const PS2Plane<T>& cu, PS2Plane<T>& nx, int zz) {
+        // nx still holds plane zz - 2, whose U neighbour is this column at zz - 1
+        // (at the bottom face a finite value times the zero D coefficient)
+        T vD = 0;
+        if (!ZIN) vD = nx.vU;
+        load(nx, zz + 1 <= z1 ? zz + 1 : z1);
+        const T mD = zz > 0 ? (T)1 : (T)0;
+        T term = 0;
+        if (!ZIN) {
+            term -= cu.cE * cu.vE;
+            term -= (cu.cW * mW) * cu.vW;
+            term -= cu.cN * cu.vN;
+            term -= (cu.cS * mS) * cu.vS;
+            term -= cu.cU * cu.vU;
+            term -= (pv.cU * mD) * vD;
         }
+        const T x1 = cu.inv * (term + cu.rhs);
         lds[((zz + 3) % 3) * PS2_THREADS + tid] = x1;
         __syncthreads();
         const int zo = zz - 1;
         if (inner && zo >= z0 && zo < z1) {
             const int P = (zo * p.Y + y) * p.X + x;
-            T v = 0;
-            if (prev.fl) {
-                const T* L = lds + ((zo + 3) % 3) * PS2_THREADS;
-                const T vE = hE ? L[tid + 1] : x1m1, vW = hW ? L[tid - 1] : x1m1;
-                const T vN = hN ? L[tid + PS2_W] : x1m1, vS = hS ? L[tid - PS2_W] : x1m1;
-                const T vU = zo + 1 < p.Z ? x1 : x1m1, vD = zo > 0 ? x1m2 : x1m1;
-                T term = 0;
-                term -= prev.cE * vE;
-                term -= prev.cW * vW;
-                term -= prev.cN * vN;
-                term -= prev.cS * vS;
-                term -= prev.cU * vU;
-                term -= prev.cD * vD;
-                v = prev.inv * (term + prev.rhs);
-            }
-            out[P] = v;
+            const T* L = lds + ((zo + 3) % 3) * PS2_THREADS;
+            const T lE = L[tid + 1], lW = L[tid - 1], lN = L[tid + PS2_W], lS = L[tid - PS2_W];
+            const T vE = hE ? lE : x1m1, vW = hW ? lW : x1m1;
+            const T vN = hN ? lN : x1m1, vS = hS ? lS : x1m1;
+            const T vU = zo + 1 < p.Z ? x1 : x1m1, vD = zo > 0 ? x1m2 : x1m1;
+            const T mDo = zo > 0 ? (T)1 : (T)0;
+            T t2 = 0;
+            t2 -= pv.cE * vE;
+            t2 -= (pv.cW * mW) * vW;
+            t2 -= pv.cN * vN;
+            t2 -= (pv.cS * mS) * vS;
+            t2 -= pv.cU * vU;
+            t2 -= (pdU * mDo) * vD;
+            out[P] = pv.inv * (t2 + pv.rhs);
         }
         x1m2 = x1m1;
         x1m1 = x1;
-        prev = cur;
-        cur = nxt;
+        pdU = pv.cU;
+    };
+    // records on entry of step(pv, cu, nx, zz): pv = plane zz - 1, cu = plane
+    // zz, nx = plane zz - 2; so A = plane z0 - 2 (its U coefficient is the D
+    // coefficient of plane z0 - 1), B = plane z0 - 1, C = plane z0 - 3 (its U
+    // neighbour is the iterate below z0 - 1)
+    PS2Plane<T> A, B, C;
+    load(A, z0 - 2);
+    load(B, z0 - 1);
+    load(C, z0 - 3);
+    for (int zz = z0 - 1;;) {
+        step(A, B, C, zz);
+        if (++zz > z1) break;
+        step(B, C, A, zz);
+        if (++zz > z1) break;
+        step(C, A, B, zz);
+        if (++zz > z1) break;
     }
 }
 
@@ -1137,6 +1164,16 @@ __global__ __launch_bounds__(1024) void k_pres_sweep2(ParamsT<T> p, FieldsT<T> f
     __shared__ __align__(16) unsigned char lds_raw[3 * PS2_THREADS * sizeof(T)];
     const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
     pres_sweep2_block<T, ZIN>(p, f, r, xcd_tile((int)blockIdx.x, nb), reinterpret_cast<T*>(lds_raw));
+}
+// FP32: held to 64 VGPRs, so that 8 waves run per SIMD and two 16-wave
+// workgroups share a CU (the FP32 body fits without spilling)
+template <bool ZIN>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_pres_sweep2_f32(ParamsT<float> p,
+                                                                                                  FieldsT<float> f,
+                                                                                                  Roles r) {
+    __shared__ __align__(16) float lds_f[3 * PS2_THREADS];
+    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+    pres_sweep2_block<float, ZIN>(p, f, r, xcd_tile((int)blockIdx.x, nb), lds_f);
 }
 
 
@@ -1298,8 +1335,13 @@ template <class T>
 void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s, int slot) {
     if (k == KN_PRES_SWEEP2) {
         const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
-        if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep2<T, true>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
-        else hipLaunchKernelGGL((k_pres_sweep2<T, false>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+        if constexpr (std::is_same_v<T, float>) {
+            if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep2_f32<true>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+            else hipLaunchKernelGGL((k_pres_sweep2_f32<false>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+        } else {
+            if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep2<T, true>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+            else hipLaunchKernelGGL((k_pres_sweep2<T, false>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+        }
         return;
     }
     if (k == KN_SUM_PARTIALS || k == KN_DEFECT_SCALE) {
