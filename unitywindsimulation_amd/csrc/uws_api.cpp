@@ -78,6 +78,22 @@ StepCfg step_cfg(const uws_config& c) {
     return s;
 }
 
+// z planes per two-sweep workgroup: each workgroup recomputes one halo plane
+// above and below its chunk, so longer chunks waste less, as long as the
+// grid still gives one workgroup per CU; a chunk that divides Z avoids a
+// ragged last chunk.  Measured on 256 x 96 x 128 (k_pres_sweep2): zchunk 16
+// (256 workgroups) 22.8 us, 8 (512) 24.2 us, 12 and 24 (ragged) 29.5-30 us.
+int choose_zchunk(int X, int Y, int Z) {
+    int best = 8, best_score = -1;
+    for (int zc : {16, 8, 4}) {
+        const int blocks = pres_sweep2_grid(X, Y, Z, zc);
+        if (blocks < FUSE_MIN_BLOCKS) continue;
+        const int score = (Z % zc == 0 ? 2 : 0) + (zc == 16 ? 1 : 0);
+        if (score > best_score) { best = zc; best_score = score; }
+    }
+    return best;
+}
+
 template <class T>
 void fill_params(const uws_config& c, ParamsT<T>& p) {
     p.X = c.grid_res[0]; p.Y = c.grid_res[1]; p.Z = c.grid_res[2];
@@ -107,7 +123,7 @@ void fill_params(const uws_config& c, ParamsT<T>& p) {
     // (FvmSolverGpu.cs:1425-1436); its cell (2, Y/2, 2) outside the grid reads 0
     p.normRef = (!anyFixed && 2 < p.X && 2 < p.Z) ? (2 * p.Y + p.Y / 2) * p.X + 2 : -1;
     // z planes per workgroup of the two-sweep kernels (UWS_ZCHUNK overrides, for tuning)
-    p.zchunk = 8;
+    p.zchunk = choose_zchunk(p.X, p.Y, p.Z);
     if (const char* e = std::getenv("UWS_ZCHUNK")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 1024) p.zchunk = v;

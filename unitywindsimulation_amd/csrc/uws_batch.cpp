@@ -89,6 +89,26 @@ struct uws_batch {
         }
         for (auto& g : groups) {
             const int m = (int)g.cases.size();
+            // z chunk of the group's two-sweep launches, chosen over the whole
+            // group as choose_zchunk does for one grid: the longest of 16, 8
+            // and 4 planes that still gives a workgroup per CU, preferring one
+            // that divides every case's Z (UWS_ZCHUNK overrides, for tuning)
+            if (!std::getenv("UWS_ZCHUNK")) {
+                int zbest = 8, sbest = -1;
+                for (int zc : {16, 8, 4}) {
+                    int blocks = 0;
+                    bool divides = true;
+                    for (int c : g.cases) {
+                        const ParamsT<float>& p = params(c);
+                        blocks += pres_sweep2_grid(p.X, p.Y, p.Z, zc);
+                        divides = divides && p.Z % zc == 0;
+                    }
+                    if (blocks < FUSE_MIN_BLOCKS) continue;
+                    const int score = (divides ? 2 : 0) + (zc == 16 ? 1 : 0);
+                    if (score > sbest) { zbest = zc; sbest = score; }
+                }
+                for (int c : g.cases) params(c).zchunk = zbest;
+            }
             std::vector<int> first(m + 1, 0), first2(m + 1, 0);
             std::vector<ParamsT<float>> ps(m);
             std::vector<FieldsT<float>> fs(m);

@@ -84,9 +84,9 @@ struct SolverBase {
     bool nonuniform() const { return cfg.grid_type == UWS_GRID_COLL_NONUNIFORM; }
 };
 // The two-sweep pressure kernel marches z-chunks of 64 x 13-cell tiles; on a
-// grid with fewer workgroups than two per CU (256 CUs) it leaves the chip
-// idle and two one-sweep launches are faster.
-constexpr int FUSE_MIN_BLOCKS = 512;
+// grid with fewer workgroups than CUs (256) it leaves the chip idle and two
+// one-sweep launches are faster.
+constexpr int FUSE_MIN_BLOCKS = 256;
 // UWS_SWEEP_FUSION in the environment: 0 = one-sweep kernels only, 2 = the
 // two-sweep kernel at any grid size (for tests), otherwise by grid size
 inline int fusion_mode() {
@@ -117,6 +117,7 @@ int init_vel_pres(SolverBase& s, int32_t inflow_type, float wind_speed);
 int flags_from_int(SolverBase& s, const int32_t* flags, int64_t n);
 int normalise_norms(const double* raw, double ncells, double* out, int n);
 int stretched_face_positions(int n, float dx, int uniform, int one_side, float ratio, float* out);
+int choose_zchunk(int X, int Y, int Z);
 
 // ---------------------------------------------------------------------------
 // Step driver.  L is a launcher: l(kernel, roles, slot) enqueues one kernel

@@ -1049,9 +1049,28 @@ inline __host__ __device__ int pres_sweep2_blocks(int X, int Y, int Z, int zchun
 // coefficients on all its faces, so both sweeps give it 0 (+0 or -0; zeros
 // are only ever multiplied by the zero coefficients of its neighbours), and
 // values of cells outside the grid are never used by an output cell.
+// what one two-sweep workgroup reads: the grid, the coefficient set and the
+// p' buffers (a compact copy, so that an ensemble launch keeps no more
+// registers than a single-case one)
+template <class T>
+struct PS2Args {
+    int X, Y, Z, zchunk;
+    PresCoef<T> k;
+    const T* x0;
+    T* out;
+};
+template <class T>
+__device__ __forceinline__ PS2Args<T> ps2_args(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r) {
+    PS2Args<T> a;
+    a.X = p.X; a.Y = p.Y; a.Z = p.Z; a.zchunk = p.zchunk;
+    a.k = pres_coef(f, r.coef);
+    a.x0 = r.pin >= 0 ? f.pc[r.pin] : nullptr;
+    a.out = f.pc[r.pout];
+    return a;
+}
+
 template <class T, bool ZIN>
-__device__ __forceinline__ void pres_sweep2_block(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, int b,
-                                                  T* lds) {
+__device__ __forceinline__ void pres_sweep2_block(const PS2Args<T>& p, int b, T* lds) {
     const int tid = (int)threadIdx.x;
     const int lx = tid % PS2_W, ly = tid / PS2_W;
     const int ntx = (p.X + 63) / 64, nty = (p.Y + PS2_BY - 3) / (PS2_BY - 2);
@@ -1062,9 +1081,9 @@ __device__ __forceinline__ void pres_sweep2_block(const ParamsT<T>& p, const Fie
     const int z0 = bz * p.zchunk, z1 = min(z0 + p.zchunk, p.Z);
     const bool cell = x >= 0 && x < p.X && y >= 0 && y < p.Y;
     const bool inner = cell && lx >= 1 && lx <= 64 && ly >= 1 && ly <= PS2_BY - 2;
-    const PresCoef<T> k = pres_coef(f, r.coef);
-    const T* __restrict__ x0 = ZIN ? nullptr : f.pc[r.pin];
-    T* __restrict__ out = f.pc[r.pout];
+    const PresCoef<T> k = p.k;
+    const T* __restrict__ x0 = ZIN ? nullptr : p.x0;
+    T* __restrict__ out = p.out;
     const int sy = p.X, sz = p.X * p.Y;
     const bool hW = x > 0, hE = x + 1 < p.X, hS = y > 0, hN = y + 1 < p.Y;
     const int xc = min(max(x, 0), p.X - 1), yc = min(max(y, 0), p.Y - 1);
@@ -1163,7 +1182,7 @@ template <class T, bool ZIN>
 __global__ __launch_bounds__(1024) void k_pres_sweep2(ParamsT<T> p, FieldsT<T> f, Roles r) {
     __shared__ __align__(16) unsigned char lds_raw[3 * PS2_THREADS * sizeof(T)];
     const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
-    pres_sweep2_block<T, ZIN>(p, f, r, xcd_tile((int)blockIdx.x, nb), reinterpret_cast<T*>(lds_raw));
+    pres_sweep2_block<T, ZIN>(ps2_args(p, f, r), xcd_tile((int)blockIdx.x, nb), reinterpret_cast<T*>(lds_raw));
 }
 // FP32: held to 64 VGPRs, so that 8 waves run per SIMD and two 16-wave
 // workgroups share a CU (the FP32 body fits without spilling)
@@ -1173,7 +1192,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
                                                                                                   Roles r) {
     __shared__ __align__(16) float lds_f[3 * PS2_THREADS];
     const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
-    pres_sweep2_block<float, ZIN>(p, f, r, xcd_tile((int)blockIdx.x, nb), lds_f);
+    pres_sweep2_block<float, ZIN>(ps2_args(p, f, r), xcd_tile((int)blockIdx.x, nb), lds_f);
 }
 
 
@@ -1199,7 +1218,8 @@ __global__ __launch_bounds__(1024) void k_batch_pres_sweep2(BatchTable<T> tab, R
         const int mid = (lo + hi + 1) >> 1;
         if (tab.first_ps2[mid] <= t) lo = mid; else hi = mid - 1;
     }
-    pres_sweep2_block<T, ZIN>(tab.params[lo], tab.fields[lo], r, t - tab.first_ps2[lo], reinterpret_cast<T*>(lds_raw));
+    pres_sweep2_block<T, ZIN>(ps2_args(tab.params[lo], tab.fields[lo], r), t - tab.first_ps2[lo],
+                              reinterpret_cast<T*>(lds_raw));
 }
 
 // Persistent ensemble launch: the grid is a multiple of 8 workgroups, XCD j
