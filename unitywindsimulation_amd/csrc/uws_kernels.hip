@@ -102,6 +102,63 @@ __device__ __forceinline__ bool fluid(const FieldsT<T>& f, int i) { return f.fla
 template <class T>
 __device__ __forceinline__ bool solid(const FieldsT<T>& f, int i) { return f.flag[i] == UWS_CELL_SOLID; }
 
+// The six neighbours of cell c in the face order E W N S U D: their index
+// (the cell itself where the face is on the domain boundary), whether they
+// lie in the grid, and whether they are fluid.  All six flags are loaded
+// unconditionally, so the loads that depend on them can be issued together.
+template <class T>
+__device__ __forceinline__ void nbr6(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, int Q[6], bool in[6],
+                                     bool fl[6]) {
+    const int P = c.P;
+    in[E_] = c.x + 1 < p.X; in[W_] = c.x > 0;
+    in[N_] = c.y + 1 < p.Y; in[S_] = c.y > 0;
+    in[U_] = c.z + 1 < p.Z; in[D_] = c.z > 0;
+    const int st[6] = {1, -1, p.X, -p.X, p.X * p.Y, -p.X * p.Y};
+    uint8_t fv[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        Q[d] = in[d] ? P + st[d] : P;
+        fv[d] = f.flag[Q[d]];
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d) fl[d] = in[d] && fv[d] == UWS_CELL_FLUID;
+}
+// nbr6 plus which neighbours are solid cells in the grid
+struct Nbr6 {
+    int Q[6];
+    bool in[6], fl[6], sl[6];
+};
+template <class T>
+__device__ __forceinline__ Nbr6 nbrs_of(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c) {
+    Nbr6 n;
+    const int P = c.P;
+    n.in[E_] = c.x + 1 < p.X; n.in[W_] = c.x > 0;
+    n.in[N_] = c.y + 1 < p.Y; n.in[S_] = c.y > 0;
+    n.in[U_] = c.z + 1 < p.Z; n.in[D_] = c.z > 0;
+    const int st[6] = {1, -1, p.X, -p.X, p.X * p.Y, -p.X * p.Y};
+    uint8_t fv[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        n.Q[d] = n.in[d] ? P + st[d] : P;
+        fv[d] = f.flag[n.Q[d]];
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        n.fl[d] = n.in[d] && fv[d] == UWS_CELL_FLUID;
+        n.sl[d] = n.in[d] && fv[d] == UWS_CELL_SOLID;
+    }
+    return n;
+}
+
+// the neighbours of cell c that are fluid cells in the grid (index: the cell
+// itself for the other faces)
+template <class T>
+__device__ __forceinline__ void fluid_nbrs(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, int Qs[6],
+                                           bool ok[6]) {
+    bool in[6];
+    nbr6(p, f, c, Qs, in, ok);
+}
+
 // ---------------------------------------------------------------------------
 // LogLawWallFunc (WallFunc.compute:77-123); the rough-wall branch is an
 // extension: u_tau = kappa u_P / ln((dx/2)/z0).
@@ -135,40 +192,44 @@ __device__ T log_law_wall_func(const ParamsT<T>& p, T uP) {
     return eddyVis;
 }
 
-// Get{E,W,N,S,U,D}FaceVelLastTime (LES.compute:104-258)
+// Get{E,W,N,S,U,D}FaceVelLastTime (LES.compute:104-258): the six face
+// velocities of a fluid cell -- the average with a fluid neighbour, 0 at a
+// solid one, and by BC type on the domain boundary (fixed: the boundary
+// texture, zero gradient: u_P, symmetry: u_P without its normal component) --
+// from neighbour data loaded unconditionally (Q, in, fl from nbr6)
 template <class T>
-__device__ __forceinline__ V3<T> face_vel(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], int x, int y,
-                                          int z, int dir) {
-    const int nx = x + c_off[dir][0], ny = y + c_off[dir][1], nz = z + c_off[dir][2];
-    const int P = lin(p, x, y, z);
-    const V3<T> uP = ld3(un, P);
-    if (inreg(p, nx, ny, nz)) {
-        const int Q = lin(p, nx, ny, nz);
-        if (fluid(f, Q)) {
-            const V3<T> uQ = ld3(un, Q);
-            return mk<T>((T)0.5f * (uP.x + uQ.x), (T)0.5f * (uP.y + uQ.y), (T)0.5f * (uP.z + uQ.z));
+__device__ __forceinline__ void face_vel6(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], const Cell& c,
+                                          const int Q[6], const bool in[6], const bool fl[6], V3<T> fv[6]) {
+    const V3<T> uP = ld3(un, c.P);
+    V3<T> uQ[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) uQ[d] = ld3(un, Q[d]);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int t = p.velBnd[bface_of(d)];
+        V3<T> bv = mk<T>(0, 0, 0);
+        if (t == UWS_VEL_FIXED_VALUE) {
+            bv = bnd_at(p, f, d, c.x, c.y, c.z);
+        } else if (t == UWS_VEL_ZERO_GRAD) {
+            bv = uP;
+        } else if (t == UWS_VEL_SYMMETRY) {
+            bv = uP;
+            if (d < 2) bv.x = 0; else if (d < 4) bv.y = 0; else bv.z = 0;
         }
-        return mk<T>(0, 0, 0);
+        const V3<T> avg = mk<T>((T)0.5f * (uP.x + uQ[d].x), (T)0.5f * (uP.y + uQ[d].y), (T)0.5f * (uP.z + uQ[d].z));
+        const V3<T> zero = mk<T>(0, 0, 0);
+        fv[d] = in[d] ? (fl[d] ? avg : zero) : bv;
     }
-    const int t = p.velBnd[bface_of(dir)];
-    if (t == UWS_VEL_FIXED_VALUE) return bnd_at(p, f, dir, x, y, z);
-    if (t == UWS_VEL_ZERO_GRAD) return uP;
-    if (t == UWS_VEL_SYMMETRY) {
-        V3<T> v = uP;
-        if (dir < 2) v.x = 0; else if (dir < 4) v.y = 0; else v.z = 0;
-        return v;
-    }
-    return mk<T>(0, 0, 0);
 }
 
 // cell velocity gradient g[i][j] = d u_i / d x_j from face velocities
 // (LES.compute:294-306, 353-365)
 template <class T>
-__device__ __forceinline__ void cell_grad(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], int x, int y,
-                                          int z, T g[3][3]) {
-    const V3<T> e = face_vel(p, f, un, x, y, z, E_), w = face_vel(p, f, un, x, y, z, W_);
-    const V3<T> n = face_vel(p, f, un, x, y, z, N_), s = face_vel(p, f, un, x, y, z, S_);
-    const V3<T> u = face_vel(p, f, un, x, y, z, U_), d = face_vel(p, f, un, x, y, z, D_);
+__device__ __forceinline__ void cell_grad(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], const Cell& c,
+                                          const int Q[6], const bool in[6], const bool fl[6], T g[3][3]) {
+    V3<T> fv[6];
+    face_vel6(p, f, un, c, Q, in, fl, fv);
+    const V3<T> e = fv[E_], w = fv[W_], n = fv[N_], s = fv[S_], u = fv[U_], d = fv[D_];
     g[0][0] = (e.x - w.x) / p.dx; g[0][1] = (n.x - s.x) / p.dx; g[0][2] = (u.x - d.x) / p.dx;
     g[1][0] = (e.y - w.y) / p.dx; g[1][1] = (n.y - s.y) / p.dx; g[1][2] = (u.y - d.y) / p.dx;
     g[2][0] = (e.z - w.z) / p.dx; g[2][1] = (n.z - s.z) / p.dx; g[2][2] = (u.z - d.z) / p.dx;
@@ -176,47 +237,26 @@ __device__ __forceinline__ void cell_grad(const ParamsT<T>& p, const FieldsT<T>&
 
 // face viscosities with eddy viscosity (FvmColl.compute:1596-1635; LES.compute:643-679)
 template <class T>
-__device__ __forceinline__ void face_vis(const ParamsT<T>& p, const FieldsT<T>& f, int x, int y, int z, int P,
+__device__ __forceinline__ void face_vis(const ParamsT<T>& p, const FieldsT<T>& f, const Nbr6& nb, int P,
                                          T vis[6]) {
     const T nu = p.nu;
     const T nutP = f.nut[P];
     const T base = nu + nutP;
+    // E/N/U: a solid neighbour contributes its own W/S/D slot; W/S/D: this cell's slot
+    T nutQ[6], wall[6];
 #pragma unroll
-    for (int d = 0; d < 6; ++d) vis[d] = base;
-    // E: a solid neighbour contributes its own .x slot; W: this cell's .x slot (N/S, U/D alike)
-    if (x + 1 < p.X) {
-        const int Q = P + 1;
-        if (fluid(f, Q)) vis[E_] = nu + (T)0.5f * (nutP + f.nut[Q]);
-        else if (solid(f, Q)) vis[E_] = nu + f.nuw[0][Q];
+    for (int d = 0; d < 6; ++d) {
+        nutQ[d] = f.nut[nb.Q[d]];
+        wall[d] = f.nuw[d >> 1][(d & 1) == 0 ? nb.Q[d] : P];
     }
-    if (x - 1 >= 0) {
-        const int Q = P - 1;
-        if (fluid(f, Q)) vis[W_] = nu + (T)0.5f * (nutP + f.nut[Q]);
-        else if (solid(f, Q)) vis[W_] = nu + f.nuw[0][P];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const T vf = nu + (T)0.5f * (nutP + nutQ[d]);
+        const T vs = nu + wall[d];
+        vis[d] = nb.fl[d] ? vf : (nb.sl[d] ? vs : base);
     }
-    if (y + 1 < p.Y) {
-        const int Q = P + p.X;
-        if (fluid(f, Q)) vis[N_] = nu + (T)0.5f * (nutP + f.nut[Q]);
-        else if (solid(f, Q)) vis[N_] = nu + f.nuw[1][Q];
-    }
-    if (y - 1 >= 0) {
-        const int Q = P - p.X;
-        if (fluid(f, Q)) vis[S_] = nu + (T)0.5f * (nutP + f.nut[Q]);
-        else if (solid(f, Q)) vis[S_] = nu + f.nuw[1][P];
-    } else if (p.velBnd[UWS_Y0] == UWS_VEL_FIXED_VALUE) {
-        vis[S_] = nu + f.nuw[1][P];
-    }
-    const int sz = p.X * p.Y;
-    if (z + 1 < p.Z) {
-        const int Q = P + sz;
-        if (fluid(f, Q)) vis[U_] = nu + (T)0.5f * (nutP + f.nut[Q]);
-        else if (solid(f, Q)) vis[U_] = nu + f.nuw[2][Q];
-    }
-    if (z - 1 >= 0) {
-        const int Q = P - sz;
-        if (fluid(f, Q)) vis[D_] = nu + (T)0.5f * (nutP + f.nut[Q]);
-        else if (solid(f, Q)) vis[D_] = nu + f.nuw[2][P];
-    }
+    // the ground below the domain is a wall when Y0 is a fixed-value face
+    if (!nb.in[S_] && p.velBnd[UWS_Y0] == UWS_VEL_FIXED_VALUE) vis[S_] = nu + wall[S_];
 }
 
 // ---------------------------------------------------------------------------
@@ -256,8 +296,11 @@ __device__ void body_wall_eddy(const ParamsT<T>& p, const FieldsT<T>& f, const C
         f.nuw[0][P] = fx; f.nuw[1][P] = fy; f.nuw[2][P] = fz;
     }
     if ((r.flags & K_EDDY) && isF) {
+        int Q[6];
+        bool in[6], fl[6];
+        nbr6(p, f, c, Q, in, fl);
         T g[3][3];
-        cell_grad(p, f, un, x, y, z, g);
+        cell_grad(p, f, un, c, Q, in, fl, g);
         // kept for the LES term, which needs it at this cell and its neighbours
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -286,45 +329,53 @@ __device__ __forceinline__ void load_grad(const FieldsT<T>& f, int P, T g[3][3])
 // cell gradients of this cell and its fluid neighbours are those the
 // eddy-viscosity kernel computed from the same u^n (cell_grad), read back.
 template <class T>
-__device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], int x, int y,
-                                          int z, int P) {
+__device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], const Cell& c,
+                                          const Nbr6& nb) {
+    const int P = c.P;
     const V3<T> uP = ld3(un, P);
     const T hdx = (T)0.5f * p.dx;
     T g[3][3];
     load_grad(f, P, g);
-    T fg[6][3];
+    // row cc of each neighbour's gradient (cc: the face's axis), loaded unconditionally
+    T gq[6][3];
+    T ubn[6];
 #pragma unroll
-    for (int d = 0; d < 6; ++d) { fg[d][0] = 0; fg[d][1] = 0; fg[d][2] = 0; }
+    for (int d = 0; d < 6; ++d) {
+        const int cc = d >> 1;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gq[d][j] = f.grad[3 * cc + j][nb.Q[d]];
+        const int bf = bface_of(d);
+        const int k = bf < 2 ? c.z * p.Y + c.y : (bf < 4 ? c.z * p.X + c.x : c.y * p.X + c.x);
+        ubn[d] = f.bnd[bf][cc][k];
+    }
+    T fg[6][3];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         const int cc = d >> 1;
         const bool plus = (d & 1) == 0;
-        const int nx = x + c_off[d][0], ny = y + c_off[d][1], nz = z + c_off[d][2];
         const T uPc = comp(uP, cc);
-        if (inreg(p, nx, ny, nz)) {
-            const int Q = lin(p, nx, ny, nz);
-            if (fluid(f, Q)) {
-                T gn[3][3];
-                load_grad(f, Q, gn);
+        // fluid neighbour: the average of the two cell gradients
+        T ff[3];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) fg[d][j] = (T)0.5f * (g[cc][j] + gn[cc][j]);
-            } else {
-                fg[d][cc] = plus ? -uPc / hdx : uPc / hdx;
-            }
-        } else {
-            const int t = p.velBnd[bface_of(d)];
-            if (t == UWS_VEL_FIXED_VALUE) {
-                const T ubc = comp(bnd_at(p, f, d, x, y, z), cc);
-                fg[d][cc] = plus ? (ubc - uPc) / hdx : (uPc - ubc) / hdx;
-            } else if (t == UWS_VEL_ZERO_GRAD || t == UWS_VEL_SYMMETRY) {
+        for (int j = 0; j < 3; ++j) ff[j] = (T)0.5f * (g[cc][j] + gq[d][j]);
+        // solid neighbour: only the normal derivative, towards u = 0 at the wall
+        T fs[3] = {0, 0, 0};
+        fs[cc] = plus ? -uPc / hdx : uPc / hdx;
+        // domain boundary, by BC type
+        T fb[3] = {0, 0, 0};
+        const int t = p.velBnd[bface_of(d)];
+        if (t == UWS_VEL_FIXED_VALUE) {
+            fb[cc] = plus ? (ubn[d] - uPc) / hdx : (uPc - ubn[d]) / hdx;
+        } else if (t == UWS_VEL_ZERO_GRAD || t == UWS_VEL_SYMMETRY) {
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    if (j != cc) fg[d][j] = g[cc][j];
-            }
+            for (int j = 0; j < 3; ++j)
+                if (j != cc) fb[j] = g[cc][j];
         }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) fg[d][j] = nb.in[d] ? (nb.fl[d] ? ff[j] : fs[j]) : fb[j];
     }
     T vis[6];
-    face_vis(p, f, x, y, z, P, vis);
+    face_vis(p, f, nb, P, vis);
     T rr[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -337,7 +388,7 @@ __device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>&
 template <class T>
 __device__ void body_les_term(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok || !fluid(f, c.P)) return;
-    const V3<T> v = les_term(p, f, f.vel[r.state], c.x, c.y, c.z, c.P);
+    const V3<T> v = les_term(p, f, f.vel[r.state], c, nbrs_of(p, f, c));
     f.b[0][c.P] = v.x; f.b[1][c.P] = v.y; f.b[2][c.P] = v.z;
 }
 
@@ -353,26 +404,32 @@ __device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, co
     T* const* un = f.vel[r.state];
     const int x = c.x, y = c.y, z = c.z, P = c.P;
     const bool isF = fluid(f, P);
+    const Nbr6 nb = nbrs_of(p, f, c);
     const V3<T> uP = ld3(un, P);
+    // neighbour values loaded unconditionally: the normal velocity component,
+    // p, and the boundary texture on each face
+    T uNn[6], pQ[6];
+    V3<T> ub[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        uNn[d] = un[d >> 1][nb.Q[d]];
+        pQ[d] = f.p[nb.Q[d]];
+        ub[d] = bnd_at(p, f, d, x, y, z);
+    }
     T vis[6];
-    if (p.useLES && isF) face_vis(p, f, x, y, z, P, vis);
+    if (p.useLES && isF) face_vis(p, f, nb, P, vis);
     else {
 #pragma unroll
         for (int d = 0; d < 6; ++d) vis[d] = p.nu;
     }
     {
-        const int nbr[3] = {x + 1 < p.X ? P + 1 : -1, y + 1 < p.Y ? P + p.X : -1, z + 1 < p.Z ? P + p.X * p.Y : -1};
         const int dirs[3] = {E_, N_, U_};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            T Sv = 0, Gv = 0;
-            const int Q = nbr[a];
-            if (isF && Q >= 0 && fluid(f, Q)) {
-                Sv = comp(uP, a) + un[a][Q];
-                Gv = vis[dirs[a]] * p.ds / p.dx;
-            }
-            f.S[a][P] = Sv;
-            f.G[a][P] = Gv;
+            const int d = dirs[a];
+            const bool face = isF && nb.fl[d];
+            f.S[a][P] = face ? comp(uP, a) + uNn[d] : (T)0;
+            f.G[a][P] = face ? vis[d] * p.ds / p.dx : (T)0;
         }
     }
     if (!isF) {
@@ -387,39 +444,33 @@ __device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, co
     for (int d = 0; d < 6; ++d) {
         const int cc = d >> 1;
         const bool plus = (d & 1) == 0;
-        const int nx = x + c_off[d][0], ny = y + c_off[d][1], nz = z + c_off[d][2];
         const T uPn = comp(uP, cc);
-        if (inreg(p, nx, ny, nz)) {
-            const int Q = lin(p, nx, ny, nz);
-            if (fluid(f, Q)) {
-                const T t = ds * (T)0.25f * (uPn + un[cc][Q]);
-                if (plus) { Dx += t; Dy += t; Dz += t; } else { Dx -= t; Dy -= t; Dz -= t; }
-            }
-        } else {
-            const int bf = bface_of(d);
-            if (p.velBnd[bf] == UWS_VEL_FIXED_VALUE) {
-                const V3<T> ub = bnd_at(p, f, d, x, y, z);
-                const T ubn = comp(ub, cc);
-                if (plus) { bU -= ds * ub.x * ubn; bV -= ds * ub.y * ubn; bW -= ds * ub.z * ubn; }
-                else { bU += ds * ub.x * ubn; bV += ds * ub.y * ubn; bW += ds * ub.z * ubn; }
-            } else if (p.velBnd[bf] == UWS_VEL_ZERO_GRAD) {
-                const T t = ds * uPn;
-                if (plus) { Dx += t; Dy += t; Dz += t; } else { Dx -= t; Dy -= t; Dz -= t; }
-            }
-        }
+        const int bf = bface_of(d);
+        const int t = p.velBnd[bf];
+        // fluid neighbour
+        const T ti = ds * (T)0.25f * (uPn + uNn[d]);
+        const bool addD_i = nb.fl[d];
+        // domain boundary: fixed value feeds b, zero gradient D
+        const T tz = ds * uPn;
+        const bool addD_b = !nb.in[d] && t == UWS_VEL_ZERO_GRAD;
+        const bool addb = !nb.in[d] && t == UWS_VEL_FIXED_VALUE;
+        const T tt = addD_i ? ti : tz;
+        const bool addD = addD_i || addD_b;
+        const T Dx1 = plus ? Dx + tt : Dx - tt, Dy1 = plus ? Dy + tt : Dy - tt, Dz1 = plus ? Dz + tt : Dz - tt;
+        Dx = addD ? Dx1 : Dx; Dy = addD ? Dy1 : Dy; Dz = addD ? Dz1 : Dz;
+        const T ubn = comp(ub[d], cc);
+        const T bU1 = plus ? bU - ds * ub[d].x * ubn : bU + ds * ub[d].x * ubn;
+        const T bV1 = plus ? bV - ds * ub[d].y * ubn : bV + ds * ub[d].y * ubn;
+        const T bW1 = plus ? bW - ds * ub[d].z * ubn : bW + ds * ub[d].z * ubn;
+        bU = addb ? bU1 : bU; bV = addb ? bV1 : bV; bW = addb ? bW1 : bW;
     }
     // pressure face values (:1484-1591)
     const T pP = f.p[P];
     T pf[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-        const int nx = x + c_off[d][0], ny = y + c_off[d][1], nz = z + c_off[d][2];
-        if (inreg(p, nx, ny, nz)) {
-            const int Q = lin(p, nx, ny, nz);
-            pf[d] = fluid(f, Q) ? (T)0.5f * (pP + f.p[Q]) : pP;
-        } else {
-            pf[d] = p.presBnd[bface_of(d)] == UWS_PRES_FIXED_VALUE ? (T)0 : pP;
-        }
+        const T pb = p.presBnd[bface_of(d)] == UWS_PRES_FIXED_VALUE ? (T)0 : pP;
+        pf[d] = nb.in[d] ? (nb.fl[d] ? (T)0.5f * (pP + pQ[d]) : pP) : pb;
     }
     bU += -((T)1.0f / p.den) * ds * (pf[E_] - pf[W_]);
     bV += -((T)1.0f / p.den) * ds * (pf[N_] - pf[S_]);
@@ -427,33 +478,30 @@ __device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, co
     // diffusion (:1593-1836)
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-        const int nx = x + c_off[d][0], ny = y + c_off[d][1], nz = z + c_off[d][2];
-        if (inreg(p, nx, ny, nz)) {
-            const int Q = lin(p, nx, ny, nz);
-            const T t = fluid(f, Q) ? vis[d] * ds / p.dx : (T)2.0f * vis[d] * ds / p.dx;
-            Dx += t; Dy += t; Dz += t;
-        } else {
-            const int bf = bface_of(d);
-            if (p.velBnd[bf] == UWS_VEL_FIXED_VALUE) {
-                const V3<T> ub = bnd_at(p, f, d, x, y, z);
-                bU += (T)2.0f * vis[d] * ds / p.dx * ub.x;
-                bV += (T)2.0f * vis[d] * ds / p.dx * ub.y;
-                bW += (T)2.0f * vis[d] * ds / p.dx * ub.z;
-                const T t = (T)2.0f * vis[d] * ds / p.dx;
-                Dx += t; Dy += t; Dz += t;
-            } else if (p.velBnd[bf] == UWS_VEL_SYMMETRY) {
-                const T t = (T)2.0f * vis[d] * ds / p.dx;
-                const int cc = d >> 1;
-                if (cc == 0) Dx += t; else if (cc == 1) Dy += t; else Dz += t;
-            }
-        }
+        const int cc = d >> 1;
+        const int t = p.velBnd[bface_of(d)];
+        const T tin = nb.fl[d] ? vis[d] * ds / p.dx : (T)2.0f * vis[d] * ds / p.dx;
+        const T tw = (T)2.0f * vis[d] * ds / p.dx;
+        // in the grid: all three components; fixed value: all three (and b); symmetry: the normal one
+        const bool all3 = nb.in[d] || t == UWS_VEL_FIXED_VALUE;
+        const bool norm = !nb.in[d] && t == UWS_VEL_SYMMETRY;
+        const T tt = nb.in[d] ? tin : tw;
+        const bool fixedb = !nb.in[d] && t == UWS_VEL_FIXED_VALUE;
+        const T bU1 = bU + (T)2.0f * vis[d] * ds / p.dx * ub[d].x;
+        const T bV1 = bV + (T)2.0f * vis[d] * ds / p.dx * ub[d].y;
+        const T bW1 = bW + (T)2.0f * vis[d] * ds / p.dx * ub[d].z;
+        bU = fixedb ? bU1 : bU; bV = fixedb ? bV1 : bV; bW = fixedb ? bW1 : bW;
+        const bool ax = all3 || (norm && cc == 0), ay = all3 || (norm && cc == 1), az = all3 || (norm && cc == 2);
+        Dx = ax ? Dx + tt : Dx;
+        Dy = ay ? Dy + tt : Dy;
+        Dz = az ? Dz + tt : Dz;
     }
     // time derivative, force, LES (:1838-1854)
     bU += p.dv / p.dt * uP.x; bV += p.dv / p.dt * uP.y; bW += p.dv / p.dt * uP.z;
     bU += p.dv * p.fext[0]; bV += p.dv * p.fext[1]; bW += p.dv * p.fext[2];
     if (p.useLES) {
         V3<T> bl;
-        if (r.flags & K_FUSE_LES) bl = les_term(p, f, un, x, y, z, P);
+        if (r.flags & K_FUSE_LES) bl = les_term(p, f, un, c, nb);
         else bl = mk<T>(f.b[0][P], f.b[1][P], f.b[2][P]);
         bU += bl.x; bV += bl.y; bW += bl.z;
     }
@@ -527,19 +575,50 @@ __device__ void body_pres_precompute(const ParamsT<T>& p, const FieldsT<T>& f, c
     const int x = c.x, y = c.y, z = c.z, P = c.P;
     const bool isF = fluid(f, P);
     const T dvden = p.dv / p.den;
+    // every neighbour value is loaded unconditionally (indices fall back to the
+    // cell itself, or to the first neighbour, outside the grid); the face terms
+    // then pick their branch by selects, each branch computed as written
+    int Q[6];
+    bool in[6], fl[6];
+    nbr6(p, f, c, Q, in, fl);
+    const int st[6] = {1, -1, p.X, -p.X, p.X * p.Y, -p.X * p.Y};
+    const int crd[6] = {x, x, y, y, z, z};
+    const int lim[6] = {p.X, p.X, p.Y, p.Y, p.Z, p.Z};
+    int F[6];
+    bool farF[6];
+    uint8_t ffl[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int cf = crd[d] + ((d & 1) ? -2 : 2);
+        const bool in2 = cf >= 0 && cf < lim[d];
+        F[d] = in2 ? P + 2 * st[d] : Q[d];
+        ffl[d] = f.flag[F[d]];
+        farF[d] = in2 && ffl[d] == UWS_CELL_FLUID;
+    }
+    const T invP[3] = {f.invD[0][P], f.invD[1][P], f.invD[2][P]};
+    const T uP[3] = {u[0][P], u[1][P], u[2][P]};
+    const T pP = f.p[P];
+    T invQ[6], uQ[6], pQ[6], pF[6], ubc[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int cc = d >> 1;
+        invQ[d] = f.invD[cc][Q[d]];
+        uQ[d] = u[cc][Q[d]];
+        pQ[d] = f.p[Q[d]];
+        pF[d] = f.p[F[d]];
+        const int bf = bface_of(d);
+        const int k = bf < 2 ? z * p.Y + y : (bf < 4 ? z * p.X + x : y * p.X + x);
+        ubc[d] = f.bnd[bf][cc][k];
+    }
     {
-        const int nbr[3] = {x + 1 < p.X ? P + 1 : -1, y + 1 < p.Y ? P + p.X : -1, z + 1 < p.Z ? P + p.X * p.Y : -1};
+        const int dirs[3] = {E_, N_, U_};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            T cv = 0, ce = 0;
-            const int Q = nbr[a];
-            if (isF && Q >= 0 && fluid(f, Q)) {
-                const T DInv = (T)0.5f * (f.invD[a][P] + f.invD[a][Q]);
-                cv = dvden * DInv / p.dx;
-                ce = dvden * (DInv / p.dx);
-            }
-            f.cp[a][P] = cv;
-            if (a == 0) f.cpisoE[P] = ce;
+            const int d = dirs[a];
+            const bool face = isF && fl[d];
+            const T DInv = (T)0.5f * (invP[a] + invQ[d]);
+            f.cp[a][P] = face ? dvden * DInv / p.dx : (T)0;
+            if (a == 0) f.cpisoE[P] = face ? dvden * (DInv / p.dx) : (T)0;
         }
     }
     if (!isF) {
@@ -553,66 +632,38 @@ __device__ void body_pres_precompute(const ParamsT<T>& p, const FieldsT<T>& f, c
     for (int d = 0; d < 6; ++d) {
         const int cc = d >> 1;
         const bool plus = (d & 1) == 0;
-        const int ox = c_off[d][0], oy = c_off[d][1], oz = c_off[d][2];
-        const int nx = x + ox, ny = y + oy, nz = z + oz;
-        const T uPc = u[cc][P];
-        if (inreg(p, nx, ny, nz)) {
-            const int Q = lin(p, nx, ny, nz);
-            if (!fluid(f, Q)) continue;
-            const T DInv = (T)0.5f * (f.invD[cc][P] + f.invD[cc][Q]);
-            Dp -= dvden * DInv / p.dx;
-            const T half = (T)0.5f * (uPc + u[cc][Q]);
-            if (plus) bp += half; else bp -= half;
-            const int fx = x + 2 * ox, fy = y + 2 * oy, fz = z + 2 * oz;
-            const int bx = x - ox, by = y - oy, bz = z - oz;
-            const bool farF = inreg(p, fx, fy, fz) && fluid(f, lin(p, fx, fy, fz));
-            const bool backF = inreg(p, bx, by, bz) && fluid(f, lin(p, bx, by, bz));
-            const T pP = f.p[P], pQ = f.p[Q];
-            T g, ga;
-            if (plus) {
-                g = (pQ - pP) / p.dx;
-                if (farF && backF) {
-                    const T pF = f.p[lin(p, fx, fy, fz)], pB = f.p[lin(p, bx, by, bz)];
-                    ga = (T)0.5f * ((pF - pP) / two_dx + (pQ - pB) / two_dx);
-                } else if (farF) {
-                    const T pF = f.p[lin(p, fx, fy, fz)];
-                    ga = (T)0.5f * ((pF - pP) / two_dx + (pQ - pP) / two_dx);
-                } else if (backF) {
-                    const T pB = f.p[lin(p, bx, by, bz)];
-                    ga = (T)0.5f * ((pQ - pP) / two_dx + (pQ - pB) / two_dx);
-                } else {
-                    ga = g;
-                }
-            } else {
-                g = (pP - pQ) / p.dx;
-                if (farF && backF) {
-                    const T pF = f.p[lin(p, fx, fy, fz)], pB = f.p[lin(p, bx, by, bz)];
-                    ga = (T)0.5f * ((pB - pQ) / two_dx + (pP - pF) / two_dx);
-                } else if (farF) {
-                    const T pF = f.p[lin(p, fx, fy, fz)];
-                    ga = (T)0.5f * ((pP - pQ) / two_dx + (pP - pF) / two_dx);
-                } else if (backF) {
-                    const T pB = f.p[lin(p, bx, by, bz)];
-                    ga = (T)0.5f * ((pB - pQ) / two_dx + (pP - pQ) / two_dx);
-                } else {
-                    ga = g;
-                }
-            }
-            const T vct = -(p.dv * DInv) / p.den * (g - ga);
-            if (plus) bp += vct; else bp -= vct;
+        const T uPc = uP[cc];
+        // interior face with a fluid neighbour
+        const T DInv = (T)0.5f * (invP[cc] + invQ[d]);
+        const T Dp_i = Dp - dvden * DInv / p.dx;
+        const T half = (T)0.5f * (uPc + uQ[d]);
+        T bp_i = plus ? bp + half : bp - half;
+        const bool fF = farF[d], bF = fl[d ^ 1];
+        const T pq = pQ[d], pf = pF[d], pb = pQ[d ^ 1];
+        T g, ga;
+        if (plus) {
+            g = (pq - pP) / p.dx;
+            const T a_fb = (T)0.5f * ((pf - pP) / two_dx + (pq - pb) / two_dx);
+            const T a_f = (T)0.5f * ((pf - pP) / two_dx + (pq - pP) / two_dx);
+            const T a_b = (T)0.5f * ((pq - pP) / two_dx + (pq - pb) / two_dx);
+            ga = fF ? (bF ? a_fb : a_f) : (bF ? a_b : g);
         } else {
-            const int bf = bface_of(d);
-            if (p.velBnd[bf] == UWS_VEL_FIXED_VALUE) {
-                const T ubc = comp(bnd_at(p, f, d, x, y, z), cc);
-                if (plus) bp += ubc; else bp -= ubc;
-            } else if (p.velBnd[bf] == UWS_VEL_ZERO_GRAD) {
-                if (plus) bp += uPc; else bp -= uPc;
-            }
-            if (p.presBnd[bf] == UWS_PRES_FIXED_VALUE) {
-                const T DInv = (T)1.0f / f.D[cc][P];
-                Dp -= (T)2.0f * dvden * DInv / p.dx;
-            }
+            g = (pP - pq) / p.dx;
+            const T a_fb = (T)0.5f * ((pb - pq) / two_dx + (pP - pf) / two_dx);
+            const T a_f = (T)0.5f * ((pP - pq) / two_dx + (pP - pf) / two_dx);
+            const T a_b = (T)0.5f * ((pb - pq) / two_dx + (pP - pq) / two_dx);
+            ga = fF ? (bF ? a_fb : a_f) : (bF ? a_b : g);
         }
+        const T vct = -(p.dv * DInv) / p.den * (g - ga);
+        bp_i = plus ? bp_i + vct : bp_i - vct;
+        // domain boundary face
+        const int bf = bface_of(d);
+        T bp_b = bp;
+        if (p.velBnd[bf] == UWS_VEL_FIXED_VALUE) bp_b = plus ? bp + ubc[d] : bp - ubc[d];
+        else if (p.velBnd[bf] == UWS_VEL_ZERO_GRAD) bp_b = plus ? bp + uPc : bp - uPc;
+        const T Dp_b = p.presBnd[bf] == UWS_PRES_FIXED_VALUE ? Dp - (T)2.0f * dvden * invP[cc] / p.dx : Dp;
+        Dp = in[d] ? (fl[d] ? Dp_i : Dp) : Dp_b;
+        bp = in[d] ? (fl[d] ? bp_i : bp) : bp_b;
     }
     f.Dp[P] = Dp;
     f.invDp[P] = (T)1.0f / Dp;
@@ -702,30 +753,38 @@ __device__ void body_pres_normalise(const ParamsT<T>& p, const FieldsT<T>& f, co
 template <class T>
 __device__ void body_vel_correct(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
-    const int x = c.x, y = c.y, z = c.z, P = c.P;
+    const int P = c.P;
     if (!fluid(f, P)) return;
     const T* __restrict__ pc = f.pc[r.pin];
+    int Qs[6];
+    bool in[6], fl[6];
+    nbr6(p, f, c, Qs, in, fl);
     const T pcP = pc[P];
+    T pQ[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) pQ[d] = pc[Qs[d]];
+    const T iv[3] = {(T)1.0f / f.D[0][P], (T)1.0f / f.D[1][P], (T)1.0f / f.D[2][P]};
     T pf[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-        const int nx = x + c_off[d][0], ny = y + c_off[d][1], nz = z + c_off[d][2];
-        if (inreg(p, nx, ny, nz)) {
-            const int Q = lin(p, nx, ny, nz);
-            pf[d] = fluid(f, Q) ? (T)0.5f * (pcP + pc[Q]) : pcP;
-        } else {
-            pf[d] = p.presBnd[bface_of(d)] == UWS_PRES_FIXED_VALUE ? (T)0 : pcP;
-        }
+        const T bnd = p.presBnd[bface_of(d)] == UWS_PRES_FIXED_VALUE ? (T)0 : pcP;
+        pf[d] = in[d] ? (fl[d] ? (T)0.5f * (pcP + pQ[d]) : pcP) : bnd;
     }
     const T dvden = p.dv / p.den;
-    const T cx = -dvden * ((T)1.0f / f.D[0][P]) * (pf[E_] - pf[W_]) / p.dx;
-    const T cy = -dvden * ((T)1.0f / f.D[1][P]) * (pf[N_] - pf[S_]) / p.dx;
-    const T cz = -dvden * ((T)1.0f / f.D[2][P]) * (pf[U_] - pf[D_]) / p.dx;
+    const T cx = -dvden * iv[0] * (pf[E_] - pf[W_]) / p.dx;
+    const T cy = -dvden * iv[1] * (pf[N_] - pf[S_]) / p.dx;
+    const T cz = -dvden * iv[2] * (pf[U_] - pf[D_]) / p.dx;
     f.ucorr[0][P] = cx; f.ucorr[1][P] = cy; f.ucorr[2][P] = cz;
     if (r.flags & K_NO_INPLACE) return;
     T* const* u = f.vel[r.vout];
     u[0][P] += cx; u[1][P] += cy; u[2][P] += cz;
 }
+
+// The one-shot kernels below read their six neighbours unconditionally:
+// indices of missing or non-fluid neighbours fall back to the cell itself,
+// and the terms of those faces are dropped by selecting the unchanged
+// accumulator (bit-identical to skipping them).  Loads guarded by per-face
+// branches would each wait for the previous ones.
 
 // CSPisoCalAodUCorrect (PisoColl.compute:70-178): laminar nu, no BC terms
 template <class T>
@@ -739,30 +798,33 @@ __device__ void body_piso_aodu(const ParamsT<T>& p, const FieldsT<T>& f, const C
     const T q = p.ds * (T)0.25f;
     int Qs[6];
     bool ok[6];
+    fluid_nbrs(p, f, c, Qs, ok);
+    T uN[6], uc[6][3];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-        const int nx = c.x + c_off[d][0], ny = c.y + c_off[d][1], nz = c.z + c_off[d][2];
-        ok[d] = inreg(p, nx, ny, nz) && fluid(f, lin(p, nx, ny, nz));
-        Qs[d] = ok[d] ? lin(p, nx, ny, nz) : P;
+        uN[d] = un[d >> 1][Qs[d]];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) uc[d][a] = f.ucorr[a][Qs[d]];
     }
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-        if (!ok[d]) continue;
         const int cc = d >> 1;
-        const int Q = Qs[d];
-        const T sum = comp(uP, cc) + un[cc][Q];
+        const T sum = comp(uP, cc) + uN[d];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const T t = q * f.ucorr[a][Q] * sum;
-            if ((d & 1) == 0) A[a] += t; else A[a] -= t;
+            const T t = q * uc[d][a] * sum;
+            const T v = (d & 1) == 0 ? A[a] + t : A[a] - t;
+            A[a] = ok[d] ? v : A[a];
         }
     }
     const T g = p.nu * p.ds / p.dx;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-        if (!ok[d]) continue;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) Df[a] -= g * f.ucorr[a][Qs[d]];
+        for (int a = 0; a < 3; ++a) {
+            const T v = Df[a] - g * uc[d][a];
+            Df[a] = ok[d] ? v : Df[a];
+        }
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) f.aodu[a][P] = A[a] + Df[a];
@@ -779,20 +841,31 @@ __device__ void body_piso_precompute(const ParamsT<T>& p, const FieldsT<T>& f, c
         f.bpiso[P] = 0;
         return;
     }
+    int Qs[6];
+    bool ok[6];
+    fluid_nbrs(p, f, c, Qs, ok);
+    const T invP[3] = {f.invD[0][P], f.invD[1][P], f.invD[2][P]};
+    const T aoP[3] = {f.aodu[0][P], f.aodu[1][P], f.aodu[2][P]};
+    T invQ[6], aoQ[6], cf[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int cc = d >> 1;
+        invQ[d] = f.invD[cc][Qs[d]];
+        aoQ[d] = f.aodu[cc][Qs[d]];
+        // this face's coefficient as stored on the cell owning it
+        const int owner = (d & 1) == 0 ? P : Qs[d];
+        cf[d] = cc == 0 ? f.cpisoE[owner] : f.cp[cc][owner];
+    }
     T presCoeff = 0, velTerm = 0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         const int cc = d >> 1;
-        const int nx = c.x + c_off[d][0], ny = c.y + c_off[d][1], nz = c.z + c_off[d][2];
-        if (!(inreg(p, nx, ny, nz) && fluid(f, lin(p, nx, ny, nz)))) continue;
-        const int Q = lin(p, nx, ny, nz);
-        const T AdInv = (T)0.5f * (f.invD[cc][P] + f.invD[cc][Q]);
-        const T AodUCorrect = (T)0.5f * (f.aodu[cc][P] + f.aodu[cc][Q]);
-        // this face's coefficient as stored on the cell owning it
-        const int owner = (d & 1) == 0 ? P : Q;
-        const T cf = cc == 0 ? f.cpisoE[owner] : f.cp[cc][owner];
-        presCoeff -= cf;
-        if ((d & 1) == 0) velTerm -= AdInv * AodUCorrect; else velTerm += AdInv * AodUCorrect;
+        const T AdInv = (T)0.5f * (invP[cc] + invQ[d]);
+        const T AodUCorrect = (T)0.5f * (aoP[cc] + aoQ[d]);
+        const T pc2 = presCoeff - cf[d];
+        const T vt2 = (d & 1) == 0 ? velTerm - AdInv * AodUCorrect : velTerm + AdInv * AodUCorrect;
+        presCoeff = ok[d] ? pc2 : presCoeff;
+        velTerm = ok[d] ? vt2 : velTerm;
     }
     f.invDpiso[P] = (T)1.0f / presCoeff;
     f.bpiso[P] = velTerm;
@@ -805,21 +878,25 @@ __device__ void body_piso_correct(const ParamsT<T>& p, const FieldsT<T>& f, cons
     const int P = c.P;
     if (!fluid(f, P)) return;
     const T* __restrict__ pc = f.pc[r.pin];
+    int Qs[6];
+    bool ok[6];
+    fluid_nbrs(p, f, c, Qs, ok);
     const T pcP = pc[P];
+    T pQ[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) pQ[d] = pc[Qs[d]];
+    const T ao[3] = {f.aodu[0][P], f.aodu[1][P], f.aodu[2][P]};
+    const T iv[3] = {f.invD[0][P], f.invD[1][P], f.invD[2][P]};
     const bool inplace = (r.flags & K_NO_INPLACE) == 0;
     if (inplace) f.p[P] += pcP;
     T pf[6];
 #pragma unroll
-    for (int d = 0; d < 6; ++d) {
-        const int nx = c.x + c_off[d][0], ny = c.y + c_off[d][1], nz = c.z + c_off[d][2];
-        pf[d] = (inreg(p, nx, ny, nz) && fluid(f, lin(p, nx, ny, nz))) ? (T)0.5f * (pcP + pc[lin(p, nx, ny, nz)])
-                                                                       : pcP;
-    }
+    for (int d = 0; d < 6; ++d) pf[d] = ok[d] ? (T)0.5f * (pcP + pQ[d]) : pcP;
     const T gx = (pf[E_] - pf[W_]) / p.dx, gy = (pf[N_] - pf[S_]) / p.dx, gz = (pf[U_] - pf[D_]) / p.dx;
     const T dvden = p.dv / p.den;
-    const T cx = -f.invD[0][P] * (f.aodu[0][P] + dvden * gx);
-    const T cy = -f.invD[1][P] * (f.aodu[1][P] + dvden * gy);
-    const T cz = -f.invD[2][P] * (f.aodu[2][P] + dvden * gz);
+    const T cx = -iv[0] * (ao[0] + dvden * gx);
+    const T cy = -iv[1] * (ao[1] + dvden * gy);
+    const T cz = -iv[2] * (ao[2] + dvden * gz);
     f.ucorr[0][P] = cx; f.ucorr[1][P] = cy; f.ucorr[2][P] = cz;
     if (!inplace) return;
     T* const* u = f.vel[r.vout];

@@ -218,7 +218,7 @@ __device__ void body_nu_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f,
     bU += g.dv * p.fext[0]; bV += g.dv * p.fext[1]; bW += g.dv * p.fext[2];
     if (p.useLES) {
         V3<T> bl;
-        if (r.flags & K_FUSE_LES) bl = les_term(p, f, un, x, y, z, P);
+        if (r.flags & K_FUSE_LES) bl = les_term(p, f, un, c, nbrs_of(p, f, c));
         else bl = mk<T>(f.b[0][P], f.b[1][P], f.b[2][P]);
         bU += bl.x; bV += bl.y; bW += bl.z;
     }
