@@ -769,7 +769,7 @@ int uws_get_norms(uws_handle h, double* out, int32_t n) {
 }
 
 int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms) {
-    if (!avg_ms || reps <= 0 || kernel < UWS_KERNEL_VEL_SWEEP || kernel > UWS_KERNEL_PRES_SWEEP2)
+    if (!avg_ms || reps <= 0 || kernel < UWS_KERNEL_VEL_SWEEP || kernel > UWS_KERNEL_VEL_SWEEP2)
         return UWS_ERR_INVALID_ARG;
     return with_solver(h, [&](SolverBase& s) { return s.time_kernel(kernel, reps, avg_ms); });
 }

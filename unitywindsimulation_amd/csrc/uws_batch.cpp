@@ -173,6 +173,7 @@ struct uws_batch {
         int read_norm(int, double*) { return UWS_ERR_UNSUPPORTED; }
         double ncells() const { return 1.0; }
         bool fused() const { return use_fusion(b->fuse_mode, g->tab.total_ps2); }
+        bool fused_vel() const { return false; }
     };
 
     void sync_case_roles() {

@@ -133,10 +133,23 @@ int stage_vel_sweeps(Lr& l, const StepCfg& c, RoleState& rs) {
     const int s = rs.state, w0 = (s + 1) % 3, w1 = (s + 2) % 3;
     int in = s, last_in = s, result = -1;
     bool broke = false;
-    for (int k = 0; k < c.V; ++k) {
-        const int out = (k & 1) ? w1 : w0;
+    // pairs of sweeps run as one two-sweep launch where the launcher has it
+    // (the first sweep's result stays on chip: the pair writes w1 only); the
+    // residual-based exit and the last sweep's residual need single sweeps
+    const bool pairs = l.fused_vel() && !c.cal_residual;
+    for (int k = 0; k < c.V;) {
+        // a sweep writes the work buffer it does not read (w0 first, then alternating)
+        const int out = in == w0 ? w1 : w0;
         Roles r{};
         r.state = s; r.vin = in; r.vout = out;
+        if (pairs && k + 2 <= c.V && !(c.norms && k + 2 == c.V)) {
+            // a pair from u^n or w1 ends in w1 or w0 (never in the buffer it reads)
+            r.vout = in == w1 ? w0 : w1;
+            l(KN_VEL_SWEEP2, r);
+            in = r.vout;
+            k += 2;
+            continue;
+        }
         l(KN_VEL_SWEEP, r);
         if (c.cal_residual && k % c.vci == 0) {
             l(KN_RES_VEL, r);
@@ -153,6 +166,7 @@ int stage_vel_sweeps(Lr& l, const StepCfg& c, RoleState& rs) {
         }
         last_in = in;
         in = out;
+        ++k;
     }
     if (result < 0) result = in;
     if (c.norms && c.V >= 1 && !broke) {
@@ -337,7 +351,8 @@ int enqueue_step(Lr& l, const StepCfg& c, RoleState& rs) {
     return run_stage(l, c, rs, UWS_STAGE_FINISH);
 }
 
-// the role change of one enqueue_step without a residual-based early exit
+// the role change of one enqueue_step without a residual-based early exit,
+// for launchers whose momentum sweeps are all single (fused_vel() false)
 inline void advance_roles(const StepCfg& c, RoleState& rs) {
     const int w0 = (rs.state + 1) % 3, w1 = (rs.state + 2) % 3;
     rs.state = c.V == 0 ? w0 : (((c.V - 1) & 1) ? w1 : w0);

@@ -117,7 +117,8 @@ enum Kernel {
     KN_RES_VEL, KN_RES_PRES, KN_DIVERGENCE, KN_NONFINITE, KN_SUM_PARTIALS, KN_DEFECT_SCALE,
     KN_PRES_SWEEP2,  // two pressure Jacobi sweeps in one launch (single-case only)
     // stretched-grid variants, selected by launch() when ParamsT::nonuniform is set
-    KN_NU_VEL_PRECOMP, KN_NU_VEL_SWEEP, KN_NU_PRES_PRECOMP, KN_NU_VEL_CORRECT, KN_NU_DIVERGENCE
+    KN_NU_VEL_PRECOMP, KN_NU_VEL_SWEEP, KN_NU_PRES_PRECOMP, KN_NU_VEL_CORRECT, KN_NU_DIVERGENCE,
+    KN_VEL_SWEEP2  // two momentum Jacobi sweeps in one launch (single case, uniform grid, FP32)
 };
 
 // Single-case launch of kernel k on stream s.  `slot` selects the norms

@@ -1273,6 +1273,247 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 }
 
 
+// ---------------------------------------------------------------------------
+// Two momentum Jacobi sweeps in one launch: the tiling, z-march and x1 ring
+// of the two-sweep pressure kernel, for the three velocity components at
+// once (the face coefficients S, G are shared by them).  Both sweeps use the
+// expressions and operation order of body_vel_sweep.
+//
+// Registers are what limits this kernel (a 16-wave workgroup must fit in 128
+// VGPRs without spilling: a spill reload would make every iteration wait for
+// the loads of the next plane too), so the in-plane neighbours of the input
+// iterate come from LDS rather than from per-thread loads:
+//  - each thread loads its own column's iterate two planes ahead and stores
+//    it in a two-plane LDS ring one iteration ahead of its use; the first
+//    sweep at plane zz reads the E, W, N, S values of plane zz from the ring
+//    and the D value (plane zz - 1) from its own ring entry;
+//  - the tile's edge threads (lx 0 or 65, ly 0 or 14) also load the one
+//    neighbour outside the tile they need (corner threads' results are never
+//    used);
+//  - the coefficients (own E/N/U faces and the W, S faces of the neighbours)
+//    are loaded per thread one plane ahead, in three records that rotate
+//    through named variables (the loop is unrolled three times).
+// Differences from the pressure kernel:
+//  - the W, S and D coefficients are zeroed on the domain boundary by a bit
+//    mask (S can be negative: a product with 0 would give -0 where the
+//    one-sweep kernel has the +0 of its select);
+//  - the flag is read: the first sweep's value of a non-fluid cell is +0 (its
+//    neighbours' coefficients towards it are zero), and the second sweep
+//    writes fluid cells only, as the one-sweep kernel does.
+// The one-sweep kernel reads, at a non-fluid neighbour, whatever its input
+// buffer holds there (zero: solid cells are zeroed in every buffer and no
+// sweep writes them); the product with the zero coefficient makes any finite
+// value there a signed zero, so the two paths agree in every value.
+// ---------------------------------------------------------------------------
+template <class T>
+struct VS2Coef {
+    T SE, SN, SU, GE, GN, GU, SW, GW, SS, GS;
+    T inv[3], b[3];
+    T vO[3];  // edge threads: the input iterate at the neighbour outside the tile
+    int fl;
+};
+template <class T>
+struct VS2Args {
+    int X, Y, Z, zchunk;
+    T q;  // ds / 4
+    const T* S[3];
+    const T* G[3];
+    const T* invD[3];
+    const T* b[3];
+    const uint8_t* flag;
+    const T* vin[3];
+    T* out[3];
+};
+template <class T>
+__device__ __forceinline__ VS2Args<T> vs2_args(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r) {
+    VS2Args<T> a;
+    a.X = p.X; a.Y = p.Y; a.Z = p.Z; a.zchunk = p.zchunk;
+    a.q = p.ds * (T)0.25f;
+    for (int c = 0; c < 3; ++c) {
+        a.S[c] = f.S[c]; a.G[c] = f.G[c]; a.invD[c] = f.invD[c]; a.b[c] = f.b[c];
+        a.vin[c] = f.vel[r.vin][c];
+        a.out[c] = f.vel[r.vout][c];
+    }
+    a.flag = f.flag;
+    return a;
+}
+// v where keep holds, else +0 (a bit mask: the value stays used, so its load
+// is not moved under a branch)
+__device__ __forceinline__ float keep_or_zero(float v, bool keep) {
+    return __uint_as_float(__float_as_uint(v) & (keep ? 0xffffffffu : 0u));
+}
+__device__ __forceinline__ double keep_or_zero(double v, bool keep) {
+    return __longlong_as_double(__double_as_longlong(v) & (keep ? -1ll : 0ll));
+}
+// LDS of one workgroup: the x1 ring (3 planes) and the input ring (2 planes),
+// three components each
+constexpr int VS2_LDS_FLOATS = (3 + 2) * 3 * PS2_THREADS;
+
+template <class T>
+__device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* lds) {
+    const int tid = (int)threadIdx.x;
+    const int lx = tid % PS2_W, ly = tid / PS2_W;
+    const int ntx = (p.X + 63) / 64, nty = (p.Y + PS2_BY - 3) / (PS2_BY - 2);
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int x = bx * 64 - 1 + lx, y = by * (PS2_BY - 2) - 1 + ly;
+    const int z0 = bz * p.zchunk, z1 = min(z0 + p.zchunk, p.Z);
+    const bool cell = x >= 0 && x < p.X && y >= 0 && y < p.Y;
+    const bool inner = cell && lx >= 1 && lx <= 64 && ly >= 1 && ly <= PS2_BY - 2;
+    const int sy = p.X;
+    const bool hW = x > 0, hE = x + 1 < p.X, hS = y > 0, hN = y + 1 < p.Y;
+    const int xc = min(max(x, 0), p.X - 1), yc = min(max(y, 0), p.Y - 1);
+    const bool gE = xc + 1 < p.X, gW = xc > 0, gN = yc + 1 < p.Y, gS = yc > 0;
+    // tile edges: the neighbour outside the tile comes from the thread's own load
+    const bool eW = lx == 0, eE = lx == PS2_W - 1, eS = ly == 0, eN = ly == PS2_BY - 1;
+    const int oO = eW ? (gW ? -1 : 0) : eE ? (gE ? 1 : 0) : eS ? (gS ? -sy : 0) : eN ? (gN ? sy : 0) : 0;
+    // LDS indices of the in-tile neighbours (the thread itself on a tile edge)
+    const int iE = eE ? tid : tid + 1, iW = eW ? tid : tid - 1;
+    const int iN = eN ? tid : tid + PS2_W, iS = eS ? tid : tid - PS2_W;
+    T* const lx1 = lds;                      // [3 planes][3 components][PS2_THREADS]
+    T* const lvi = lds + 9 * PS2_THREADS;    // [2 planes][3 components][PS2_THREADS]
+    const T q = p.q;
+    const int col = yc * p.X + xc;
+    auto plane = [&](int zz) { return min(max(zz, 0), p.Z - 1) * (p.X * p.Y) + col; };
+    auto loadc = [&](VS2Coef<T>& r, int zz) {
+        const int P = plane(zz);
+        const int PW = max(P - 1, 0), PS = max(P - sy, 0);
+        r.SE = p.S[0][P]; r.SN = p.S[1][P]; r.SU = p.S[2][P];
+        r.GE = p.G[0][P]; r.GN = p.G[1][P]; r.GU = p.G[2][P];
+        r.SW = p.S[0][PW]; r.GW = p.G[0][PW];
+        r.SS = p.S[1][PS]; r.GS = p.G[1][PS];
+        r.fl = p.flag[P];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            r.inv[a] = p.invD[a][P];
+            r.b[a] = p.b[a][P];
+            r.vO[a] = p.vin[a][P + oO];
+        }
+    };
+    auto loadv = [&](T v[3], int zz) {
+        const int P = plane(zz);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = p.vin[a][P];
+    };
+    auto ring = [&](int zz) { return lvi + (zz & 1) * (3 * PS2_THREADS); };
+    T x1m1[3] = {0, 0, 0}, x1m2[3] = {0, 0, 0};
+    T pdS = 0, pdG = 0;  // U coefficients of the plane below the previous one
+    // one plane: loads of the coefficients of zz + 1 into nx and of the own
+    // iterate at zz + 2 into vn; first sweep at zz (vu: own iterate at zz + 1);
+    // vu into the input ring; second sweep at zz - 1 from pv and the x1 ring
+    auto step = [&](const VS2Coef<T>& pv, const VS2Coef<T>& cu, VS2Coef<T>& nx, const T vu[3], T vn[3], int zz) {
+        loadc(nx, zz + 1 <= z1 ? zz + 1 : z1);
+        loadv(vn, zz + 2 <= z1 + 1 ? zz + 2 : z1 + 1);
+        const T SW = keep_or_zero(cu.SW, hW), GW = keep_or_zero(cu.GW, hW);
+        const T SS = keep_or_zero(cu.SS, hS), GS = keep_or_zero(cu.GS, hS);
+        const T SD = keep_or_zero(pv.SU, zz > 0), GD = keep_or_zero(pv.GU, zz > 0);
+        const T* Lc = ring(zz);
+        const T* Ld = ring(zz - 1);
+        T x1[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const T* L = Lc + a * PS2_THREADS;
+            const T lE = L[iE], lW = L[iW], lN = L[iN], lS = L[iS];
+            const T vE = eE ? cu.vO[a] : lE, vW = eW ? cu.vO[a] : lW;
+            const T vN = eN ? cu.vO[a] : lN, vS = eS ? cu.vO[a] : lS;
+            const T vU = vu[a], vD = Ld[a * PS2_THREADS + tid];
+            T A = 0;
+            A -= q * vE * cu.SE;
+            A += q * vW * SW;
+            A -= q * vN * cu.SN;
+            A += q * vS * SS;
+            A -= q * vU * cu.SU;
+            A += q * vD * SD;
+            T Df = 0;
+            Df += cu.GE * vE;
+            Df += GW * vW;
+            Df += cu.GN * vN;
+            Df += GS * vS;
+            Df += cu.GU * vU;
+            Df += GD * vD;
+            const T v = cu.inv[a] * (A + Df + cu.b[a]);
+            x1[a] = cu.fl == UWS_CELL_FLUID ? v : (T)0;
+            lx1[((zz + 3) % 3) * (3 * PS2_THREADS) + a * PS2_THREADS + tid] = x1[a];
+        }
+        // the own iterate at zz + 1 for the next plane's first sweep (ring slot
+        // of zz - 1: read above by this thread only)
+        T* Ln = ring(zz + 1);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Ln[a * PS2_THREADS + tid] = vu[a];
+        __syncthreads();
+        const int zo = zz - 1;
+        if (inner && zo >= z0 && zo < z1 && pv.fl == UWS_CELL_FLUID) {
+            const int P = (zo * p.Y + y) * p.X + x;
+            const T SW2 = keep_or_zero(pv.SW, hW), GW2 = keep_or_zero(pv.GW, hW);
+            const T SS2 = keep_or_zero(pv.SS, hS), GS2 = keep_or_zero(pv.GS, hS);
+            const T SD2 = keep_or_zero(pdS, zo > 0), GD2 = keep_or_zero(pdG, zo > 0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const T* L = lx1 + ((zo + 3) % 3) * (3 * PS2_THREADS) + a * PS2_THREADS;
+                const T lE = L[tid + 1], lW = L[tid - 1], lN = L[tid + PS2_W], lS = L[tid - PS2_W];
+                const T vE = hE ? lE : x1m1[a], vW = hW ? lW : x1m1[a];
+                const T vN = hN ? lN : x1m1[a], vS = hS ? lS : x1m1[a];
+                const T vU = zo + 1 < p.Z ? x1[a] : x1m1[a], vD2 = zo > 0 ? x1m2[a] : x1m1[a];
+                T A = 0;
+                A -= q * vE * pv.SE;
+                A += q * vW * SW2;
+                A -= q * vN * pv.SN;
+                A += q * vS * SS2;
+                A -= q * vU * pv.SU;
+                A += q * vD2 * SD2;
+                T Df = 0;
+                Df += pv.GE * vE;
+                Df += GW2 * vW;
+                Df += pv.GN * vN;
+                Df += GS2 * vS;
+                Df += pv.GU * vU;
+                Df += GD2 * vD2;
+                p.out[a][P] = pv.inv[a] * (A + Df + pv.b[a]);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            x1m2[a] = x1m1[a];
+            x1m1[a] = x1[a];
+        }
+        pdS = pv.SU;
+        pdG = pv.GU;
+    };
+    // prologue: the input ring holds the own iterate at z0 - 2 and z0 - 1
+    // (the first plane's D and in-plane values), V1 that at z0 (its U value)
+    {
+        T v[3];
+        loadv(v, z0 - 2);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ring(z0 - 2)[a * PS2_THREADS + tid] = v[a];
+        loadv(v, z0 - 1);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ring(z0 - 1)[a * PS2_THREADS + tid] = v[a];
+    }
+    // records on entry of step(pv, cu, nx, ...): pv = plane zz - 1, cu = plane zz
+    VS2Coef<T> A, B, C;
+    T V1[3], V2[3], V3[3];
+    loadc(A, z0 - 2);
+    loadc(B, z0 - 1);
+    loadv(V1, z0);
+    __syncthreads();
+    for (int zz = z0 - 1;;) {
+        step(A, B, C, V1, V2, zz);
+        if (++zz > z1) break;
+        step(B, C, A, V2, V3, zz);
+        if (++zz > z1) break;
+        step(C, A, B, V3, V1, zz);
+        if (++zz > z1) break;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_vel_sweep2_f32(ParamsT<float> p, FieldsT<float> f, Roles r) {
+    __shared__ __align__(16) float lds_v[VS2_LDS_FLOATS];
+    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+    vel_sweep2_block<float>(vs2_args(p, f, r), xcd_tile((int)blockIdx.x, nb), lds_v);
+}
+
 // the case owning tile t: the last c with first_tile[c] <= t (binary search)
 template <class T>
 __device__ __forceinline__ int case_of(const BatchTable<T>& tab, int t) {
@@ -1430,6 +1671,16 @@ void go_batch_reduce(const BatchTable<T>& tab, int grid, const Roles& r, hipStre
 
 template <class T>
 void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s, int slot) {
+    if (k == KN_VEL_SWEEP2) {
+        // FP32 on the uniform grid only (the step driver asks for it there)
+        if constexpr (std::is_same_v<T, float>) {
+            if (!p.nonuniform) {
+                const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+                hipLaunchKernelGGL(k_vel_sweep2_f32, dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+            }
+        }
+        return;
+    }
     if (k == KN_PRES_SWEEP2) {
         const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
         if constexpr (std::is_same_v<T, float>) {

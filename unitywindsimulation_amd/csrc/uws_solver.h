@@ -27,6 +27,7 @@ struct Solver : SolverBase {
     ParamsT<T> p{};
     FieldsT<T> f{};
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // one per starting state buffer
+    RoleState graph_after[3] = {};                          // the roles each graph's step leaves
 
     double* mean_sum[3] = {nullptr, nullptr, nullptr};  // FP64 time sum of u (allocated on first use)
     int64_t mean_count = 0;
@@ -66,6 +67,8 @@ struct Solver : SolverBase {
         bool fused() const {
             return use_fusion(s->fuse_mode, pres_sweep2_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk));
         }
+        // the two-sweep momentum kernel: FP32 on the uniform grid
+        bool fused_vel() const { return std::is_same<T, float>::value && !s->p.nonuniform && fused(); }
     };
     int fuse_mode = fusion_mode();  // two-sweep pressure kernel: see fusion_mode()
 
@@ -445,6 +448,7 @@ struct Solver : SolverBase {
             HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             int rc = enqueue_step();
             hipError_t e = hipStreamEndCapture(stream, &g);
+            graph_after[s0] = rs;
             rs = saved;
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
             if (e != hipSuccess) return UWS_ERR_DEVICE;
@@ -453,7 +457,7 @@ struct Solver : SolverBase {
             if (e != hipSuccess) { graph[s0] = nullptr; return UWS_ERR_DEVICE; }
         }
         HIP_OK(hipGraphLaunch(graph[s0], stream));
-        advance_roles(sc, rs);
+        rs = graph_after[s0];
         return UWS_OK;
     }
     int step_n(int n) override {
@@ -473,9 +477,17 @@ struct Solver : SolverBase {
         HIP_OK(hipEventCreate(&e0));
         if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return UWS_ERR_DEVICE; }
         const int w0 = (rs.state + 1) % 3;
+        if (kernel == UWS_KERNEL_VEL_SWEEP2 && !(std::is_same<T, float>::value && !p.nonuniform)) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return UWS_ERR_UNSUPPORTED;
+        }
         auto go = [&](int i) {
             Roles r{};
-            if (kernel == UWS_KERNEL_VEL_SWEEP) {
+            if (kernel == UWS_KERNEL_VEL_SWEEP2) {
+                r.state = rs.state; r.vin = rs.state; r.vout = w0;
+                launch<T>(KN_VEL_SWEEP2, p, f, r, stream);
+            } else if (kernel == UWS_KERNEL_VEL_SWEEP) {
                 r.state = rs.state; r.vin = rs.state; r.vout = w0;
                 launch<T>(KN_VEL_SWEEP, p, f, r, stream);
             } else if (kernel == UWS_KERNEL_PRES_SWEEP2) {
