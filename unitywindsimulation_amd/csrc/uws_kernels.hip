@@ -1129,6 +1129,12 @@ inline __host__ __device__ int pres_sweep2_blocks(int X, int Y, int Z, int zchun
 // what one two-sweep workgroup reads: the grid, the coefficient set and the
 // p' buffers (a compact copy, so that an ensemble launch keeps no more
 // registers than a single-case one)
+// *(base + off) for a byte offset off (uniform base: the SGPR-base form of
+// the global load, one 32-bit offset register shared by arrays)
+template <class T>
+__device__ __forceinline__ T ld_at(const T* base, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
 template <class T>
 struct PS2Args {
     int X, Y, Z, zchunk;
@@ -1146,7 +1152,10 @@ __device__ __forceinline__ PS2Args<T> ps2_args(const ParamsT<T>& p, const Fields
     return a;
 }
 
-template <class T, bool ZIN>
+// OFF: byte-offset addressing and unconditional LDS reads (fewer VALU, more
+// VGPRs; the ensemble kernel keeps element indices and selected LDS reads,
+// which stay within 64 VGPRs)
+template <class T, bool ZIN, bool OFF = true>
 __device__ __forceinline__ void pres_sweep2_block(const PS2Args<T>& p, int b, T* lds) {
     const int tid = (int)threadIdx.x;
     const int lx = tid % PS2_W, ly = tid / PS2_W;
@@ -1171,21 +1180,45 @@ __device__ __forceinline__ void pres_sweep2_block(const PS2Args<T>& p, int b, T*
     // neighbour flags of the clamped column: the loads of threads outside the
     // grid stay inside it (their values are never used)
     const bool gE = xc + 1 < p.X, gW = xc > 0, gN = yc + 1 < p.Y, gS = yc > 0;
+    // addresses: uniform base + 32-bit byte offset (the column's offset plus
+    // a uniform plane offset); the W and S coefficients are read at the
+    // clamped W/S neighbour (the cell itself on the boundary, scaled by 0)
+    const uint32_t colB = (uint32_t)(yc * p.X + xc) * (uint32_t)sizeof(T);
+    const uint32_t planeB = (uint32_t)(p.X * p.Y) * (uint32_t)sizeof(T);
+    const uint32_t rowB = (uint32_t)(sy * sizeof(T));
     auto load = [&](PS2Plane<T>& q, int zz) {
-        const int zc = min(max(zz, 0), p.Z - 1);
-        const int P = (zc * p.Y + yc) * p.X + xc;
-        const int PU = (min(max(zz + 1, 0), p.Z - 1) * p.Y + yc) * p.X + xc;
-        q.cE = k.cE[P]; q.cN = k.cN[P]; q.cU = k.cU[P];
-        q.cW = k.cE[max(P - 1, 0)];
-        q.cS = k.cN[max(P - sy, 0)];
-        q.inv = k.invDp[P];
-        q.rhs = k.rhs[P];
-        if (!ZIN) {
-            q.vE = x0[gE ? P + 1 : P];
-            q.vW = x0[gW ? P - 1 : P];
-            q.vN = x0[gN ? P + sy : P];
-            q.vS = x0[gS ? P - sy : P];
-            q.vU = x0[PU];
+        if constexpr (OFF) {
+            const uint32_t oP = (uint32_t)min(max(zz, 0), p.Z - 1) * planeB + colB;
+            const uint32_t oU = (uint32_t)min(max(zz + 1, 0), p.Z - 1) * planeB + colB;
+            const uint32_t oW = gW ? oP - (uint32_t)sizeof(T) : oP, oS = gS ? oP - rowB : oP;
+            q.cE = ld_at(k.cE, oP); q.cN = ld_at(k.cN, oP); q.cU = ld_at(k.cU, oP);
+            q.cW = ld_at(k.cE, oW);
+            q.cS = ld_at(k.cN, oS);
+            q.inv = ld_at(k.invDp, oP);
+            q.rhs = ld_at(k.rhs, oP);
+            if (!ZIN) {
+                q.vE = ld_at(x0, gE ? oP + (uint32_t)sizeof(T) : oP);
+                q.vW = ld_at(x0, oW);
+                q.vN = ld_at(x0, gN ? oP + rowB : oP);
+                q.vS = ld_at(x0, oS);
+                q.vU = ld_at(x0, oU);
+            }
+        } else {
+            const int zc = min(max(zz, 0), p.Z - 1);
+            const int P = (zc * p.Y + yc) * p.X + xc;
+            const int PU = (min(max(zz + 1, 0), p.Z - 1) * p.Y + yc) * p.X + xc;
+            q.cE = k.cE[P]; q.cN = k.cN[P]; q.cU = k.cU[P];
+            q.cW = k.cE[max(P - 1, 0)];
+            q.cS = k.cN[max(P - sy, 0)];
+            q.inv = k.invDp[P];
+            q.rhs = k.rhs[P];
+            if (!ZIN) {
+                q.vE = x0[gE ? P + 1 : P];
+                q.vW = x0[gW ? P - 1 : P];
+                q.vN = x0[gN ? P + sy : P];
+                q.vS = x0[gS ? P - sy : P];
+                q.vU = x0[PU];
+            }
         }
     };
     // a face on the domain boundary has no coefficient (the loaded neighbour
@@ -1218,10 +1251,18 @@ __device__ __forceinline__ void pres_sweep2_block(const PS2Args<T>& p, int b, T*
         const int zo = zz - 1;
         if (inner && zo >= z0 && zo < z1) {
             const int P = (zo * p.Y + y) * p.X + x;
+            // on the domain boundary the neighbour is the cell itself (its
+            // own first-sweep value, x1m1, times a zero coefficient)
             const T* L = lds + ((zo + 3) % 3) * PS2_THREADS;
-            const T lE = L[tid + 1], lW = L[tid - 1], lN = L[tid + PS2_W], lS = L[tid - PS2_W];
-            const T vE = hE ? lE : x1m1, vW = hW ? lW : x1m1;
-            const T vN = hN ? lN : x1m1, vS = hS ? lS : x1m1;
+            T vE, vW, vN, vS;
+            if constexpr (OFF) {
+                vE = L[hE ? tid + 1 : tid]; vW = L[hW ? tid - 1 : tid];
+                vN = L[hN ? tid + PS2_W : tid]; vS = L[hS ? tid - PS2_W : tid];
+            } else {
+                const T lE = L[tid + 1], lW = L[tid - 1], lN = L[tid + PS2_W], lS = L[tid - PS2_W];
+                vE = hE ? lE : x1m1; vW = hW ? lW : x1m1;
+                vN = hN ? lN : x1m1; vS = hS ? lS : x1m1;
+            }
             const T vU = zo + 1 < p.Z ? x1 : x1m1, vD = zo > 0 ? x1m2 : x1m1;
             const T mDo = zo > 0 ? (T)1 : (T)0;
             T t2 = 0;
@@ -1374,28 +1415,39 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
     T* const lx1 = lds;                      // [3 planes][3 components][PS2_THREADS]
     T* const lvi = lds + 9 * PS2_THREADS;    // [2 planes][3 components][PS2_THREADS]
     const T q = p.q;
+    // addresses: uniform base + 32-bit byte offset (column offset plus a
+    // uniform plane offset); the W and S coefficients are read at the clamped
+    // W/S neighbour (the cell itself on the boundary, masked to +0)
     const int col = yc * p.X + xc;
-    auto plane = [&](int zz) { return min(max(zz, 0), p.Z - 1) * (p.X * p.Y) + col; };
+    const uint32_t colB = (uint32_t)col * (uint32_t)sizeof(T);
+    const uint32_t planeB = (uint32_t)(p.X * p.Y) * (uint32_t)sizeof(T);
+    const uint32_t dW = gW ? 0u - (uint32_t)sizeof(T) : 0u, dS = gS ? 0u - (uint32_t)(sy * sizeof(T)) : 0u;
+    const uint32_t dO = (uint32_t)oO * (uint32_t)sizeof(T);
+    auto planeoff = [&](int zz) { return (uint32_t)min(max(zz, 0), p.Z - 1) * planeB + colB; };
     auto loadc = [&](VS2Coef<T>& r, int zz) {
-        const int P = plane(zz);
-        const int PW = max(P - 1, 0), PS = max(P - sy, 0);
-        r.SE = p.S[0][P]; r.SN = p.S[1][P]; r.SU = p.S[2][P];
-        r.GE = p.G[0][P]; r.GN = p.G[1][P]; r.GU = p.G[2][P];
-        r.SW = p.S[0][PW]; r.GW = p.G[0][PW];
-        r.SS = p.S[1][PS]; r.GS = p.G[1][PS];
-        r.fl = p.flag[P];
+        const uint32_t oP = planeoff(zz);
+        const uint32_t oW = oP + dW, oS = oP + dS;
+        r.SE = ld_at(p.S[0], oP); r.SN = ld_at(p.S[1], oP); r.SU = ld_at(p.S[2], oP);
+        r.GE = ld_at(p.G[0], oP); r.GN = ld_at(p.G[1], oP); r.GU = ld_at(p.G[2], oP);
+        r.SW = ld_at(p.S[0], oW); r.GW = ld_at(p.G[0], oW);
+        r.SS = ld_at(p.S[1], oS); r.GS = ld_at(p.G[1], oS);
+        r.fl = *(p.flag + oP / (uint32_t)sizeof(T));
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            r.inv[a] = p.invD[a][P];
-            r.b[a] = p.b[a][P];
-            r.vO[a] = p.vin[a][P + oO];
+            r.inv[a] = ld_at(p.invD[a], oP);
+            r.b[a] = ld_at(p.b[a], oP);
+            r.vO[a] = ld_at(p.vin[a], oP + dO);
         }
     };
     auto loadv = [&](T v[3], int zz) {
-        const int P = plane(zz);
+        const uint32_t oP = planeoff(zz);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = p.vin[a][P];
+        for (int a = 0; a < 3; ++a) v[a] = ld_at(p.vin[a], oP);
     };
+    // LDS index of each in-plane neighbour's first-sweep value in the second
+    // sweep (the thread's own entry on the domain boundary)
+    const int jE = hE ? tid + 1 : tid, jW = hW ? tid - 1 : tid;
+    const int jN = hN ? tid + PS2_W : tid, jS = hS ? tid - PS2_W : tid;
     auto ring = [&](int zz) { return lvi + (zz & 1) * (3 * PS2_THREADS); };
     T x1m1[3] = {0, 0, 0}, x1m2[3] = {0, 0, 0};
     T pdS = 0, pdG = 0;  // U coefficients of the plane below the previous one
@@ -1451,9 +1503,7 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const T* L = lx1 + ((zo + 3) % 3) * (3 * PS2_THREADS) + a * PS2_THREADS;
-                const T lE = L[tid + 1], lW = L[tid - 1], lN = L[tid + PS2_W], lS = L[tid - PS2_W];
-                const T vE = hE ? lE : x1m1[a], vW = hW ? lW : x1m1[a];
-                const T vN = hN ? lN : x1m1[a], vS = hS ? lS : x1m1[a];
+                const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
                 const T vU = zo + 1 < p.Z ? x1[a] : x1m1[a], vD2 = zo > 0 ? x1m2[a] : x1m1[a];
                 T A = 0;
                 A -= q * vE * pv.SE;
@@ -1536,7 +1586,7 @@ __global__ __launch_bounds__(1024) void k_batch_pres_sweep2(BatchTable<T> tab, R
         const int mid = (lo + hi + 1) >> 1;
         if (tab.first_ps2[mid] <= t) lo = mid; else hi = mid - 1;
     }
-    pres_sweep2_block<T, ZIN>(ps2_args(tab.params[lo], tab.fields[lo], r), t - tab.first_ps2[lo],
+    pres_sweep2_block<T, ZIN, false>(ps2_args(tab.params[lo], tab.fields[lo], r), t - tab.first_ps2[lo],
                               reinterpret_cast<T*>(lds_raw));
 }
 
