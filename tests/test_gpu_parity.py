@@ -132,10 +132,12 @@ def test_deterministic(gpu):
                                                               ("tiny", 6, 8, True), ("tiny", 5, 8, True),
                                                               ("tiny", 2, 2, False), ("isolated_building", 4, 5, False)])
 def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters, pres_iters, norms):
-    """The two-sweep pressure kernel (temporal blocking over z-marching tiles)
-    gives bit-identical fields to the one-sweep kernel: odd sweep counts end
-    with a single sweep and with norms the last sweep stays single; momentum
-    sweep counts vary alongside."""
+    """The multi-sweep kernels (temporal blocking over z-marching tiles: two
+    momentum sweeps, two or three pressure sweeps per launch) give
+    bit-identical fields to the one-sweep kernels: counts that do not divide
+    end with shorter launches and with norms the last sweep stays single.
+    Runs: one-sweep kernels only; pairs only; triples with the grid's z chunk;
+    triples with 5-plane z chunks."""
     if name == "tiny":
         w = workloads.tiny(vel_iters=vel_iters, pres_iters=pres_iters)
     else:
@@ -144,13 +146,20 @@ def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters
         w.cf.velMaxNumIter = vel_iters
     w.cf.computeNorms = norms
     w.cf.Init()
+    # UWS_SWEEP_FUSION 0: one-sweep kernels only, 2: multi-sweep kernels at any grid size
+    runs = [{"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "2"},
+            {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"}]
     res = []
-    for fuse in ("2", "0"):  # 2: the two-sweep kernel at any grid size, 0: one-sweep kernels only
-        monkeypatch.setenv("UWS_SWEEP_FUSION", fuse)
+    for env in runs:
+        for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_ZCHUNK3"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
         with solver_for(w, p0=seeded_pressure(w)) as s:
             s.step(2)
             res.append((s.velocity(3), s.pressure(), s.norms() if norms else None))
-    assert np.array_equal(res[0][0], res[1][0])
-    assert np.array_equal(res[0][1], res[1][1])
-    if norms:
-        assert np.array_equal(res[0][2], res[1][2])
+    for env, r in zip(runs[1:], res[1:]):
+        assert np.array_equal(r[0], res[0][0]), env
+        assert np.array_equal(r[1], res[0][1]), env
+        if norms:
+            assert np.array_equal(r[2], res[0][2]), env

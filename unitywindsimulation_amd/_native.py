@@ -31,7 +31,7 @@ FIELD_EDDY_VIS, FIELD_WALL_VIS, FIELD_B, FIELD_D, FIELD_DP, FIELD_BP, FIELD_AODU
 FIELD_FACE_VEL = 12
 FIELD_NCOMP = {0: 3, 1: 1, 2: 3, 3: 3, 4: 1, 5: 1, 6: 3, 7: 3, 8: 3, 9: 1, 10: 1, 11: 3, 12: 3}
 
-KERNEL_VEL_SWEEP, KERNEL_PRES_SWEEP, KERNEL_PRES_SWEEP2, KERNEL_VEL_SWEEP2 = 0, 1, 2, 3
+KERNEL_VEL_SWEEP, KERNEL_PRES_SWEEP, KERNEL_PRES_SWEEP2, KERNEL_VEL_SWEEP2, KERNEL_PRES_SWEEP3 = 0, 1, 2, 3, 4
 PREC_FP32, PREC_FP64, PREC_FP32_FP16, PREC_FP32_BF16, PREC_CSTEP = 0, 1, 2, 3, 4
 CSTEP_VEL, CSTEP_PRES, CSTEP_BND, CSTEP_CS = 0, 1, 2, 3
 GRAD_VEL0, GRAD_PRES0, GRAD_BND, GRAD_CS = 0, 1, 2, 3

@@ -94,6 +94,21 @@ int choose_zchunk(int X, int Y, int Z) {
     return best;
 }
 
+// z planes per three-sweep workgroup: each workgroup recomputes two halo
+// planes above and below its chunk and runs alone on its CU (16 waves), so
+// the chunk count is chosen to minimise rounds of 256 workgroups times the
+// planes each marches
+int choose_zchunk3(int X, int Y, int Z) {
+    int best = Z, best_cost = -1;
+    for (int nz = 1; nz <= Z; ++nz) {
+        const int zc = (Z + nz - 1) / nz;
+        const int blocks = pres_sweep3_grid(X, Y, Z, zc);
+        const int cost = ((blocks + 255) / 256) * (zc + 4);
+        if (best_cost < 0 || cost < best_cost) { best = zc; best_cost = cost; }
+    }
+    return best;
+}
+
 template <class T>
 void fill_params(const uws_config& c, ParamsT<T>& p) {
     p.X = c.grid_res[0]; p.Y = c.grid_res[1]; p.Z = c.grid_res[2];
@@ -127,6 +142,11 @@ void fill_params(const uws_config& c, ParamsT<T>& p) {
     if (const char* e = std::getenv("UWS_ZCHUNK")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 1024) p.zchunk = v;
+    }
+    p.zchunk3 = choose_zchunk3(p.X, p.Y, p.Z);
+    if (const char* e = std::getenv("UWS_ZCHUNK3")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= 1024) p.zchunk3 = v;
     }
     p.nonuniform = c.grid_type == UWS_GRID_COLL_NONUNIFORM ? 1 : 0;
 }
@@ -769,7 +789,7 @@ int uws_get_norms(uws_handle h, double* out, int32_t n) {
 }
 
 int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms) {
-    if (!avg_ms || reps <= 0 || kernel < UWS_KERNEL_VEL_SWEEP || kernel > UWS_KERNEL_VEL_SWEEP2)
+    if (!avg_ms || reps <= 0 || kernel < UWS_KERNEL_VEL_SWEEP || kernel > UWS_KERNEL_PRES_SWEEP3)
         return UWS_ERR_INVALID_ARG;
     return with_solver(h, [&](SolverBase& s) { return s.time_kernel(kernel, reps, avg_ms); });
 }

@@ -174,6 +174,7 @@ struct uws_batch {
         double ncells() const { return 1.0; }
         bool fused() const { return use_fusion(b->fuse_mode, g->tab.total_ps2); }
         bool fused_vel() const { return false; }
+        bool fused3() const { return false; }
     };
 
     void sync_case_roles() {

@@ -96,6 +96,16 @@ inline int fusion_mode() {
     return 1;
 }
 inline bool use_fusion(int mode, int blocks) { return mode == 2 || (mode == 1 && blocks >= FUSE_MIN_BLOCKS); }
+// The three-sweep pressure kernel: by grid size, when its z chunk is long
+// enough for the two halo planes on each side to pay (UWS_SWEEP_FUSION = 2
+// forces it at any size unless UWS_SWEEP_DEPTH = 2)
+constexpr int FUSE3_MIN_BLOCKS = 192;
+constexpr int FUSE3_MIN_ZCHUNK = 12;
+inline bool use_fusion3(int mode, int blocks, int zchunk3) {
+    const char* e = std::getenv("UWS_SWEEP_DEPTH");
+    if (e && e[0] == '2') return false;
+    return mode == 2 || (mode == 1 && blocks >= FUSE3_MIN_BLOCKS && zchunk3 >= FUSE3_MIN_ZCHUNK);
+}
 
 // runs whose host-side profiles are evaluated in double
 inline bool wide_host(int precision) { return precision == UWS_PREC_FP64 || precision == UWS_PREC_CSTEP; }
@@ -118,6 +128,7 @@ int flags_from_int(SolverBase& s, const int32_t* flags, int64_t n);
 int normalise_norms(const double* raw, double ncells, double* out, int n);
 int stretched_face_positions(int n, float dx, int uniform, int one_side, float ratio, float* out);
 int choose_zchunk(int X, int Y, int Z);
+int choose_zchunk3(int X, int Y, int Z);
 
 // ---------------------------------------------------------------------------
 // Step driver.  L is a launcher: l(kernel, roles, slot) enqueues one kernel
@@ -212,14 +223,21 @@ int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest) {
     }
     int in = -1, last_in = -1;
     bool broke = false;
-    // pairs of sweeps run as one two-sweep launch where the launcher has it;
-    // the residual-based exit and the last sweep's residual need single sweeps
+    // triples and pairs of sweeps run as one three- or two-sweep launch where
+    // the launcher has them; the residual-based exit and the last sweep's
+    // residual need single sweeps
     const bool residual_exit = coef == 0 && c.cal_residual;
     const bool last_single = coef == 0 && c.norms;
     for (int k = 0; k < c.P;) {
         const int out = in == 0 ? 1 : 0;
         Roles r{};
         r.coef = coef; r.pin = in; r.pout = out;
+        if (l.fused3() && !residual_exit && k + 3 <= c.P && !(last_single && k + 3 == c.P)) {
+            l(KN_PRES_SWEEP3, r);
+            in = out;
+            k += 3;
+            continue;
+        }
         if (l.fused() && !residual_exit && k + 2 <= c.P && !(last_single && k + 2 == c.P)) {
             l(KN_PRES_SWEEP2, r);
             in = out;

@@ -28,6 +28,7 @@ struct ParamsT {
     T wallZ0;
     int normRef;          // flat index of the pressure normalisation cell, -1 when it is inactive
     int zchunk;           // z planes per workgroup of the two-sweep kernels
+    int zchunk3;          // z planes per workgroup of the three-sweep pressure kernel
     int nonuniform;       // 1: stretched grid (GridType.CollNonUniform), spacings from FieldsT::h
 };
 
@@ -118,7 +119,8 @@ enum Kernel {
     KN_PRES_SWEEP2,  // two pressure Jacobi sweeps in one launch (single-case only)
     // stretched-grid variants, selected by launch() when ParamsT::nonuniform is set
     KN_NU_VEL_PRECOMP, KN_NU_VEL_SWEEP, KN_NU_PRES_PRECOMP, KN_NU_VEL_CORRECT, KN_NU_DIVERGENCE,
-    KN_VEL_SWEEP2  // two momentum Jacobi sweeps in one launch (single case, uniform grid, FP32)
+    KN_VEL_SWEEP2,  // two momentum Jacobi sweeps in one launch (single case, uniform grid, FP32)
+    KN_PRES_SWEEP3  // three pressure Jacobi sweeps in one launch (single case, FP32)
 };
 
 // Single-case launch of kernel k on stream s.  `slot` selects the norms
@@ -151,6 +153,7 @@ void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s);
 
 // workgroups of the two-sweep pressure kernel for one grid
 int pres_sweep2_grid(int X, int Y, int Z, int zchunk);
+int pres_sweep3_grid(int X, int Y, int Z, int zchunk);
 
 inline int tiles_of(int X, int Y, int Z, int ty) { return ((X + 63) / 64) * ((Y + ty - 1) / ty) * Z; }
 

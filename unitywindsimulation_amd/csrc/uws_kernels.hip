@@ -1315,6 +1315,166 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 
 
 // ---------------------------------------------------------------------------
+// Three pressure Jacobi sweeps in one launch: the z-march of the two-sweep
+// kernel one level deeper.  A workgroup of 68 x 15 threads owns a 64 x 11
+// column of cells (two halo cells on each side) and marches z from two planes
+// below its chunk to two above.  At plane zz every thread computes the first
+// sweep at zz (s1, into LDS ring 1), the second sweep at zz - 1 (s2, from
+// ring 1's plane zz - 1, written one iteration earlier, into ring 2), and the
+// interior threads the third sweep at zz - 2 (from ring 2's plane zz - 2)
+// into the output; one barrier per plane.  Own-column values above and below
+// come from registers.  Four plane records (coefficients and the input
+// iterate's neighbours) rotate through named variables (the loop is unrolled
+// four times); the loads of plane zz + 1 are issued first.  Expressions,
+// boundary masking and the handling of non-fluid cells are those of the
+// two-sweep kernel, so the result equals three single sweeps.  A thread
+// outside the second sweep's region (the outer halo ring) computes it too, on
+// its own entry, and its value is never used.
+// ---------------------------------------------------------------------------
+constexpr int PS3_W = 68;                    // threads per tile row: 64 outputs + 2 x 2 halo
+constexpr int PS3_BY = 15;                   // tile rows: 11 outputs + 2 x 2 halo
+constexpr int PS3_THREADS = PS3_W * PS3_BY;  // 1020
+
+inline __host__ __device__ int pres_sweep3_blocks(int X, int Y, int Z, int zchunk) {
+    return ((X + 63) / 64) * ((Y + PS3_BY - 5) / (PS3_BY - 4)) * ((Z + zchunk - 1) / zchunk);
+}
+
+template <class T, bool ZIN>
+__device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T* lds) {
+    const int tid = (int)threadIdx.x;
+    const int lx = tid % PS3_W, ly = tid / PS3_W;
+    const int ntx = (p.X + 63) / 64, nty = (p.Y + PS3_BY - 5) / (PS3_BY - 4);
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int x = bx * 64 - 2 + lx, y = by * (PS3_BY - 4) - 2 + ly;
+    const int z0 = bz * p.zchunk, z1 = min(z0 + p.zchunk, p.Z);
+    const bool cell = x >= 0 && x < p.X && y >= 0 && y < p.Y;
+    const bool inner = cell && lx >= 2 && lx <= PS3_W - 3 && ly >= 2 && ly <= PS3_BY - 3;
+    const PresCoef<T> k = p.k;
+    const T* __restrict__ x0 = ZIN ? nullptr : p.x0;
+    T* __restrict__ out = p.out;
+    const bool hW = x > 0, hE = x + 1 < p.X, hS = y > 0, hN = y + 1 < p.Y;
+    const int xc = min(max(x, 0), p.X - 1), yc = min(max(y, 0), p.Y - 1);
+    const bool gE = xc + 1 < p.X, gW = xc > 0, gN = yc + 1 < p.Y, gS = yc > 0;
+    const uint32_t colB = (uint32_t)(yc * p.X + xc) * (uint32_t)sizeof(T);
+    const uint32_t planeB = (uint32_t)(p.X * p.Y) * (uint32_t)sizeof(T);
+    const uint32_t rowB = (uint32_t)(p.X * sizeof(T));
+    // the record of plane zz: its coefficients and the input iterate at its
+    // E, W, N, S neighbours and at the plane above (clamped into the grid)
+    auto load = [&](PS2Plane<T>& q, int zz) {
+        const uint32_t oP = (uint32_t)min(max(zz, 0), p.Z - 1) * planeB + colB;
+        const uint32_t oU = (uint32_t)min(max(zz + 1, 0), p.Z - 1) * planeB + colB;
+        const uint32_t oW = gW ? oP - (uint32_t)sizeof(T) : oP, oS = gS ? oP - rowB : oP;
+        q.cE = ld_at(k.cE, oP); q.cN = ld_at(k.cN, oP); q.cU = ld_at(k.cU, oP);
+        q.cW = ld_at(k.cE, oW);
+        q.cS = ld_at(k.cN, oS);
+        q.inv = ld_at(k.invDp, oP);
+        q.rhs = ld_at(k.rhs, oP);
+        if (!ZIN) {
+            q.vE = ld_at(x0, gE ? oP + (uint32_t)sizeof(T) : oP);
+            q.vW = ld_at(x0, oW);
+            q.vN = ld_at(x0, gN ? oP + rowB : oP);
+            q.vS = ld_at(x0, oS);
+            q.vU = ld_at(x0, oU);
+        }
+    };
+    const T mW = hW ? (T)1 : (T)0, mS = hS ? (T)1 : (T)0;
+    // LDS index of each in-plane neighbour: the thread's own entry on the
+    // domain boundary and on the tile's outer edge
+    const int jE = (hE && lx + 1 < PS3_W) ? tid + 1 : tid, jW = (hW && lx > 0) ? tid - 1 : tid;
+    const int jN = (hN && ly + 1 < PS3_BY) ? tid + PS3_W : tid, jS = (hS && ly > 0) ? tid - PS3_W : tid;
+    T* const L1 = lds;                     // first-sweep values, 3 planes
+    T* const L2 = lds + 3 * PS3_THREADS;   // second-sweep values, 3 planes
+    T s1m1 = 0, s1m2 = 0;  // own first-sweep values at zz - 1, zz - 2
+    T s2m1 = 0, s2m2 = 0;  // own second-sweep values at zz - 2, zz - 3
+    // iteration zz: r0 = plane zz, r1 = zz - 1, r2 = zz - 2, nx = zz - 3
+    // (receives zz + 1)
+    auto step = [&](const PS2Plane<T>& r2, const PS2Plane<T>& r1, const PS2Plane<T>& r0, PS2Plane<T>& nx, int zz) {
+        const T dU3 = nx.cU;  // D coefficient of plane zz - 2
+        T vD = 0;
+        if (!ZIN) vD = r2.vU;  // the input iterate below plane zz (the record of zz - 2 read it)
+        load(nx, zz + 1 <= z1 + 1 ? zz + 1 : z1 + 1);
+        // first sweep at zz
+        T t1 = 0;
+        if (!ZIN) {
+            const T mD = zz > 0 ? (T)1 : (T)0;
+            t1 -= r0.cE * r0.vE;
+            t1 -= (r0.cW * mW) * r0.vW;
+            t1 -= r0.cN * r0.vN;
+            t1 -= (r0.cS * mS) * r0.vS;
+            t1 -= r0.cU * r0.vU;
+            t1 -= (r1.cU * mD) * vD;
+        }
+        const T s1 = r0.inv * (t1 + r0.rhs);
+        L1[((zz + 3) % 3) * PS3_THREADS + tid] = s1;
+        // second sweep at z2 = zz - 1 (needed from z0 - 1)
+        const int z2 = zz - 1;
+        T s2 = 0;
+        if (zz >= z0) {
+            const T* L = L1 + ((z2 + 3) % 3) * PS3_THREADS;
+            const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
+            const T vU = z2 + 1 < p.Z ? s1 : s1m1, vDn = z2 > 0 ? s1m2 : s1m1;
+            const T mD = z2 > 0 ? (T)1 : (T)0;
+            T t2 = 0;
+            t2 -= r1.cE * vE;
+            t2 -= (r1.cW * mW) * vW;
+            t2 -= r1.cN * vN;
+            t2 -= (r1.cS * mS) * vS;
+            t2 -= r1.cU * vU;
+            t2 -= (r2.cU * mD) * vDn;
+            s2 = r1.inv * (t2 + r1.rhs);
+        }
+        L2[((z2 + 3) % 3) * PS3_THREADS + tid] = s2;
+        // third sweep at z3 = zz - 2 into the output
+        const int z3 = zz - 2;
+        if (inner && z3 >= z0 && z3 < z1) {
+            const T* L = L2 + ((z3 + 3) % 3) * PS3_THREADS;
+            const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
+            const T vU = z3 + 1 < p.Z ? s2 : s2m1, vDn = z3 > 0 ? s2m2 : s2m1;
+            const T mD = z3 > 0 ? (T)1 : (T)0;
+            T t3 = 0;
+            t3 -= r2.cE * vE;
+            t3 -= (r2.cW * mW) * vW;
+            t3 -= r2.cN * vN;
+            t3 -= (r2.cS * mS) * vS;
+            t3 -= r2.cU * vU;
+            t3 -= (dU3 * mD) * vDn;
+            out[(z3 * p.Y + y) * p.X + x] = r2.inv * (t3 + r2.rhs);
+        }
+        __syncthreads();
+        s1m2 = s1m1;
+        s1m1 = s1;
+        s2m2 = s2m1;
+        s2m1 = s2;
+    };
+    PS2Plane<T> A, B, C, D;
+    load(A, z0 - 4);
+    load(B, z0 - 3);
+    load(C, z0 - 2);
+    load(D, z0 - 5);
+    for (int zz = z0 - 2;;) {
+        step(A, B, C, D, zz);
+        if (++zz > z1 + 1) break;
+        step(B, C, D, A, zz);
+        if (++zz > z1 + 1) break;
+        step(C, D, A, B, zz);
+        if (++zz > z1 + 1) break;
+        step(D, A, B, C, zz);
+        if (++zz > z1 + 1) break;
+    }
+}
+
+template <bool ZIN>
+__global__ __launch_bounds__(1024) void k_pres_sweep3_f32(ParamsT<float> p, FieldsT<float> f, Roles r) {
+    __shared__ __align__(16) float lds_f[6 * PS3_THREADS];
+    const int nb = pres_sweep3_blocks(p.X, p.Y, p.Z, p.zchunk3);
+    PS2Args<float> a = ps2_args(p, f, r);
+    a.zchunk = p.zchunk3;
+    pres_sweep3_block<float, ZIN>(a, xcd_tile((int)blockIdx.x, nb), lds_f);
+}
+
+// ---------------------------------------------------------------------------
 // Two momentum Jacobi sweeps in one launch: the tiling, z-march and x1 ring
 // of the two-sweep pressure kernel, for the three velocity components at
 // once (the face coefficients S, G are shared by them).  Both sweeps use the
@@ -1721,6 +1881,15 @@ void go_batch_reduce(const BatchTable<T>& tab, int grid, const Roles& r, hipStre
 
 template <class T>
 void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s, int slot) {
+    if (k == KN_PRES_SWEEP3) {
+        // FP32 only (the step driver asks for it there)
+        if constexpr (std::is_same_v<T, float>) {
+            const int nb = pres_sweep3_blocks(p.X, p.Y, p.Z, p.zchunk3);
+            if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep3_f32<true>), dim3(nb), dim3(PS3_THREADS), 0, s, p, f, r);
+            else hipLaunchKernelGGL((k_pres_sweep3_f32<false>), dim3(nb), dim3(PS3_THREADS), 0, s, p, f, r);
+        }
+        return;
+    }
     if (k == KN_VEL_SWEEP2) {
         // FP32 on the uniform grid only (the step driver asks for it there)
         if constexpr (std::is_same_v<T, float>) {
@@ -1801,6 +1970,7 @@ __global__ __launch_bounds__(256) void k_mean_accumulate(const T* u0, const T* u
 }  // namespace
 
 int pres_sweep2_grid(int X, int Y, int Z, int zchunk) { return pres_sweep2_blocks(X, Y, Z, zchunk); }
+int pres_sweep3_grid(int X, int Y, int Z, int zchunk) { return pres_sweep3_blocks(X, Y, Z, zchunk); }
 
 // ---------------------------------------------------------------------------
 // CSInitVelPresFieldsFromBGFlowRotate (InitShaderColl.compute:184-226), the

@@ -69,6 +69,11 @@ struct Solver : SolverBase {
         }
         // the two-sweep momentum kernel: FP32 on the uniform grid
         bool fused_vel() const { return std::is_same<T, float>::value && !s->p.nonuniform && fused(); }
+        // the three-sweep pressure kernel: FP32
+        bool fused3() const {
+            return std::is_same<T, float>::value &&
+                   use_fusion3(s->fuse_mode, pres_sweep3_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk3), s->p.zchunk3);
+        }
     };
     int fuse_mode = fusion_mode();  // two-sweep pressure kernel: see fusion_mode()
 
@@ -477,7 +482,8 @@ struct Solver : SolverBase {
         HIP_OK(hipEventCreate(&e0));
         if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return UWS_ERR_DEVICE; }
         const int w0 = (rs.state + 1) % 3;
-        if (kernel == UWS_KERNEL_VEL_SWEEP2 && !(std::is_same<T, float>::value && !p.nonuniform)) {
+        if ((kernel == UWS_KERNEL_VEL_SWEEP2 && !(std::is_same<T, float>::value && !p.nonuniform)) ||
+            (kernel == UWS_KERNEL_PRES_SWEEP3 && !std::is_same<T, float>::value)) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             return UWS_ERR_UNSUPPORTED;
@@ -490,6 +496,9 @@ struct Solver : SolverBase {
             } else if (kernel == UWS_KERNEL_VEL_SWEEP) {
                 r.state = rs.state; r.vin = rs.state; r.vout = w0;
                 launch<T>(KN_VEL_SWEEP, p, f, r, stream);
+            } else if (kernel == UWS_KERNEL_PRES_SWEEP3) {
+                r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
+                launch<T>(KN_PRES_SWEEP3, p, f, r, stream);
             } else if (kernel == UWS_KERNEL_PRES_SWEEP2) {
                 r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
                 launch<T>(KN_PRES_SWEEP2, p, f, r, stream);
