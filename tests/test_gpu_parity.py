@@ -51,6 +51,16 @@ def variants():
     cf.presBndCondXn = PresBndCond.ZeroGrad
     cf.Init()
     out.append(("mixed_bc_normalised", w))
+    # cell size and density: dx = 0.75 (the kernels divide by dx, 2 dx, dx / 2
+    # and den), dx = 2 with den = 2 (powers of two: products with the exact
+    # reciprocals instead)
+    for tag, dx, den in (("dx075", 0.75, 1.0), ("dx2_den2", 2.0, 2.0)):
+        w = workloads.tiny()
+        X, Y, Z = w.grid
+        w.cf.physDomainSize = (X * dx, Y * dx, Z * dx)
+        w.cf.den = den
+        w.cf.Init()
+        out.append((f"les_piso_{tag}", w))
     return out
 
 
@@ -115,6 +125,27 @@ def test_graph_and_eager_bitwise(gpu, monkeypatch):
         ub, pb = b.velocity(3), b.pressure()
     np.testing.assert_array_equal(ua, ub)
     np.testing.assert_array_equal(pa, pb)
+
+
+@pytest.mark.parametrize("dx,den", [(1.0, 1.0), (2.0, 2.0)])
+def test_reciprocal_path_bitwise(gpu, monkeypatch, dx, den):
+    """Where dx and den are powers of two the cell kernels multiply by exact
+    reciprocals and read host-computed uniform quotients; the fields equal
+    those of the dividing kernels (UWS_NO_P2) bit for bit."""
+    w = workloads.tiny()
+    X, Y, Z = w.grid
+    w.cf.physDomainSize = (X * dx, Y * dx, Z * dx)
+    w.cf.den = den
+    w.cf.Init()
+    out = []
+    for no_p2 in (False, True):
+        if no_p2:
+            monkeypatch.setenv("UWS_NO_P2", "1")
+        with solver_for(w, p0=seeded_pressure(w)) as s:
+            s.step(3)
+            out.append((s.velocity(3), s.pressure(), s.field(N.FIELD_EDDY_VIS), s.field(N.FIELD_B)))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_deterministic(gpu):

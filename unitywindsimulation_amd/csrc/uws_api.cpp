@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -149,6 +150,26 @@ void fill_params(const uws_config& c, ParamsT<T>& p) {
         if (v >= 1 && v <= 1024) p.zchunk3 = v;
     }
     p.nonuniform = c.grid_type == UWS_GRID_COLL_NONUNIFORM ? 1 : 0;
+    // powers-of-two dx and den: products with exact reciprocals in the cell
+    // kernels (UWS_NO_P2 keeps the divisions, for tests)
+    p.p2 = 0;
+    if constexpr (std::is_same_v<T, float> || std::is_same_v<T, double>) {
+        auto pow2 = [](T v) {
+            int e = 0;
+            return v > (T)0 && std::isfinite(v) && std::frexp(v, &e) == (T)0.5;
+        };
+        if (pow2(p.dx) && pow2(p.den) && std::getenv("UWS_NO_P2") == nullptr) {
+            p.p2 = 1;
+            p.rdx = (T)1.0f / p.dx;
+            p.r2dx = (T)1.0f / ((T)2.0f * p.dx);
+            p.rhdx = (T)1.0f / ((T)0.5f * p.dx);
+            p.rden = (T)1.0f / p.den;
+            p.dv_dt = p.dv / p.dt;
+            p.dvden = p.dv / p.den;
+            p.inv_den = (T)1.0f / p.den;
+            p.nuds_dx = p.nu * p.ds / p.dx;
+        }
+    }
 }
 template void fill_params<float>(const uws_config&, ParamsT<float>&);
 template void fill_params<double>(const uws_config&, ParamsT<double>&);

@@ -30,6 +30,14 @@ struct ParamsT {
     int zchunk;           // z planes per workgroup of the two-sweep kernels
     int zchunk3;          // z planes per workgroup of the three-sweep pressure kernel
     int nonuniform;       // 1: stretched grid (GridType.CollNonUniform), spacings from FieldsT::h
+    // p2 = 1 (float, double) where dx and den are powers of two: the cell
+    // kernels multiply by these exact reciprocals instead of dividing by dx,
+    // 2 dx, dx / 2 and den (the same real number, so the same rounding) and
+    // read the uniform quotients below, computed on the host with the same
+    // IEEE operations as the expressions they replace
+    int p2;
+    T rdx, r2dx, rhdx, rden;
+    T dv_dt, dvden, inv_den, nuds_dx;  // dv / dt, dv / den, 1 / den, (nu ds) / dx
 };
 
 // Device fields, structure-of-arrays: every vector field is three planes of

@@ -102,6 +102,24 @@ __device__ __forceinline__ bool fluid(const FieldsT<T>& f, int i) { return f.fla
 template <class T>
 __device__ __forceinline__ bool solid(const FieldsT<T>& f, int i) { return f.flag[i] == UWS_CELL_SOLID; }
 
+// x / d for a uniform divisor d (dx, 2 dx, dx / 2, den): with P2 (d a power
+// of two, plain float types) the product with its exact reciprocal rd, which
+// is the same real number and so rounds the same
+template <bool P2, class T>
+__device__ __forceinline__ T quo(T x, T d, T rd) {
+    if constexpr (P2) return x * rd;
+    else return x / d;
+}
+// body B<T, true> where ParamsT::p2 is set (float, double), else B<T, false>
+#define UWS_P2_DISPATCH(B)                                                  \
+    if constexpr (std::is_same_v<T, float> || std::is_same_v<T, double>) { \
+        if (p.p2) {                                                         \
+            B<T, true>(p, f, c, r);                                         \
+            return;                                                         \
+        }                                                                   \
+    }                                                                       \
+    B<T, false>(p, f, c, r);
+
 // The six neighbours of cell c in the face order E W N S U D: their index
 // (the cell itself where the face is on the domain boundary), whether they
 // lie in the grid, and whether they are fluid.  All six flags are loaded
@@ -163,7 +181,7 @@ __device__ __forceinline__ void fluid_nbrs(const ParamsT<T>& p, const FieldsT<T>
 // LogLawWallFunc (WallFunc.compute:77-123); the rough-wall branch is an
 // extension: u_tau = kappa u_P / ln((dx/2)/z0).
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, bool P2 = false>
 __device__ T log_law_wall_func(const ParamsT<T>& p, T uP) {
     const T EConst = (T)9.8f, k = (T)0.41f;
     const T C = EConst * (p.dx / (T)2.0f) / p.nu;
@@ -185,7 +203,7 @@ __device__ T log_law_wall_func(const ParamsT<T>& p, T uP) {
             }
         }
         const T tauWall = uTau * uTau * p.den;
-        const T strainRate = uP / (p.dx / (T)2.0f);
+        const T strainRate = quo<P2>(uP, p.dx / (T)2.0f, p.rhdx);
         eddyVis = tauWall / (strainRate * p.den) - p.nu;
         if (eddyVis < 0) eddyVis = 0;
     }
@@ -224,15 +242,16 @@ __device__ __forceinline__ void face_vel6(const ParamsT<T>& p, const FieldsT<T>&
 
 // cell velocity gradient g[i][j] = d u_i / d x_j from face velocities
 // (LES.compute:294-306, 353-365)
-template <class T>
+template <class T, bool P2 = false>
 __device__ __forceinline__ void cell_grad(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], const Cell& c,
                                           const int Q[6], const bool in[6], const bool fl[6], T g[3][3]) {
     V3<T> fv[6];
     face_vel6(p, f, un, c, Q, in, fl, fv);
     const V3<T> e = fv[E_], w = fv[W_], n = fv[N_], s = fv[S_], u = fv[U_], d = fv[D_];
-    g[0][0] = (e.x - w.x) / p.dx; g[0][1] = (n.x - s.x) / p.dx; g[0][2] = (u.x - d.x) / p.dx;
-    g[1][0] = (e.y - w.y) / p.dx; g[1][1] = (n.y - s.y) / p.dx; g[1][2] = (u.y - d.y) / p.dx;
-    g[2][0] = (e.z - w.z) / p.dx; g[2][1] = (n.z - s.z) / p.dx; g[2][2] = (u.z - d.z) / p.dx;
+    const T dx = p.dx, rdx = p.rdx;
+    g[0][0] = quo<P2>(e.x - w.x, dx, rdx); g[0][1] = quo<P2>(n.x - s.x, dx, rdx); g[0][2] = quo<P2>(u.x - d.x, dx, rdx);
+    g[1][0] = quo<P2>(e.y - w.y, dx, rdx); g[1][1] = quo<P2>(n.y - s.y, dx, rdx); g[1][2] = quo<P2>(u.y - d.y, dx, rdx);
+    g[2][0] = quo<P2>(e.z - w.z, dx, rdx); g[2][1] = quo<P2>(n.z - s.z, dx, rdx); g[2][2] = quo<P2>(u.z - d.z, dx, rdx);
 }
 
 // face viscosities with eddy viscosity (FvmColl.compute:1596-1635; LES.compute:643-679)
@@ -263,8 +282,8 @@ __device__ __forceinline__ void face_vis(const ParamsT<T>& p, const FieldsT<T>& 
 // CSCalLogLawWallFunc (WallFunc.compute:125-190) and CSCalEddyVis
 // (LES.compute:267-324), fused: both read only u^n and the flags.
 // ---------------------------------------------------------------------------
-template <class T>
-__device__ void body_wall_eddy(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+template <class T, bool P2>
+__device__ void body_wall_eddy_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     T* const* un = f.vel[r.state];
     const int x = c.x, y = c.y, z = c.z, P = c.P;
@@ -273,22 +292,22 @@ __device__ void body_wall_eddy(const ParamsT<T>& p, const FieldsT<T>& f, const C
         T fx = 0, fy = 0, fz = 0;
         if (isF) {
             const V3<T> uP = ld3(un, P);
-            if (x - 1 >= 0 && solid(f, P - 1)) fx += log_law_wall_func(p, sqrt(uP.y * uP.y + uP.z * uP.z));
+            if (x - 1 >= 0 && solid(f, P - 1)) fx += log_law_wall_func<T, P2>(p, sqrt(uP.y * uP.y + uP.z * uP.z));
             if ((y - 1 >= 0 && solid(f, P - p.X)) || (y - 1 < 0 && p.velBnd[UWS_Y0] == UWS_VEL_FIXED_VALUE))
-                fy += log_law_wall_func(p, sqrt(uP.x * uP.x + uP.z * uP.z));
-            if (z - 1 >= 0 && solid(f, P - p.X * p.Y)) fz += log_law_wall_func(p, sqrt(uP.x * uP.x + uP.y * uP.y));
+                fy += log_law_wall_func<T, P2>(p, sqrt(uP.x * uP.x + uP.z * uP.z));
+            if (z - 1 >= 0 && solid(f, P - p.X * p.Y)) fz += log_law_wall_func<T, P2>(p, sqrt(uP.x * uP.x + uP.y * uP.y));
         } else if (solid(f, P)) {
             if (x - 1 >= 0 && fluid(f, P - 1)) {
                 const V3<T> v = ld3(un, P - 1);
-                fx += log_law_wall_func(p, sqrt(v.y * v.y + v.z * v.z));
+                fx += log_law_wall_func<T, P2>(p, sqrt(v.y * v.y + v.z * v.z));
             }
             if (y - 1 >= 0 && fluid(f, P - p.X)) {
                 const V3<T> v = ld3(un, P - p.X);
-                fy += log_law_wall_func(p, sqrt(v.x * v.x + v.z * v.z));
+                fy += log_law_wall_func<T, P2>(p, sqrt(v.x * v.x + v.z * v.z));
             }
             if (z - 1 >= 0 && fluid(f, P - p.X * p.Y)) {
                 const V3<T> v = ld3(un, P - p.X * p.Y);
-                fz += log_law_wall_func(p, sqrt(v.x * v.x + v.y * v.y));
+                fz += log_law_wall_func<T, P2>(p, sqrt(v.x * v.x + v.y * v.y));
             }
         }
         // a float4 written into the reference's single-channel texture keeps only .x
@@ -300,7 +319,7 @@ __device__ void body_wall_eddy(const ParamsT<T>& p, const FieldsT<T>& f, const C
         bool in[6], fl[6];
         nbr6(p, f, c, Q, in, fl);
         T g[3][3];
-        cell_grad(p, f, un, c, Q, in, fl, g);
+        cell_grad<T, P2>(p, f, un, c, Q, in, fl, g);
         // kept for the LES term, which needs it at this cell and its neighbours
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -316,6 +335,11 @@ __device__ void body_wall_eddy(const ParamsT<T>& p, const FieldsT<T>& f, const C
     }
 }
 
+template <class T>
+__device__ void body_wall_eddy(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+    UWS_P2_DISPATCH(body_wall_eddy_t)
+}
+
 // the cell gradient stored by the eddy-viscosity kernel
 template <class T>
 __device__ __forceinline__ void load_grad(const FieldsT<T>& f, int P, T g[3][3]) {
@@ -328,7 +352,7 @@ __device__ __forceinline__ void load_grad(const FieldsT<T>& f, int P, T g[3][3])
 // LES transposed-stress term of one fluid cell (LES.compute:326-683).  The
 // cell gradients of this cell and its fluid neighbours are those the
 // eddy-viscosity kernel computed from the same u^n (cell_grad), read back.
-template <class T>
+template <class T, bool P2 = false>
 __device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>& f, T* const un[3], const Cell& c,
                                           const Nbr6& nb) {
     const int P = c.P;
@@ -360,12 +384,12 @@ __device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>&
         for (int j = 0; j < 3; ++j) ff[j] = (T)0.5f * (g[cc][j] + gq[d][j]);
         // solid neighbour: only the normal derivative, towards u = 0 at the wall
         T fs[3] = {0, 0, 0};
-        fs[cc] = plus ? -uPc / hdx : uPc / hdx;
+        fs[cc] = plus ? quo<P2>(-uPc, hdx, p.rhdx) : quo<P2>(uPc, hdx, p.rhdx);
         // domain boundary, by BC type
         T fb[3] = {0, 0, 0};
         const int t = p.velBnd[bface_of(d)];
         if (t == UWS_VEL_FIXED_VALUE) {
-            fb[cc] = plus ? (ubn[d] - uPc) / hdx : (uPc - ubn[d]) / hdx;
+            fb[cc] = plus ? quo<P2>(ubn[d] - uPc, hdx, p.rhdx) : quo<P2>(uPc - ubn[d], hdx, p.rhdx);
         } else if (t == UWS_VEL_ZERO_GRAD || t == UWS_VEL_SYMMETRY) {
 #pragma unroll
             for (int j = 0; j < 3; ++j)
@@ -385,11 +409,15 @@ __device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>&
 }
 
 // CSCalLesDeferCorrectTerm (LES.compute:326-687) -> bField
+template <class T, bool P2>
+__device__ void body_les_term_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+    if (!c.ok || !fluid(f, c.P)) return;
+    const V3<T> v = les_term<T, P2>(p, f, f.vel[r.state], c, nbrs_of(p, f, c));
+    f.b[0][c.P] = v.x; f.b[1][c.P] = v.y; f.b[2][c.P] = v.z;
+}
 template <class T>
 __device__ void body_les_term(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
-    if (!c.ok || !fluid(f, c.P)) return;
-    const V3<T> v = les_term(p, f, f.vel[r.state], c, nbrs_of(p, f, c));
-    f.b[0][c.P] = v.x; f.b[1][c.P] = v.y; f.b[2][c.P] = v.z;
+    UWS_P2_DISPATCH(body_les_term_t)
 }
 
 // ---------------------------------------------------------------------------
@@ -398,8 +426,8 @@ __device__ void body_les_term(const ParamsT<T>& p, const FieldsT<T>& f, const Ce
 // and b it stores the sweep's face coefficients S, G of the owned E/N/U
 // faces (zero unless both cells are fluid) and 1/D.
 // ---------------------------------------------------------------------------
-template <class T>
-__device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+template <class T, bool P2>
+__device__ void body_vel_precompute_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     T* const* un = f.vel[r.state];
     const int x = c.x, y = c.y, z = c.z, P = c.P;
@@ -429,14 +457,15 @@ __device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, co
             const int d = dirs[a];
             const bool face = isF && nb.fl[d];
             f.S[a][P] = face ? comp(uP, a) + uNn[d] : (T)0;
-            f.G[a][P] = face ? vis[d] * p.ds / p.dx : (T)0;
+            f.G[a][P] = face ? quo<P2>(vis[d] * p.ds, p.dx, p.rdx) : (T)0;
         }
     }
     if (!isF) {
         f.invD[0][P] = 0; f.invD[1][P] = 0; f.invD[2][P] = 0;
         return;
     }
-    T Dx = p.dv / p.dt, Dy = p.dv / p.dt, Dz = p.dv / p.dt;
+    const T dv_dt = P2 ? p.dv_dt : p.dv / p.dt;
+    T Dx = dv_dt, Dy = dv_dt, Dz = dv_dt;
     T bU = 0, bV = 0, bW = 0;
     const T ds = p.ds;
     // advection (:1314-1482)
@@ -472,24 +501,25 @@ __device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, co
         const T pb = p.presBnd[bface_of(d)] == UWS_PRES_FIXED_VALUE ? (T)0 : pP;
         pf[d] = nb.in[d] ? (nb.fl[d] ? (T)0.5f * (pP + pQ[d]) : pP) : pb;
     }
-    bU += -((T)1.0f / p.den) * ds * (pf[E_] - pf[W_]);
-    bV += -((T)1.0f / p.den) * ds * (pf[N_] - pf[S_]);
-    bW += -((T)1.0f / p.den) * ds * (pf[U_] - pf[D_]);
+    const T inv_den = P2 ? p.inv_den : (T)1.0f / p.den;
+    bU += -inv_den * ds * (pf[E_] - pf[W_]);
+    bV += -inv_den * ds * (pf[N_] - pf[S_]);
+    bW += -inv_den * ds * (pf[U_] - pf[D_]);
     // diffusion (:1593-1836)
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         const int cc = d >> 1;
         const int t = p.velBnd[bface_of(d)];
-        const T tin = nb.fl[d] ? vis[d] * ds / p.dx : (T)2.0f * vis[d] * ds / p.dx;
-        const T tw = (T)2.0f * vis[d] * ds / p.dx;
+        const T tw = quo<P2>((T)2.0f * vis[d] * ds, p.dx, p.rdx);
+        const T tin = nb.fl[d] ? quo<P2>(vis[d] * ds, p.dx, p.rdx) : tw;
         // in the grid: all three components; fixed value: all three (and b); symmetry: the normal one
         const bool all3 = nb.in[d] || t == UWS_VEL_FIXED_VALUE;
         const bool norm = !nb.in[d] && t == UWS_VEL_SYMMETRY;
         const T tt = nb.in[d] ? tin : tw;
         const bool fixedb = !nb.in[d] && t == UWS_VEL_FIXED_VALUE;
-        const T bU1 = bU + (T)2.0f * vis[d] * ds / p.dx * ub[d].x;
-        const T bV1 = bV + (T)2.0f * vis[d] * ds / p.dx * ub[d].y;
-        const T bW1 = bW + (T)2.0f * vis[d] * ds / p.dx * ub[d].z;
+        const T bU1 = bU + tw * ub[d].x;
+        const T bV1 = bV + tw * ub[d].y;
+        const T bW1 = bW + tw * ub[d].z;
         bU = fixedb ? bU1 : bU; bV = fixedb ? bV1 : bV; bW = fixedb ? bW1 : bW;
         const bool ax = all3 || (norm && cc == 0), ay = all3 || (norm && cc == 1), az = all3 || (norm && cc == 2);
         Dx = ax ? Dx + tt : Dx;
@@ -497,17 +527,22 @@ __device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, co
         Dz = az ? Dz + tt : Dz;
     }
     // time derivative, force, LES (:1838-1854)
-    bU += p.dv / p.dt * uP.x; bV += p.dv / p.dt * uP.y; bW += p.dv / p.dt * uP.z;
+    bU += dv_dt * uP.x; bV += dv_dt * uP.y; bW += dv_dt * uP.z;
     bU += p.dv * p.fext[0]; bV += p.dv * p.fext[1]; bW += p.dv * p.fext[2];
     if (p.useLES) {
         V3<T> bl;
-        if (r.flags & K_FUSE_LES) bl = les_term(p, f, un, c, nb);
+        if (r.flags & K_FUSE_LES) bl = les_term<T, P2>(p, f, un, c, nb);
         else bl = mk<T>(f.b[0][P], f.b[1][P], f.b[2][P]);
         bU += bl.x; bV += bl.y; bW += bl.z;
     }
     f.D[0][P] = Dx; f.D[1][P] = Dy; f.D[2][P] = Dz;
     f.invD[0][P] = (T)1.0f / Dx; f.invD[1][P] = (T)1.0f / Dy; f.invD[2][P] = (T)1.0f / Dz;
     f.b[0][P] = bU; f.b[1][P] = bV; f.b[2][P] = bW;
+}
+
+template <class T>
+__device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+    UWS_P2_DISPATCH(body_vel_precompute_t)
 }
 
 // ---------------------------------------------------------------------------
@@ -568,13 +603,13 @@ __device__ void body_vel_sweep(const ParamsT<T>& p, const FieldsT<T>& f, const C
 // CSPresCorrectRhieChowPreCompute (FvmColl.compute:2480-2787) -> Dp, bp,
 // 1/Dp, and the pressure face coefficients of the owned E/N/U faces.
 // ---------------------------------------------------------------------------
-template <class T>
-__device__ void body_pres_precompute(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+template <class T, bool P2>
+__device__ void body_pres_precompute_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     T* const* u = f.vel[r.vout];
     const int x = c.x, y = c.y, z = c.z, P = c.P;
     const bool isF = fluid(f, P);
-    const T dvden = p.dv / p.den;
+    const T dvden = P2 ? p.dvden : p.dv / p.den;
     // every neighbour value is loaded unconditionally (indices fall back to the
     // cell itself, or to the first neighbour, outside the grid); the face terms
     // then pick their branch by selects, each branch computed as written
@@ -617,8 +652,8 @@ __device__ void body_pres_precompute(const ParamsT<T>& p, const FieldsT<T>& f, c
             const int d = dirs[a];
             const bool face = isF && fl[d];
             const T DInv = (T)0.5f * (invP[a] + invQ[d]);
-            f.cp[a][P] = face ? dvden * DInv / p.dx : (T)0;
-            if (a == 0) f.cpisoE[P] = face ? dvden * (DInv / p.dx) : (T)0;
+            f.cp[a][P] = face ? quo<P2>(dvden * DInv, p.dx, p.rdx) : (T)0;
+            if (a == 0) f.cpisoE[P] = face ? dvden * quo<P2>(DInv, p.dx, p.rdx) : (T)0;
         }
     }
     if (!isF) {
@@ -626,7 +661,7 @@ __device__ void body_pres_precompute(const ParamsT<T>& p, const FieldsT<T>& f, c
         f.bp[P] = 0;
         return;
     }
-    const T two_dx = (T)2.0f * p.dx;
+    const T two_dx = (T)2.0f * p.dx, r2dx = p.r2dx;
     T Dp = 0, bp = 0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
@@ -635,39 +670,43 @@ __device__ void body_pres_precompute(const ParamsT<T>& p, const FieldsT<T>& f, c
         const T uPc = uP[cc];
         // interior face with a fluid neighbour
         const T DInv = (T)0.5f * (invP[cc] + invQ[d]);
-        const T Dp_i = Dp - dvden * DInv / p.dx;
+        const T Dp_i = Dp - quo<P2>(dvden * DInv, p.dx, p.rdx);
         const T half = (T)0.5f * (uPc + uQ[d]);
         T bp_i = plus ? bp + half : bp - half;
         const bool fF = farF[d], bF = fl[d ^ 1];
         const T pq = pQ[d], pf = pF[d], pb = pQ[d ^ 1];
         T g, ga;
         if (plus) {
-            g = (pq - pP) / p.dx;
-            const T a_fb = (T)0.5f * ((pf - pP) / two_dx + (pq - pb) / two_dx);
-            const T a_f = (T)0.5f * ((pf - pP) / two_dx + (pq - pP) / two_dx);
-            const T a_b = (T)0.5f * ((pq - pP) / two_dx + (pq - pb) / two_dx);
+            g = quo<P2>(pq - pP, p.dx, p.rdx);
+            const T a_fb = (T)0.5f * (quo<P2>(pf - pP, two_dx, r2dx) + quo<P2>(pq - pb, two_dx, r2dx));
+            const T a_f = (T)0.5f * (quo<P2>(pf - pP, two_dx, r2dx) + quo<P2>(pq - pP, two_dx, r2dx));
+            const T a_b = (T)0.5f * (quo<P2>(pq - pP, two_dx, r2dx) + quo<P2>(pq - pb, two_dx, r2dx));
             ga = fF ? (bF ? a_fb : a_f) : (bF ? a_b : g);
         } else {
-            g = (pP - pq) / p.dx;
-            const T a_fb = (T)0.5f * ((pb - pq) / two_dx + (pP - pf) / two_dx);
-            const T a_f = (T)0.5f * ((pP - pq) / two_dx + (pP - pf) / two_dx);
-            const T a_b = (T)0.5f * ((pb - pq) / two_dx + (pP - pq) / two_dx);
+            g = quo<P2>(pP - pq, p.dx, p.rdx);
+            const T a_fb = (T)0.5f * (quo<P2>(pb - pq, two_dx, r2dx) + quo<P2>(pP - pf, two_dx, r2dx));
+            const T a_f = (T)0.5f * (quo<P2>(pP - pq, two_dx, r2dx) + quo<P2>(pP - pf, two_dx, r2dx));
+            const T a_b = (T)0.5f * (quo<P2>(pb - pq, two_dx, r2dx) + quo<P2>(pP - pq, two_dx, r2dx));
             ga = fF ? (bF ? a_fb : a_f) : (bF ? a_b : g);
         }
-        const T vct = -(p.dv * DInv) / p.den * (g - ga);
+        const T vct = quo<P2>(-(p.dv * DInv), p.den, p.rden) * (g - ga);
         bp_i = plus ? bp_i + vct : bp_i - vct;
         // domain boundary face
         const int bf = bface_of(d);
         T bp_b = bp;
         if (p.velBnd[bf] == UWS_VEL_FIXED_VALUE) bp_b = plus ? bp + ubc[d] : bp - ubc[d];
         else if (p.velBnd[bf] == UWS_VEL_ZERO_GRAD) bp_b = plus ? bp + uPc : bp - uPc;
-        const T Dp_b = p.presBnd[bf] == UWS_PRES_FIXED_VALUE ? Dp - (T)2.0f * dvden * invP[cc] / p.dx : Dp;
+        const T Dp_b = p.presBnd[bf] == UWS_PRES_FIXED_VALUE ? Dp - quo<P2>((T)2.0f * dvden * invP[cc], p.dx, p.rdx) : Dp;
         Dp = in[d] ? (fl[d] ? Dp_i : Dp) : Dp_b;
         bp = in[d] ? (fl[d] ? bp_i : bp) : bp_b;
     }
     f.Dp[P] = Dp;
     f.invDp[P] = (T)1.0f / Dp;
     f.bp[P] = bp;
+}
+template <class T>
+__device__ void body_pres_precompute(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+    UWS_P2_DISPATCH(body_pres_precompute_t)
 }
 
 // pressure equation coefficient set: 0 = SIMPLE correction, 1 = PISO corrector
@@ -750,8 +789,8 @@ __device__ void body_pres_normalise(const ParamsT<T>& p, const FieldsT<T>& f, co
 }
 
 // CSApplyVelCorrectionDivergenceTheorem (FvmColl.compute:2950-3121)
-template <class T>
-__device__ void body_vel_correct(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+template <class T, bool P2>
+__device__ void body_vel_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
     if (!fluid(f, P)) return;
@@ -770,14 +809,18 @@ __device__ void body_vel_correct(const ParamsT<T>& p, const FieldsT<T>& f, const
         const T bnd = p.presBnd[bface_of(d)] == UWS_PRES_FIXED_VALUE ? (T)0 : pcP;
         pf[d] = in[d] ? (fl[d] ? (T)0.5f * (pcP + pQ[d]) : pcP) : bnd;
     }
-    const T dvden = p.dv / p.den;
-    const T cx = -dvden * iv[0] * (pf[E_] - pf[W_]) / p.dx;
-    const T cy = -dvden * iv[1] * (pf[N_] - pf[S_]) / p.dx;
-    const T cz = -dvden * iv[2] * (pf[U_] - pf[D_]) / p.dx;
+    const T dvden = P2 ? p.dvden : p.dv / p.den;
+    const T cx = quo<P2>(-dvden * iv[0] * (pf[E_] - pf[W_]), p.dx, p.rdx);
+    const T cy = quo<P2>(-dvden * iv[1] * (pf[N_] - pf[S_]), p.dx, p.rdx);
+    const T cz = quo<P2>(-dvden * iv[2] * (pf[U_] - pf[D_]), p.dx, p.rdx);
     f.ucorr[0][P] = cx; f.ucorr[1][P] = cy; f.ucorr[2][P] = cz;
     if (r.flags & K_NO_INPLACE) return;
     T* const* u = f.vel[r.vout];
     u[0][P] += cx; u[1][P] += cy; u[2][P] += cz;
+}
+template <class T>
+__device__ void body_vel_correct(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+    UWS_P2_DISPATCH(body_vel_correct_t)
 }
 
 // The one-shot kernels below read their six neighbours unconditionally:
@@ -787,8 +830,8 @@ __device__ void body_vel_correct(const ParamsT<T>& p, const FieldsT<T>& f, const
 // branches would each wait for the previous ones.
 
 // CSPisoCalAodUCorrect (PisoColl.compute:70-178): laminar nu, no BC terms
-template <class T>
-__device__ void body_piso_aodu(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+template <class T, bool P2>
+__device__ void body_piso_aodu_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
     if (!fluid(f, P)) return;
@@ -817,7 +860,7 @@ __device__ void body_piso_aodu(const ParamsT<T>& p, const FieldsT<T>& f, const C
             A[a] = ok[d] ? v : A[a];
         }
     }
-    const T g = p.nu * p.ds / p.dx;
+    const T g = P2 ? p.nuds_dx : p.nu * p.ds / p.dx;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
 #pragma unroll
@@ -828,6 +871,10 @@ __device__ void body_piso_aodu(const ParamsT<T>& p, const FieldsT<T>& f, const C
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) f.aodu[a][P] = A[a] + Df[a];
+}
+template <class T>
+__device__ void body_piso_aodu(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+    UWS_P2_DISPATCH(body_piso_aodu_t)
 }
 
 // CSPisoPresCorrection's constant parts (PisoColl.compute:196-255): presCoeff
@@ -872,8 +919,8 @@ __device__ void body_piso_precompute(const ParamsT<T>& p, const FieldsT<T>& f, c
 }
 
 // CSPisoApplyPresCorrection + CSPisoVelCorrection (PisoColl.compute:258-337)
-template <class T>
-__device__ void body_piso_correct(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+template <class T, bool P2>
+__device__ void body_piso_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
     if (!fluid(f, P)) return;
@@ -892,8 +939,9 @@ __device__ void body_piso_correct(const ParamsT<T>& p, const FieldsT<T>& f, cons
     T pf[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) pf[d] = ok[d] ? (T)0.5f * (pcP + pQ[d]) : pcP;
-    const T gx = (pf[E_] - pf[W_]) / p.dx, gy = (pf[N_] - pf[S_]) / p.dx, gz = (pf[U_] - pf[D_]) / p.dx;
-    const T dvden = p.dv / p.den;
+    const T gx = quo<P2>(pf[E_] - pf[W_], p.dx, p.rdx), gy = quo<P2>(pf[N_] - pf[S_], p.dx, p.rdx);
+    const T gz = quo<P2>(pf[U_] - pf[D_], p.dx, p.rdx);
+    const T dvden = P2 ? p.dvden : p.dv / p.den;
     const T cx = -iv[0] * (ao[0] + dvden * gx);
     const T cy = -iv[1] * (ao[1] + dvden * gy);
     const T cz = -iv[2] * (ao[2] + dvden * gz);
@@ -901,6 +949,10 @@ __device__ void body_piso_correct(const ParamsT<T>& p, const FieldsT<T>& f, cons
     if (!inplace) return;
     T* const* u = f.vel[r.vout];
     u[0][P] += cx; u[1][P] += cy; u[2][P] += cz;
+}
+template <class T>
+__device__ void body_piso_correct(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+    UWS_P2_DISPATCH(body_piso_correct_t)
 }
 
 // CSInitModelFlag's velocity reset (InitShaderColl.compute:353-359) applied
