@@ -370,7 +370,9 @@ __device__ __forceinline__ V3<T> les_term(const ParamsT<T>& p, const FieldsT<T>&
         for (int j = 0; j < 3; ++j) gq[d][j] = f.grad[3 * cc + j][nb.Q[d]];
         const int bf = bface_of(d);
         const int k = bf < 2 ? c.z * p.Y + c.y : (bf < 4 ? c.z * p.X + c.x : c.y * p.X + c.x);
-        ubn[d] = f.bnd[bf][cc][k];
+        // boundary textures: read on the domain boundary only
+        ubn[d] = 0;
+        if (!nb.in[d] && p.velBnd[bf] == UWS_VEL_FIXED_VALUE) ubn[d] = f.bnd[bf][cc][k];
     }
     T fg[6][3];
 #pragma unroll
@@ -442,7 +444,10 @@ __device__ void body_vel_precompute_t(const ParamsT<T>& p, const FieldsT<T>& f, 
     for (int d = 0; d < 6; ++d) {
         uNn[d] = un[d >> 1][nb.Q[d]];
         pQ[d] = f.p[nb.Q[d]];
-        ub[d] = bnd_at(p, f, d, x, y, z);
+        // boundary textures: read on the domain boundary only (a test of the
+        // coordinates, so no other load waits on it)
+        ub[d] = mk<T>(0, 0, 0);
+        if (!nb.in[d] && p.velBnd[bface_of(d)] == UWS_VEL_FIXED_VALUE) ub[d] = bnd_at(p, f, d, x, y, z);
     }
     T vis[6];
     if (p.useLES && isF) face_vis(p, f, nb, P, vis);
@@ -643,7 +648,8 @@ __device__ void body_pres_precompute_t(const ParamsT<T>& p, const FieldsT<T>& f,
         pF[d] = f.p[F[d]];
         const int bf = bface_of(d);
         const int k = bf < 2 ? z * p.Y + y : (bf < 4 ? z * p.X + x : y * p.X + x);
-        ubc[d] = f.bnd[bf][cc][k];
+        ubc[d] = 0;
+        if (!in[d] && p.velBnd[bf] == UWS_VEL_FIXED_VALUE) ubc[d] = f.bnd[bf][cc][k];
     }
     {
         const int dirs[3] = {E_, N_, U_};
