@@ -22,12 +22,15 @@ struct BatchGroup {
     int grid = 0;
     int* d_first = nullptr;
     int* d_first2 = nullptr;
+    int* d_first3 = nullptr;
     ParamsT<float>* d_params = nullptr;
     FieldsT<float>* d_fields = nullptr;
     void release() {
         if (d_first) (void)hipFree(d_first);
         if (d_first2) (void)hipFree(d_first2);
         d_first2 = nullptr;
+        if (d_first3) (void)hipFree(d_first3);
+        d_first3 = nullptr;
         if (d_params) (void)hipFree(d_params);
         if (d_fields) (void)hipFree(d_fields);
         d_first = nullptr;
@@ -109,7 +112,23 @@ struct uws_batch {
                 }
                 for (int c : g.cases) params(c).zchunk = zbest;
             }
-            std::vector<int> first(m + 1, 0), first2(m + 1, 0);
+            // z chunk of the group's three-sweep launches: the one that
+            // minimises rounds of 256 workgroups times planes per workgroup
+            // over the group, as choose_zchunk3 does for one grid
+            if (!std::getenv("UWS_ZCHUNK3")) {
+                int zbest = 4, cbest = -1;
+                for (int zc = 4; zc <= 64; ++zc) {
+                    int blocks = 0;
+                    for (int c : g.cases) {
+                        const ParamsT<float>& p = params(c);
+                        blocks += pres_sweep3_grid(p.X, p.Y, p.Z, zc);
+                    }
+                    const int cost = ((blocks + 255) / 256) * (zc + 4);
+                    if (cbest < 0 || cost < cbest) { zbest = zc; cbest = cost; }
+                }
+                for (int c : g.cases) params(c).zchunk3 = zbest;
+            }
+            std::vector<int> first(m + 1, 0), first2(m + 1, 0), first3(m + 1, 0);
             std::vector<ParamsT<float>> ps(m);
             std::vector<FieldsT<float>> fs(m);
             for (int i = 0; i < m; ++i) {
@@ -117,15 +136,19 @@ struct uws_batch {
                 fs[i] = fields(g.cases[i]);
                 first[i + 1] = first[i] + ps[i].tiles;
                 first2[i + 1] = first2[i] + pres_sweep2_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk);
+                first3[i + 1] = first3[i] + pres_sweep3_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk3);
             }
             if (hipMalloc(&g.d_first, sizeof(int) * (m + 1)) != hipSuccess ||
                 hipMalloc(&g.d_first2, sizeof(int) * (m + 1)) != hipSuccess ||
+                hipMalloc(&g.d_first3, sizeof(int) * (m + 1)) != hipSuccess ||
                 hipMalloc(&g.d_params, sizeof(ParamsT<float>) * m) != hipSuccess ||
                 hipMalloc(&g.d_fields, sizeof(FieldsT<float>) * m) != hipSuccess)
                 return UWS_ERR_ALLOC;
             if (hipMemcpyAsync(g.d_first, first.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
                 hipMemcpyAsync(g.d_first2, first2.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
+                    hipSuccess ||
+                hipMemcpyAsync(g.d_first3, first3.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
                 hipMemcpyAsync(g.d_params, ps.data(), sizeof(ParamsT<float>) * m, hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
@@ -139,6 +162,8 @@ struct uws_batch {
             g.tab.first_tile = g.d_first;
             g.tab.total_ps2 = first2[m];
             g.tab.first_ps2 = g.d_first2;
+            g.tab.total_ps3 = first3[m];
+            g.tab.first_ps3 = g.d_first3;
             g.tab.params = g.d_params;
             g.tab.fields = g.d_fields;
             if (persistent) {
@@ -174,7 +199,9 @@ struct uws_batch {
         double ncells() const { return 1.0; }
         bool fused() const { return use_fusion(b->fuse_mode, g->tab.total_ps2); }
         bool fused_vel() const { return false; }
-        bool fused3() const { return false; }
+        bool fused3() const {
+            return use_fusion3(b->fuse_mode, g->tab.total_ps3, b->params(g->cases[0]).zchunk3);
+        }
     };
 
     void sync_case_roles() {

@@ -1793,6 +1793,22 @@ __device__ __forceinline__ int case_of(const BatchTable<T>& tab, int t) {
     return lo;
 }
 
+// three-sweep pressure kernel over every case of an ensemble: workgroup t
+// belongs to the case c with first_ps3[c] <= t < first_ps3[c + 1]
+template <class T, bool ZIN>
+__global__ __launch_bounds__(1024) void k_batch_pres_sweep3(BatchTable<T> tab, Roles r) {
+    __shared__ __align__(16) unsigned char lds_raw[6 * PS3_THREADS * sizeof(T)];
+    const int t = xcd_tile((int)blockIdx.x, tab.total_ps3);
+    int lo = 0, hi = tab.ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab.first_ps3[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    PS2Args<T> a = ps2_args(tab.params[lo], tab.fields[lo], r);
+    a.zchunk = tab.params[lo].zchunk3;
+    pres_sweep3_block<T, ZIN>(a, t - tab.first_ps3[lo], reinterpret_cast<T*>(lds_raw));
+}
+
 // two-sweep pressure kernel over every case of an ensemble: workgroup t
 // belongs to the case c with first_ps2[c] <= t < first_ps2[c + 1]
 template <class T, bool ZIN>
@@ -1996,6 +2012,13 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
 
 template <class T>
 void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot) {
+    if (k == KN_PRES_SWEEP3) {
+        if (r.pin < 0)
+            hipLaunchKernelGGL((k_batch_pres_sweep3<T, true>), dim3(tab.total_ps3), dim3(PS3_THREADS), 0, s, tab, r);
+        else
+            hipLaunchKernelGGL((k_batch_pres_sweep3<T, false>), dim3(tab.total_ps3), dim3(PS3_THREADS), 0, s, tab, r);
+        return;
+    }
     if (k == KN_PRES_SWEEP2) {
         if (r.pin < 0)
             hipLaunchKernelGGL((k_batch_pres_sweep2<T, true>), dim3(tab.total_ps2), dim3(PS2_THREADS), 0, s, tab, r);
