@@ -197,11 +197,9 @@ struct uws_batch {
         }
         int read_norm(int, double*) { return UWS_ERR_UNSUPPORTED; }
         double ncells() const { return 1.0; }
-        bool fused() const { return use_fusion(b->fuse_mode, g->tab.total_ps2); }
+        bool fused() const { return use_fusion(b->fuse_mode); }
         bool fused_vel() const { return false; }
-        bool fused3() const {
-            return use_fusion3(b->fuse_mode, g->tab.total_ps3, b->params(g->cases[0]).zchunk3);
-        }
+        bool fused3() const { return use_fusion3(b->fuse_mode); }
     };
 
     void sync_case_roles() {

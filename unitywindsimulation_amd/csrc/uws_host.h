@@ -83,34 +83,37 @@ struct SolverBase {
     virtual int set_face_positions(const float* const pos[3]) = 0;
     bool nonuniform() const { return cfg.grid_type == UWS_GRID_COLL_NONUNIFORM; }
 };
-// The two-sweep pressure kernel marches z-chunks of 64 x 13-cell tiles; on a
-// grid with fewer workgroups than CUs (256) it leaves the chip idle and two
-// one-sweep launches are faster.
-constexpr int FUSE_MIN_BLOCKS = 256;
-// UWS_SWEEP_FUSION in the environment: 0 = one-sweep kernels only, 2 = the
-// two-sweep kernel at any grid size (for tests), otherwise by grid size
+// Which multi-sweep kernels the step uses.  UWS_SWEEP_FUSION in the
+// environment: 0 = one-sweep kernels only, 2 = every multi-sweep kernel at any
+// grid size (for tests), otherwise the rules below; UWS_SWEEP_DEPTH = 2 keeps
+// the pressure solve to pairs.
+//  - Pressure triples and pairs run at every grid size: on small grids a
+//    launch costs more than its traffic (64^3 runs at 400 MLUPS with triples
+//    in 2-plane z chunks against 308 with single sweeps, 128 x 64 x 64 at 587
+//    against 475, the 100 x 50 x 100 stretched case at 12.9 against 18.9 ms
+//    per step), and on large ones they move less.
+//  - Momentum pairs (one 16-wave workgroup per CU, 122 VGPRs) only where the
+//    z chunk has at least 8 planes and the grid fills the CUs: 384 x 128 x 256
+//    and 256 x 96 x 128 (16-plane chunks) run 13 % and 7 % faster with them,
+//    192 x 96 x 64 and 160 x 96 x 64 (4-plane chunks, 1.5 rounds of
+//    workgroups) 7 % and 8 % slower.
+constexpr int FUSE_MIN_BLOCKS = 256;     // workgroups per launch that fill the 256 CUs
+constexpr int VEL_PAIR_MIN_ZCHUNK = 8;
 inline int fusion_mode() {
     const char* e = std::getenv("UWS_SWEEP_FUSION");
     if (e && e[0] == '0') return 0;
     if (e && e[0] == '2') return 2;
     return 1;
 }
-inline bool use_fusion(int mode, int blocks) { return mode == 2 || (mode == 1 && blocks >= FUSE_MIN_BLOCKS); }
-// The three-sweep pressure kernel: at any grid size whose z chunk has at
-// least 4 planes (UWS_SWEEP_FUSION = 2 forces it everywhere unless
-// UWS_SWEEP_DEPTH = 2).  On small grids a launch costs more than its traffic,
-// so a third of the launches pays for the halo planes: 128 x 64 x 64 runs at
-// 587 MLUPS with triples (4-plane chunks, 192 workgroups) against 475 with
-// single sweeps, and 100 x 50 x 100 at 12.9 ms per step against 18.9.
-constexpr int FUSE3_MIN_BLOCKS = 1;
-constexpr int FUSE3_MIN_ZCHUNK = 4;
-inline bool use_fusion3(int mode, int blocks, int zchunk3) {
+inline bool use_fusion(int mode) { return mode != 0; }
+inline bool use_fusion3(int mode) {
     const char* e = std::getenv("UWS_SWEEP_DEPTH");
     if (e && e[0] == '2') return false;
-    int min_blocks = FUSE3_MIN_BLOCKS, min_zchunk = FUSE3_MIN_ZCHUNK;
-    if (const char* v = std::getenv("UWS_FUSE3_MIN_BLOCKS")) min_blocks = std::atoi(v);
-    if (const char* v = std::getenv("UWS_FUSE3_MIN_ZCHUNK")) min_zchunk = std::atoi(v);
-    return mode == 2 || (mode == 1 && blocks >= min_blocks && zchunk3 >= min_zchunk);
+    return mode != 0;
+}
+inline bool use_vel_pairs(int mode, int blocks, int zchunk) {
+    if (const char* e = std::getenv("UWS_VEL_PAIRS")) return e[0] == '1';
+    return mode == 2 || (mode == 1 && blocks >= FUSE_MIN_BLOCKS && zchunk >= VEL_PAIR_MIN_ZCHUNK);
 }
 
 // runs whose host-side profiles are evaluated in double

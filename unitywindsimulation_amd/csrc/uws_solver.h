@@ -64,16 +64,14 @@ struct Solver : SolverBase {
             return UWS_OK;
         }
         double ncells() const { return (double)s->p.N; }
-        bool fused() const {
-            return use_fusion(s->fuse_mode, pres_sweep2_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk));
-        }
+        bool fused() const { return use_fusion(s->fuse_mode); }
         // the two-sweep momentum kernel: FP32 on the uniform grid
-        bool fused_vel() const { return std::is_same<T, float>::value && !s->p.nonuniform && fused(); }
-        // the three-sweep pressure kernel: FP32
-        bool fused3() const {
-            return std::is_same<T, float>::value &&
-                   use_fusion3(s->fuse_mode, pres_sweep3_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk3), s->p.zchunk3);
+        bool fused_vel() const {
+            return std::is_same<T, float>::value && !s->p.nonuniform &&
+                   use_vel_pairs(s->fuse_mode, pres_sweep2_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk), s->p.zchunk);
         }
+        // the three-sweep pressure kernel: FP32
+        bool fused3() const { return std::is_same<T, float>::value && use_fusion3(s->fuse_mode); }
     };
     int fuse_mode = fusion_mode();  // two-sweep pressure kernel: see fusion_mode()
 
