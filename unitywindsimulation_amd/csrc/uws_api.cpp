@@ -83,7 +83,8 @@ StepCfg step_cfg(const uws_config& c) {
 // above and below its chunk, so longer chunks waste less, as long as the
 // grid still gives one workgroup per CU; a chunk that divides Z avoids a
 // ragged last chunk.  Measured on 256 x 96 x 128 (k_pres_sweep2): zchunk 16
-// (256 workgroups) 22.8 us, 8 (512) 24.2 us, 12 and 24 (ragged) 29.5-30 us.
+// (256 workgroups) 22.8 us, 8 (512) 24.2 us, 12 (352: a second, partial
+// round of workgroups) and 24 (192: idle CUs) 29.5-30 us.
 int choose_zchunk(int X, int Y, int Z) {
     int best = 8, best_score = -1;
     for (int zc : {16, 8, 4}) {
