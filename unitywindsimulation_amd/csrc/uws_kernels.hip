@@ -349,6 +349,12 @@ __device__ __forceinline__ void load_grad(const FieldsT<T>& f, int P, T g[3][3])
         for (int j = 0; j < 3; ++j) g[i][j] = f.grad[3 * i + j][P];
 }
 
+// complex-step and dual types take the per-face sequential forms of the
+// neighbour-heavy kernels (uws_seq.h)
+template <class T>
+constexpr bool plain_v = std::is_same_v<T, float> || std::is_same_v<T, double>;
+#include "uws_seq.h"
+
 // LES transposed-stress term of one fluid cell (LES.compute:326-683).  The
 // cell gradients of this cell and its fluid neighbours are those the
 // eddy-viscosity kernel computed from the same u^n (cell_grad), read back.
@@ -419,7 +425,11 @@ __device__ void body_les_term_t(const ParamsT<T>& p, const FieldsT<T>& f, const 
 }
 template <class T>
 __device__ void body_les_term(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
-    UWS_P2_DISPATCH(body_les_term_t)
+    if constexpr (plain_v<T>) {
+        UWS_P2_DISPATCH(body_les_term_t)
+    } else {
+        body_les_term_seq(p, f, c, r);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -547,7 +557,11 @@ __device__ void body_vel_precompute_t(const ParamsT<T>& p, const FieldsT<T>& f, 
 
 template <class T>
 __device__ void body_vel_precompute(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
-    UWS_P2_DISPATCH(body_vel_precompute_t)
+    if constexpr (plain_v<T>) {
+        UWS_P2_DISPATCH(body_vel_precompute_t)
+    } else {
+        body_vel_precompute_seq(p, f, c, r);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -712,7 +726,11 @@ __device__ void body_pres_precompute_t(const ParamsT<T>& p, const FieldsT<T>& f,
 }
 template <class T>
 __device__ void body_pres_precompute(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
-    UWS_P2_DISPATCH(body_pres_precompute_t)
+    if constexpr (plain_v<T>) {
+        UWS_P2_DISPATCH(body_pres_precompute_t)
+    } else {
+        body_pres_precompute_seq(p, f, c, r);
+    }
 }
 
 // pressure equation coefficient set: 0 = SIMPLE correction, 1 = PISO corrector
@@ -880,13 +898,17 @@ __device__ void body_piso_aodu_t(const ParamsT<T>& p, const FieldsT<T>& f, const
 }
 template <class T>
 __device__ void body_piso_aodu(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
-    UWS_P2_DISPATCH(body_piso_aodu_t)
+    if constexpr (plain_v<T>) {
+        UWS_P2_DISPATCH(body_piso_aodu_t)
+    } else {
+        body_piso_aodu_seq(p, f, c, r);
+    }
 }
 
 // CSPisoPresCorrection's constant parts (PisoColl.compute:196-255): presCoeff
 // and velTerm do not change over the sweeps and are computed once.
 template <class T>
-__device__ void body_piso_precompute(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+__device__ void body_piso_precompute_plain(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
     if (!fluid(f, P)) {
@@ -922,6 +944,12 @@ __device__ void body_piso_precompute(const ParamsT<T>& p, const FieldsT<T>& f, c
     }
     f.invDpiso[P] = (T)1.0f / presCoeff;
     f.bpiso[P] = velTerm;
+}
+
+template <class T>
+__device__ void body_piso_precompute(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
+    if constexpr (plain_v<T>) body_piso_precompute_plain(p, f, c, r);
+    else body_piso_precompute_seq(p, f, c, r);
 }
 
 // CSPisoApplyPresCorrection + CSPisoVelCorrection (PisoColl.compute:258-337)
