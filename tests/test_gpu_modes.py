@@ -73,6 +73,30 @@ def test_low_precision_defect_correction(gpu, prec, tol):
         assert rounds[1] < rounds[0] and rounds[2] < rounds[1]
 
 
+@pytest.mark.parametrize("prec", [N.PREC_FP32_FP16, N.PREC_FP32_BF16])
+@pytest.mark.parametrize("sweeps", [20, 7])
+def test_low_precision_triples_equal_single_sweeps(gpu, monkeypatch, prec, sweeps):
+    """The three-sweep low-precision kernel (its first two sweeps rounded to 2
+    bytes as the stored iterate would be, the last of a round writing p')
+    gives the fields and defect norms of single low-precision sweeps bit for
+    bit, with the grid's z chunk and with 5-plane chunks."""
+    w = workloads.tiny(grid=(20, 14, 17))
+    w.cf.precision, w.cf.refineRounds, w.cf.refineSweeps, w.cf.computeNorms = prec, 3, sweeps, True
+    runs = [{"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"}]
+    res = []
+    for env in runs:
+        for k in ("UWS_SWEEP_FUSION", "UWS_ZCHUNK3"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        with solver_for(w) as s:
+            s.step(2)
+            res.append((s.velocity(3), s.pressure(), s.norms()))
+    for env, r in zip(runs[1:], res[1:]):
+        for a, b in zip(r, res[0]):
+            assert np.array_equal(a, b), env
+
+
 def test_time_mean_matches_oracle(gpu):
     """200 steps of a small LES case; the mean velocity over the last 100
     steps agrees with the oracle's within 1e-2 relative L2."""

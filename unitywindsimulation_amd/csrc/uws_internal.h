@@ -130,7 +130,8 @@ enum Kernel {
     // stretched-grid variants, selected by launch() when ParamsT::nonuniform is set
     KN_NU_VEL_PRECOMP, KN_NU_VEL_SWEEP, KN_NU_PRES_PRECOMP, KN_NU_VEL_CORRECT, KN_NU_DIVERGENCE,
     KN_VEL_SWEEP2,  // two momentum Jacobi sweeps in one launch (single case, uniform grid, FP32)
-    KN_PRES_SWEEP3  // three pressure Jacobi sweeps in one launch (single case, FP32)
+    KN_PRES_SWEEP3,  // three pressure Jacobi sweeps in one launch (single case, FP32)
+    KN_LOWP_SWEEP3   // three low-precision correction sweeps in one launch (single case, FP32)
 };
 
 // Single-case launch of kernel k on stream s.  `slot` selects the norms

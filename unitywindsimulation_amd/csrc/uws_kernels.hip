@@ -1227,6 +1227,11 @@ struct PS2Args {
     PresCoef<T> k;
     const T* x0;
     T* out;
+    // the low-precision correction equation (three-sweep kernel only)
+    const void* ein;   // e in 2-byte storage
+    void* eout;
+    const T* xin;      // p' the round corrects (nullptr: zero)
+    T s;               // scale of e
 };
 template <class T>
 __device__ __forceinline__ PS2Args<T> ps2_args(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r) {
@@ -1425,7 +1430,13 @@ inline __host__ __device__ int pres_sweep3_blocks(int X, int Y, int Z, int zchun
     return ((X + 63) / 64) * ((Y + PS3_BY - 5) / (PS3_BY - 4)) * ((Z + zchunk - 1) / zchunk);
 }
 
-template <class T, bool ZIN>
+// LP = 0: the pressure equation x = 1/Dp (term(x) + rhs), FP32 in and out.
+// LP = 1 (FP16), 2 (BF16): the low-precision correction equation of the
+// defect correction, e = 1/Dp term(e) + s rho with e stored in 2 bytes
+// (body_lowp_sweep): the first and second sweeps' values are rounded to 2
+// bytes as the stored iterate would be, and the third goes to e_out or, when
+// it is the round's last sweep (LAST), to p' = x + e / s.
+template <class T, bool ZIN, int LP = 0, bool LAST = false>
 __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T* lds) {
     const int tid = (int)threadIdx.x;
     const int lx = tid % PS3_W, ly = tid / PS3_W;
@@ -1458,12 +1469,32 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
         q.inv = ld_at(k.invDp, oP);
         q.rhs = ld_at(k.rhs, oP);
         if (!ZIN) {
-            q.vE = ld_at(x0, gE ? oP + (uint32_t)sizeof(T) : oP);
-            q.vW = ld_at(x0, oW);
-            q.vN = ld_at(x0, gN ? oP + rowB : oP);
-            q.vS = ld_at(x0, oS);
-            q.vU = ld_at(x0, oU);
+            if constexpr (LP == 0) {
+                q.vE = ld_at(x0, gE ? oP + (uint32_t)sizeof(T) : oP);
+                q.vW = ld_at(x0, oW);
+                q.vN = ld_at(x0, gN ? oP + rowB : oP);
+                q.vS = ld_at(x0, oS);
+                q.vU = ld_at(x0, oU);
+            } else {
+                const uint32_t es = (uint32_t)sizeof(T);
+                q.vE = ld_lp<LP == 2>(p.ein, (int)((gE ? oP + es : oP) / es));
+                q.vW = ld_lp<LP == 2>(p.ein, (int)(oW / es));
+                q.vN = ld_lp<LP == 2>(p.ein, (int)((gN ? oP + rowB : oP) / es));
+                q.vS = ld_lp<LP == 2>(p.ein, (int)(oS / es));
+                q.vU = ld_lp<LP == 2>(p.ein, (int)(oU / es));
+            }
         }
+    };
+    const T sc = LP == 0 ? (T)0 : p.s;
+    // one sweep's value from its term, and the value the next sweep reads
+    auto comb = [&](T inv, T t, T rhs) {
+        if constexpr (LP == 0) return inv * (t + rhs);
+        else return inv * t + sc * rhs;
+    };
+    auto rnd = [&](T v) {
+        if constexpr (LP == 0) return v;
+        else if constexpr (LP == 2) return (T)__bfloat162float(__float2bfloat16((float)v));
+        else return (T)(float)(_Float16)(float)v;
     };
     const T mW = hW ? (T)1 : (T)0, mS = hS ? (T)1 : (T)0;
     // LDS index of each in-plane neighbour: the thread's own entry on the
@@ -1492,7 +1523,7 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
             t1 -= r0.cU * r0.vU;
             t1 -= (r1.cU * mD) * vD;
         }
-        const T s1 = r0.inv * (t1 + r0.rhs);
+        const T s1 = rnd(comb(r0.inv, t1, r0.rhs));
         L1[((zz + 3) % 3) * PS3_THREADS + tid] = s1;
         // second sweep at z2 = zz - 1 (needed from z0 - 1)
         const int z2 = zz - 1;
@@ -1509,7 +1540,7 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
             t2 -= (r1.cS * mS) * vS;
             t2 -= r1.cU * vU;
             t2 -= (r2.cU * mD) * vDn;
-            s2 = r1.inv * (t2 + r1.rhs);
+            s2 = rnd(comb(r1.inv, t2, r1.rhs));
         }
         L2[((z2 + 3) % 3) * PS3_THREADS + tid] = s2;
         // third sweep at z3 = zz - 2 into the output
@@ -1526,7 +1557,16 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
             t3 -= (r2.cS * mS) * vS;
             t3 -= r2.cU * vU;
             t3 -= (dU3 * mD) * vDn;
-            out[(z3 * p.Y + y) * p.X + x] = r2.inv * (t3 + r2.rhs);
+            const int P3 = (z3 * p.Y + y) * p.X + x;
+            const T e3 = comb(r2.inv, t3, r2.rhs);
+            if constexpr (LP == 0) {
+                out[P3] = e3;
+            } else if constexpr (LAST) {
+                const T xv = p.xin ? p.xin[P3] : (T)0;
+                out[P3] = xv + e3 / sc;
+            } else {
+                st_lp<LP == 2>(p.eout, P3, (float)e3);
+            }
         }
         __syncthreads();
         s1m2 = s1m1;
@@ -1558,6 +1598,23 @@ __global__ __launch_bounds__(1024) void k_pres_sweep3_f32(ParamsT<float> p, Fiel
     PS2Args<float> a = ps2_args(p, f, r);
     a.zchunk = p.zchunk3;
     pres_sweep3_block<float, ZIN>(a, xcd_tile((int)blockIdx.x, nb), lds_f);
+}
+// three sweeps of the low-precision correction equation (FIRST = ZIN: e = 0
+// on input; LAST: the third sweep is the round's last); roles as
+// body_lowp_sweep's
+template <bool ZIN, int LP, bool LAST>
+__global__ __launch_bounds__(1024) void k_lowp_sweep3(ParamsT<float> p, FieldsT<float> f, Roles r) {
+    __shared__ __align__(16) float lds_f[6 * PS3_THREADS];
+    const int nb = pres_sweep3_blocks(p.X, p.Y, p.Z, p.zchunk3);
+    PS2Args<float> a = ps2_args(p, f, r);
+    a.zchunk = p.zchunk3;
+    a.k.rhs = f.rho;
+    a.ein = ZIN ? nullptr : f.e16[r.ein];
+    a.eout = f.e16[r.eout];
+    a.xin = r.pin >= 0 ? f.pc[r.pin] : nullptr;
+    a.out = f.pc[r.pout];
+    a.s = f.scalar[1];
+    pres_sweep3_block<float, ZIN, LP, LAST>(a, xcd_tile((int)blockIdx.x, nb), lds_f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1989,6 +2046,24 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
             const int nb = pres_sweep3_blocks(p.X, p.Y, p.Z, p.zchunk3);
             if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep3_f32<true>), dim3(nb), dim3(PS3_THREADS), 0, s, p, f, r);
             else hipLaunchKernelGGL((k_pres_sweep3_f32<false>), dim3(nb), dim3(PS3_THREADS), 0, s, p, f, r);
+        }
+        return;
+    }
+    if (k == KN_LOWP_SWEEP3) {
+        if constexpr (std::is_same_v<T, float>) {
+            const int nb = pres_sweep3_blocks(p.X, p.Y, p.Z, p.zchunk3);
+            const bool first = (r.flags & K_LOWP_FIRST) != 0, last = (r.flags & K_LOWP_LAST) != 0;
+            const bool bf = (r.flags & K_BF16) != 0;
+#define UWS_LP3(Z, L, LAST)                                                                                \
+    hipLaunchKernelGGL((k_lowp_sweep3<Z, L, LAST>), dim3(nb), dim3(PS3_THREADS), 0, s, p, f, r)
+            if (bf) {
+                if (first) { if (last) UWS_LP3(true, 2, true); else UWS_LP3(true, 2, false); }
+                else { if (last) UWS_LP3(false, 2, true); else UWS_LP3(false, 2, false); }
+            } else {
+                if (first) { if (last) UWS_LP3(true, 1, true); else UWS_LP3(true, 1, false); }
+                else { if (last) UWS_LP3(false, 1, true); else UWS_LP3(false, 1, false); }
+            }
+#undef UWS_LP3
         }
         return;
     }
