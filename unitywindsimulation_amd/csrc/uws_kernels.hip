@@ -1245,8 +1245,9 @@ __device__ __forceinline__ PS2Args<T> ps2_args(const ParamsT<T>& p, const Fields
 
 // OFF: byte-offset addressing and unconditional LDS reads (fewer VALU, more
 // VGPRs; the ensemble kernel keeps element indices and selected LDS reads,
-// which stay within 64 VGPRs)
-template <class T, bool ZIN, bool OFF = true>
+// which stay within 64 VGPRs).  32-bit byte offsets hold N sizeof(T) for
+// types of up to 8 bytes (N < 2^29 cells); wider types index.
+template <class T, bool ZIN, bool OFF = (sizeof(T) <= 8)>
 __device__ __forceinline__ void pres_sweep2_block(const PS2Args<T>& p, int b, T* lds) {
     const int tid = (int)threadIdx.x;
     const int lx = tid % PS2_W, ly = tid / PS2_W;
