@@ -1712,6 +1712,7 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
     const bool gE = xc + 1 < p.X, gW = xc > 0, gN = yc + 1 < p.Y, gS = yc > 0;
     // tile edges: the neighbour outside the tile comes from the thread's own load
     const bool eW = lx == 0, eE = lx == PS2_W - 1, eS = ly == 0, eN = ly == PS2_BY - 1;
+    const bool edge = eW || eE || eS || eN;
     const int oO = eW ? (gW ? -1 : 0) : eE ? (gE ? 1 : 0) : eS ? (gS ? -sy : 0) : eN ? (gN ? sy : 0) : 0;
     // LDS indices of the in-tile neighbours (the thread itself on a tile edge)
     const int iE = eE ? tid : tid + 1, iW = eW ? tid : tid - 1;
@@ -1740,7 +1741,9 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
         for (int a = 0; a < 3; ++a) {
             r.inv[a] = ld_at(p.invD[a], oP);
             r.b[a] = ld_at(p.b[a], oP);
-            r.vO[a] = ld_at(p.vin[a], oP + dO);
+            // only the tile's edge threads need the neighbour outside the
+            // tile (a test of the lane's position, so no load waits on it)
+            r.vO[a] = edge ? ld_at(p.vin[a], oP + dO) : (T)0;
         }
     };
     auto loadv = [&](T v[3], int zz) {
