@@ -9,7 +9,9 @@
 //    sweeps (R rounds of K in the low-precision mode) when a correction
 //    reads it;
 //  - the residual kernels see the last sweep's input and output (a single
-//    sweep), as the FP64 norms require.
+//    sweep), as the FP64 norms require;
+//  - without a residual-based exit the new state buffer is advance_roles',
+//    whether or not the momentum sweeps run in pairs.
 // Exit status 0 when every case passes; the first failure is printed.
 #include <cstdio>
 #include <cstdlib>
@@ -168,9 +170,10 @@ void run_case(int V, int P, int C, bool norms, bool f2, bool fv, bool f3, bool l
     CHECK(rs.state >= 0 && rs.state < 3);
     CHECK(l.vel[rs.state] == V);
     if (V > 0) CHECK(rs.state != state0);
-    // without a residual-based exit the graph path's role change is the
-    // step's own only for single momentum sweeps
-    if (!fv) {
+    // without a residual-based exit the step's role change is advance_roles',
+    // with and without momentum pairs (launch groups sharing roles may differ
+    // in them)
+    if (!residual) {
         RoleState a;
         a.state = state0; a.cur = -1; a.pc_latest = -1; a.piso_done = 0;
         advance_roles(c, a);

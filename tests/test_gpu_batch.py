@@ -12,9 +12,10 @@ from unitywindsimulation_amd.solver import FvmEnsembleHip
 pytestmark = pytest.mark.gpu
 
 
-def small_cases():
-    ws = [workloads.tiny(grid=(16, 8, 12)), workloads.tiny(grid=(40, 12, 20), seed=3),
-          workloads.tiny(grid=(70, 10, 9), seed=5)]
+def small_cases(vel_iters=6):
+    ws = [workloads.tiny(grid=(16, 8, 12), vel_iters=vel_iters),
+          workloads.tiny(grid=(40, 12, 20), seed=3, vel_iters=vel_iters),
+          workloads.tiny(grid=(70, 10, 9), seed=5, vel_iters=vel_iters)]
     return ws
 
 
@@ -69,12 +70,15 @@ def test_ensemble_rejects_mixed_schedules(gpu):
         FvmEnsembleHip([a.cf, b.cf])
 
 
-@pytest.mark.parametrize("group_bytes", [0, 1])
-def test_ensemble_two_sweep_kernel_matches_single_cases(gpu, monkeypatch, group_bytes):
-    """With the two-sweep pressure kernel forced at every size, the ensemble's
-    launch over every case's z-marching blocks equals the single cases."""
+@pytest.mark.parametrize("group_bytes,vel_iters", [(0, 6), (1, 6), (0, 4), (1, 5)])
+def test_ensemble_two_sweep_kernel_matches_single_cases(gpu, monkeypatch, group_bytes, vel_iters):
+    """With the multi-sweep kernels forced at every size (momentum pairs,
+    pressure pairs and triples), the ensemble's launches over every case's
+    z-marching blocks equal the single cases.  Four momentum sweeps as two
+    pairs end in the other work buffer than four single sweeps would, and five
+    end in a single sweep: the ensemble's graph keeps the roles its step left."""
     monkeypatch.setenv("UWS_SWEEP_FUSION", "2")
-    ws = small_cases()
+    ws = small_cases(vel_iters)
     ref = singles(ws, 3)
     got = ensemble_run(ws, 3, 4, False, group_bytes=group_bytes)
     for (ug, pg), (ur, pr) in zip(got, ref):

@@ -3,6 +3,7 @@
 // the whole ensemble, and each step is one captured HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -20,6 +21,7 @@ struct BatchGroup {
     std::vector<int> cases;
     BatchTable<float> tab{};
     int grid = 0;
+    bool vel_pairs = false;  // two-sweep momentum launches
     int* d_first = nullptr;
     int* d_first2 = nullptr;
     int* d_first3 = nullptr;
@@ -51,6 +53,7 @@ struct uws_batch {
     int fuse_mode = fusion_mode();  // two-sweep pressure kernel: see fusion_mode()
     std::vector<BatchGroup> groups;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    RoleState graph_after[3] = {};  // the roles each graph's step leaves
 
     ~uws_batch() {
         drop_graphs();
@@ -166,6 +169,11 @@ struct uws_batch {
             g.tab.first_ps3 = g.d_first3;
             g.tab.params = g.d_params;
             g.tab.fields = g.d_fields;
+            // the momentum pair tiles the grid as the two-sweep pressure kernel
+            // does (first_ps2), with the group's z chunk
+            int zmin = ps[0].zchunk;
+            for (const auto& q : ps) zmin = std::min(zmin, q.zchunk);
+            g.vel_pairs = use_vel_pairs(fuse_mode, g.tab.total_ps2, zmin);
             if (persistent) {
                 // 8 XCDs x 32 CUs, up to 8 resident 256-thread workgroups per CU
                 int gr = 256 * 8 * (4 / ty);
@@ -198,7 +206,7 @@ struct uws_batch {
         int read_norm(int, double*) { return UWS_ERR_UNSUPPORTED; }
         double ncells() const { return 1.0; }
         bool fused() const { return use_fusion(b->fuse_mode); }
-        bool fused_vel() const { return false; }
+        bool fused_vel() const { return g->vel_pairs; }
         bool fused3() const { return use_fusion3(b->fuse_mode); }
     };
 
@@ -232,6 +240,7 @@ struct uws_batch {
             if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return UWS_ERR_DEVICE;
             int rc = enqueue_all();
             hipError_t e = hipStreamEndCapture(stream, &g);
+            graph_after[s0] = rs;
             rs = saved;
             if (rc == UWS_OK && hipGetLastError() != hipSuccess) rc = UWS_ERR_DEVICE;
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -241,7 +250,7 @@ struct uws_batch {
             if (e != hipSuccess) { graph[s0] = nullptr; return UWS_ERR_DEVICE; }
         }
         if (hipGraphLaunch(graph[s0], stream) != hipSuccess) return UWS_ERR_DEVICE;
-        advance_roles(sc, rs);
+        rs = graph_after[s0];
         for (auto& c : cases) {
             c->step += 1;
             c->phys_time += (double)c->cfg.dt;

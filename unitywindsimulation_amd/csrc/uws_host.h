@@ -92,7 +92,7 @@ struct SolverBase {
 //    in 2-plane z chunks against 308 with single sweeps, 128 x 64 x 64 at 587
 //    against 475, the 100 x 50 x 100 stretched case at 12.9 against 18.9 ms
 //    per step), and on large ones they move less.
-//  - Momentum pairs (one 16-wave workgroup per CU, 122 VGPRs) only where the
+//  - Momentum pairs (one 16-wave workgroup per CU, 123 VGPRs) only where the
 //    z chunk has at least 8 planes and the grid fills the CUs: 384 x 128 x 256
 //    and 256 x 96 x 128 (16-plane chunks) run 13 % and 7 % faster with them,
 //    192 x 96 x 64 and 160 x 96 x 64 (4-plane chunks, 1.5 rounds of
@@ -154,17 +154,25 @@ int stage_vel_sweeps(Lr& l, const StepCfg& c, RoleState& rs) {
     int in = s, last_in = s, result = -1;
     bool broke = false;
     // pairs of sweeps run as one two-sweep launch where the launcher has it
-    // (the first sweep's result stays on chip: the pair writes w1 only); the
+    // (the first sweep's result stays on chip: a pair writes one buffer); the
     // residual-based exit and the last sweep's residual need single sweeps
     const bool pairs = l.fused_vel() && !c.cal_residual;
+    auto is_pair = [&](int k) { return pairs && k + 2 <= c.V && !(c.norms && k + 2 == c.V); };
+    // Every launch writes the work buffer it does not read, so the outputs
+    // alternate between w0 and w1.  The first launch's output is chosen from
+    // the number of launches so that the last one writes the buffer V single
+    // sweeps would end in (w1 for even V, w0 for odd V): the resulting roles do
+    // not depend on the launcher's pairs (advance_roles), and launch groups
+    // that share roles may differ in them.
+    int launches = 0;
+    for (int k = 0; k < c.V; k += is_pair(k) ? 2 : 1) ++launches;
+    const int last_out = ((c.V - 1) & 1) ? w1 : w0;
+    const int first_out = (launches & 1) ? last_out : (last_out == w0 ? w1 : w0);
     for (int k = 0; k < c.V;) {
-        // a sweep writes the work buffer it does not read (w0 first, then alternating)
-        const int out = in == w0 ? w1 : w0;
+        const int out = in == s ? first_out : (in == w0 ? w1 : w0);
         Roles r{};
         r.state = s; r.vin = in; r.vout = out;
-        if (pairs && k + 2 <= c.V && !(c.norms && k + 2 == c.V)) {
-            // a pair from u^n or w1 ends in w1 or w0 (never in the buffer it reads)
-            r.vout = in == w1 ? w0 : w1;
+        if (is_pair(k)) {
             l(KN_VEL_SWEEP2, r);
             in = r.vout;
             k += 2;
@@ -382,8 +390,8 @@ int enqueue_step(Lr& l, const StepCfg& c, RoleState& rs) {
     return run_stage(l, c, rs, UWS_STAGE_FINISH);
 }
 
-// the role change of one enqueue_step without a residual-based early exit,
-// for launchers whose momentum sweeps are all single (fused_vel() false)
+// the role change of one enqueue_step without a residual-based early exit
+// (the same with and without two-sweep momentum launches)
 inline void advance_roles(const StepCfg& c, RoleState& rs) {
     const int w0 = (rs.state + 1) % 3, w1 = (rs.state + 2) % 3;
     rs.state = c.V == 0 ? w0 : (((c.V - 1) & 1) ? w1 : w0);

@@ -1913,6 +1913,20 @@ __global__ __launch_bounds__(1024) void k_batch_pres_sweep2(BatchTable<T> tab, R
                               reinterpret_cast<T*>(lds_raw));
 }
 
+// two-sweep momentum kernel over every case of an ensemble: it tiles each
+// grid as the two-sweep pressure kernel does, so workgroup t belongs to the
+// case c with first_ps2[c] <= t < first_ps2[c + 1]
+__global__ __launch_bounds__(1024) void k_batch_vel_sweep2(BatchTable<float> tab, Roles r) {
+    __shared__ __align__(16) float lds_v[VS2_LDS_FLOATS];
+    const int t = xcd_tile((int)blockIdx.x, tab.total_ps2);
+    int lo = 0, hi = tab.ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab.first_ps2[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    vel_sweep2_block<float>(vs2_args(tab.params[lo], tab.fields[lo], r), t - tab.first_ps2[lo], lds_v);
+}
+
 // Persistent ensemble launch: the grid is a multiple of 8 workgroups, XCD j
 // (workgroups j, j+8, ...) walks its own contiguous range of tiles.
 __device__ __forceinline__ void xcd_range(int total, int xcd, int* lo, int* hi) {
@@ -2124,6 +2138,12 @@ void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, 
             hipLaunchKernelGGL((k_batch_pres_sweep3<T, true>), dim3(tab.total_ps3), dim3(PS3_THREADS), 0, s, tab, r);
         else
             hipLaunchKernelGGL((k_batch_pres_sweep3<T, false>), dim3(tab.total_ps3), dim3(PS3_THREADS), 0, s, tab, r);
+        return;
+    }
+    if (k == KN_VEL_SWEEP2) {
+        // the ensemble runs FP32 (the only instantiation)
+        static_assert(std::is_same_v<T, float>, "ensembles run FP32");
+        hipLaunchKernelGGL(k_batch_vel_sweep2, dim3(tab.total_ps2), dim3(PS2_THREADS), 0, s, tab, r);
         return;
     }
     if (k == KN_PRES_SWEEP2) {
