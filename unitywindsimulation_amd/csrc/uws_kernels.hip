@@ -798,11 +798,13 @@ template <class T>
 __device__ void body_apply_pres(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const bool rec = c.P == p.normRef;
+    // p, p' and the flag are loaded together; the flag picks what is stored
+    const T pP = f.p[c.P], pcP = f.pc[r.pin][c.P];
     if (solid(f, c.P)) {
-        if (rec) f.scalar[0] = f.p[c.P];
+        if (rec) f.scalar[0] = pP;
         return;
     }
-    const T v = f.p[c.P] + f.pc[r.pin][c.P];
+    const T v = pP + pcP;
     f.p[c.P] = v;
     if (rec) f.scalar[0] = v;
 }
@@ -817,7 +819,9 @@ template <class T, bool P2>
 __device__ void body_vel_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
-    if (!fluid(f, P)) return;
+    // every load is issued before the cell's flag is tested (its indices
+    // depend on the coordinates only); the flag decides what is stored
+    const bool isF = fluid(f, P);
     const T* __restrict__ pc = f.pc[r.pin];
     int Qs[6];
     bool in[6], fl[6];
@@ -826,7 +830,15 @@ __device__ void body_vel_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, con
     T pQ[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) pQ[d] = pc[Qs[d]];
-    const T iv[3] = {(T)1.0f / f.D[0][P], (T)1.0f / f.D[1][P], (T)1.0f / f.D[2][P]};
+    const T DP[3] = {f.D[0][P], f.D[1][P], f.D[2][P]};
+    const bool inplace = (r.flags & K_NO_INPLACE) == 0;
+    T u0[3] = {0, 0, 0};
+    if (inplace) {
+        T* const* u = f.vel[r.vout];
+        u0[0] = u[0][P]; u0[1] = u[1][P]; u0[2] = u[2][P];
+    }
+    if (!isF) return;
+    const T iv[3] = {(T)1.0f / DP[0], (T)1.0f / DP[1], (T)1.0f / DP[2]};
     T pf[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
@@ -838,9 +850,9 @@ __device__ void body_vel_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, con
     const T cy = quo<P2>(-dvden * iv[1] * (pf[N_] - pf[S_]), p.dx, p.rdx);
     const T cz = quo<P2>(-dvden * iv[2] * (pf[U_] - pf[D_]), p.dx, p.rdx);
     f.ucorr[0][P] = cx; f.ucorr[1][P] = cy; f.ucorr[2][P] = cz;
-    if (r.flags & K_NO_INPLACE) return;
+    if (!inplace) return;
     T* const* u = f.vel[r.vout];
-    u[0][P] += cx; u[1][P] += cy; u[2][P] += cz;
+    u[0][P] = u0[0] + cx; u[1][P] = u0[1] + cy; u[2][P] = u0[2] + cz;
 }
 template <class T>
 __device__ void body_vel_correct(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
@@ -858,7 +870,8 @@ template <class T, bool P2>
 __device__ void body_piso_aodu_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
-    if (!fluid(f, P)) return;
+    // loads before the cell's flag test, which decides only the store
+    const bool isF = fluid(f, P);
     T* const* un = f.vel[r.state];
     const V3<T> uP = ld3(un, P);
     T A[3] = {0, 0, 0}, Df[3] = {0, 0, 0};
@@ -893,6 +906,7 @@ __device__ void body_piso_aodu_t(const ParamsT<T>& p, const FieldsT<T>& f, const
             Df[a] = ok[d] ? v : Df[a];
         }
     }
+    if (!isF) return;
 #pragma unroll
     for (int a = 0; a < 3; ++a) f.aodu[a][P] = A[a] + Df[a];
 }
@@ -911,11 +925,8 @@ template <class T>
 __device__ void body_piso_precompute_plain(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
-    if (!fluid(f, P)) {
-        f.invDpiso[P] = 0;
-        f.bpiso[P] = 0;
-        return;
-    }
+    // loads before the cell's flag test, which decides only the stores
+    const bool isF = fluid(f, P);
     int Qs[6];
     bool ok[6];
     fluid_nbrs(p, f, c, Qs, ok);
@@ -942,6 +953,11 @@ __device__ void body_piso_precompute_plain(const ParamsT<T>& p, const FieldsT<T>
         presCoeff = ok[d] ? pc2 : presCoeff;
         velTerm = ok[d] ? vt2 : velTerm;
     }
+    if (!isF) {
+        f.invDpiso[P] = 0;
+        f.bpiso[P] = 0;
+        return;
+    }
     f.invDpiso[P] = (T)1.0f / presCoeff;
     f.bpiso[P] = velTerm;
 }
@@ -957,7 +973,9 @@ template <class T, bool P2>
 __device__ void body_piso_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
     if (!c.ok) return;
     const int P = c.P;
-    if (!fluid(f, P)) return;
+    // every load (the in-place ones where asked) is issued before the cell's
+    // flag test, which decides only the stores
+    const bool isF = fluid(f, P);
     const T* __restrict__ pc = f.pc[r.pin];
     int Qs[6];
     bool ok[6];
@@ -969,7 +987,14 @@ __device__ void body_piso_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, co
     const T ao[3] = {f.aodu[0][P], f.aodu[1][P], f.aodu[2][P]};
     const T iv[3] = {f.invD[0][P], f.invD[1][P], f.invD[2][P]};
     const bool inplace = (r.flags & K_NO_INPLACE) == 0;
-    if (inplace) f.p[P] += pcP;
+    T pOld = 0, u0[3] = {0, 0, 0};
+    if (inplace) {
+        T* const* u = f.vel[r.vout];
+        pOld = f.p[P];
+        u0[0] = u[0][P]; u0[1] = u[1][P]; u0[2] = u[2][P];
+    }
+    if (!isF) return;
+    if (inplace) f.p[P] = pOld + pcP;
     T pf[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) pf[d] = ok[d] ? (T)0.5f * (pcP + pQ[d]) : pcP;
@@ -982,7 +1007,7 @@ __device__ void body_piso_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, co
     f.ucorr[0][P] = cx; f.ucorr[1][P] = cy; f.ucorr[2][P] = cz;
     if (!inplace) return;
     T* const* u = f.vel[r.vout];
-    u[0][P] += cx; u[1][P] += cy; u[2][P] += cz;
+    u[0][P] = u0[0] + cx; u[1][P] = u0[1] + cy; u[2][P] = u0[2] + cz;
 }
 template <class T>
 __device__ void body_piso_correct(const ParamsT<T>& p, const FieldsT<T>& f, const Cell& c, const Roles& r) {
