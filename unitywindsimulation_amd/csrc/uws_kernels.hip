@@ -1247,6 +1247,10 @@ __device__ __forceinline__ T ld_at(const T* base, uint32_t off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
 }
 template <class T>
+__device__ __forceinline__ void st_at(T* base, uint32_t off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+template <class T>
 struct PS2Args {
     int X, Y, Z, zchunk;
     PresCoef<T> k;
@@ -1451,6 +1455,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 constexpr int PS3_W = 68;                    // threads per tile row: 64 outputs + 2 x 2 halo
 constexpr int PS3_BY = 15;                   // tile rows: 11 outputs + 2 x 2 halo
 constexpr int PS3_THREADS = PS3_W * PS3_BY;  // 1020
+constexpr int PS3_RING = 4;                  // planes per LDS ring (two rings)
 
 inline __host__ __device__ int pres_sweep3_blocks(int X, int Y, int Z, int zchunk) {
     return ((X + 63) / 64) * ((Y + PS3_BY - 5) / (PS3_BY - 4)) * ((Z + zchunk - 1) / zchunk);
@@ -1522,76 +1527,92 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
         else if constexpr (LP == 2) return (T)__bfloat162float(__float2bfloat16((float)v));
         else return (T)(float)(_Float16)(float)v;
     };
-    const T mW = hW ? (T)1 : (T)0, mS = hS ? (T)1 : (T)0;
     // LDS index of each in-plane neighbour: the thread's own entry on the
     // domain boundary and on the tile's outer edge
     const int jE = (hE && lx + 1 < PS3_W) ? tid + 1 : tid, jW = (hW && lx > 0) ? tid - 1 : tid;
     const int jN = (hN && ly + 1 < PS3_BY) ? tid + PS3_W : tid, jS = (hS && ly > 0) ? tid - PS3_W : tid;
-    T* const L1 = lds;                     // first-sweep values, 3 planes
-    T* const L2 = lds + 3 * PS3_THREADS;   // second-sweep values, 3 planes
+    // two rings of PS3_RING planes: first- and second-sweep values.  Plane
+    // z0 - 2 + 4m + k sits in slot k, so unrolled step k addresses its slots
+    // with constant offsets.
+    T* const L1 = lds;
+    T* const L2 = lds + PS3_RING * PS3_THREADS;
+    // A record's boundary masks, applied once before its first use: no W or S
+    // neighbour on the domain boundary, and no D neighbour above a plane below
+    // the grid (a plane's U coefficient is the D coefficient of the plane
+    // above).  A zeroed term adds -(+0 * v), where the two-sweep kernel adds
+    // -((c * 0) * v): both are zeros and leave the sum unchanged (it starts at
+    // +0 and never becomes -0).
+    auto mask = [&](PS2Plane<T>& q, int zz) {
+        q.cW = hW ? q.cW : (T)0;
+        q.cS = hS ? q.cS : (T)0;
+        if (zz < 0) q.cU = 0;
+    };
     T s1m1 = 0, s1m2 = 0;  // own first-sweep values at zz - 1, zz - 2
     T s2m1 = 0, s2m2 = 0;  // own second-sweep values at zz - 2, zz - 3
     // iteration zz: r0 = plane zz, r1 = zz - 1, r2 = zz - 2, nx = zz - 3
-    // (receives zz + 1)
-    auto step = [&](const PS2Plane<T>& r2, const PS2Plane<T>& r1, const PS2Plane<T>& r0, PS2Plane<T>& nx, int zz) {
+    // (receives zz + 1); kc: the step's position in the unrolled loop
+    auto step = [&](auto kc, PS2Plane<T>& r2, PS2Plane<T>& r1, PS2Plane<T>& r0, PS2Plane<T>& nx, int zz) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int S0 = K, S1 = (K + 3) & 3, S2 = (K + 2) & 3;  // slots of planes zz, zz - 1, zz - 2
         const T dU3 = nx.cU;  // D coefficient of plane zz - 2
         T vD = 0;
         if (!ZIN) vD = r2.vU;  // the input iterate below plane zz (the record of zz - 2 read it)
         load(nx, zz + 1 <= z1 + 1 ? zz + 1 : z1 + 1);
+        mask(r0, zz);
         // first sweep at zz
         T t1 = 0;
         if (!ZIN) {
-            const T mD = zz > 0 ? (T)1 : (T)0;
             t1 -= r0.cE * r0.vE;
-            t1 -= (r0.cW * mW) * r0.vW;
+            t1 -= r0.cW * r0.vW;
             t1 -= r0.cN * r0.vN;
-            t1 -= (r0.cS * mS) * r0.vS;
+            t1 -= r0.cS * r0.vS;
             t1 -= r0.cU * r0.vU;
-            t1 -= (r1.cU * mD) * vD;
+            t1 -= r1.cU * vD;
         }
         const T s1 = rnd(comb(r0.inv, t1, r0.rhs));
-        L1[((zz + 3) % 3) * PS3_THREADS + tid] = s1;
+        L1[S0 * PS3_THREADS + tid] = s1;
         // second sweep at z2 = zz - 1 (needed from z0 - 1)
         const int z2 = zz - 1;
         T s2 = 0;
         if (zz >= z0) {
-            const T* L = L1 + ((z2 + 3) % 3) * PS3_THREADS;
+            const T* L = L1 + S1 * PS3_THREADS;
             const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
             const T vU = z2 + 1 < p.Z ? s1 : s1m1, vDn = z2 > 0 ? s1m2 : s1m1;
-            const T mD = z2 > 0 ? (T)1 : (T)0;
             T t2 = 0;
             t2 -= r1.cE * vE;
-            t2 -= (r1.cW * mW) * vW;
+            t2 -= r1.cW * vW;
             t2 -= r1.cN * vN;
-            t2 -= (r1.cS * mS) * vS;
+            t2 -= r1.cS * vS;
             t2 -= r1.cU * vU;
-            t2 -= (r2.cU * mD) * vDn;
+            t2 -= r2.cU * vDn;
             s2 = rnd(comb(r1.inv, t2, r1.rhs));
         }
-        L2[((z2 + 3) % 3) * PS3_THREADS + tid] = s2;
+        L2[S1 * PS3_THREADS + tid] = s2;
         // third sweep at z3 = zz - 2 into the output
         const int z3 = zz - 2;
         if (inner && z3 >= z0 && z3 < z1) {
-            const T* L = L2 + ((z3 + 3) % 3) * PS3_THREADS;
+            const T* L = L2 + S2 * PS3_THREADS;
             const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
             const T vU = z3 + 1 < p.Z ? s2 : s2m1, vDn = z3 > 0 ? s2m2 : s2m1;
-            const T mD = z3 > 0 ? (T)1 : (T)0;
             T t3 = 0;
             t3 -= r2.cE * vE;
-            t3 -= (r2.cW * mW) * vW;
+            t3 -= r2.cW * vW;
             t3 -= r2.cN * vN;
-            t3 -= (r2.cS * mS) * vS;
+            t3 -= r2.cS * vS;
             t3 -= r2.cU * vU;
-            t3 -= (dU3 * mD) * vDn;
-            const int P3 = (z3 * p.Y + y) * p.X + x;
+            t3 -= dU3 * vDn;
             const T e3 = comb(r2.inv, t3, r2.rhs);
             if constexpr (LP == 0) {
-                out[P3] = e3;
-            } else if constexpr (LAST) {
-                const T xv = p.xin ? p.xin[P3] : (T)0;
-                out[P3] = xv + e3 / sc;
+                // an inner thread's column is in the grid: colB is its own
+                st_at(out, (uint32_t)z3 * planeB + colB, e3);
             } else {
-                st_lp<LP == 2>(p.eout, P3, (float)e3);
+                const int P3 = (z3 * p.Y + y) * p.X + x;
+                if constexpr (LAST) {
+                    const T xv = p.xin ? p.xin[P3] : (T)0;
+                    out[P3] = xv + e3 / sc;
+                } else {
+                    st_lp<LP == 2>(p.eout, P3, (float)e3);
+                }
             }
         }
         __syncthreads();
@@ -1605,21 +1626,29 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
     load(B, z0 - 3);
     load(C, z0 - 2);
     load(D, z0 - 5);
+    // C is masked by its first step
+    mask(A, z0 - 4);
+    mask(B, z0 - 3);
+    mask(D, z0 - 5);
+    using K0 = std::integral_constant<int, 0>;
+    using K1 = std::integral_constant<int, 1>;
+    using K2 = std::integral_constant<int, 2>;
+    using K3 = std::integral_constant<int, 3>;
     for (int zz = z0 - 2;;) {
-        step(A, B, C, D, zz);
+        step(K0{}, A, B, C, D, zz);
         if (++zz > z1 + 1) break;
-        step(B, C, D, A, zz);
+        step(K1{}, B, C, D, A, zz);
         if (++zz > z1 + 1) break;
-        step(C, D, A, B, zz);
+        step(K2{}, C, D, A, B, zz);
         if (++zz > z1 + 1) break;
-        step(D, A, B, C, zz);
+        step(K3{}, D, A, B, C, zz);
         if (++zz > z1 + 1) break;
     }
 }
 
 template <bool ZIN>
 __global__ __launch_bounds__(1024) void k_pres_sweep3_f32(ParamsT<float> p, FieldsT<float> f, Roles r) {
-    __shared__ __align__(16) float lds_f[6 * PS3_THREADS];
+    __shared__ __align__(16) float lds_f[2 * PS3_RING * PS3_THREADS];
     const int nb = pres_sweep3_blocks(p.X, p.Y, p.Z, p.zchunk3);
     PS2Args<float> a = ps2_args(p, f, r);
     a.zchunk = p.zchunk3;
@@ -1630,7 +1659,7 @@ __global__ __launch_bounds__(1024) void k_pres_sweep3_f32(ParamsT<float> p, Fiel
 // body_lowp_sweep's
 template <bool ZIN, int LP, bool LAST>
 __global__ __launch_bounds__(1024) void k_lowp_sweep3(ParamsT<float> p, FieldsT<float> f, Roles r) {
-    __shared__ __align__(16) float lds_f[6 * PS3_THREADS];
+    __shared__ __align__(16) float lds_f[2 * PS3_RING * PS3_THREADS];
     const int nb = pres_sweep3_blocks(p.X, p.Y, p.Z, p.zchunk3);
     PS2Args<float> a = ps2_args(p, f, r);
     a.zchunk = p.zchunk3;
@@ -1911,7 +1940,7 @@ __device__ __forceinline__ int case_of(const BatchTable<T>& tab, int t) {
 // belongs to the case c with first_ps3[c] <= t < first_ps3[c + 1]
 template <class T, bool ZIN>
 __global__ __launch_bounds__(1024) void k_batch_pres_sweep3(BatchTable<T> tab, Roles r) {
-    __shared__ __align__(16) unsigned char lds_raw[6 * PS3_THREADS * sizeof(T)];
+    __shared__ __align__(16) unsigned char lds_raw[2 * PS3_RING * PS3_THREADS * sizeof(T)];
     const int t = xcd_tile((int)blockIdx.x, tab.total_ps3);
     int lo = 0, hi = tab.ncases - 1;
     while (lo < hi) {
