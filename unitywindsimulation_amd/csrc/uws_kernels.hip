@@ -1683,7 +1683,7 @@ __global__ __launch_bounds__(1024) void k_lowp_sweep3(ParamsT<float> p, FieldsT<
 // the loads of the next plane too), so the in-plane neighbours of the input
 // iterate come from LDS rather than from per-thread loads:
 //  - each thread loads its own column's iterate two planes ahead and stores
-//    it in a two-plane LDS ring one iteration ahead of its use; the first
+//    it in a three-plane LDS ring one iteration ahead of its use; the first
 //    sweep at plane zz reads the E, W, N, S values of plane zz from the ring
 //    and the D value (plane zz - 1) from its own ring entry;
 //  - the tile's edge threads (lx 0 or 65, ly 0 or 14) also load the one
@@ -1691,10 +1691,13 @@ __global__ __launch_bounds__(1024) void k_lowp_sweep3(ParamsT<float> p, FieldsT<
 //    used);
 //  - the coefficients (own E/N/U faces and the W, S faces of the neighbours)
 //    are loaded per thread one plane ahead, in three records that rotate
-//    through named variables (the loop is unrolled three times).
+//    through named variables (the loop is unrolled three times, the period of
+//    both LDS rings, so every LDS address is a constant offset).
+//  - a record's boundary masks are applied once, before the first sweep that
+//    reads it.
 // Differences from the pressure kernel:
 //  - the W, S and D coefficients are zeroed on the domain boundary by a bit
-//    mask (S can be negative: a product with 0 would give -0 where the
+//    mask, to +0 (S can be negative: a product with 0 would give -0 where the
 //    one-sweep kernel has the +0 of its select);
 //  - the flag is read: the first sweep's value of a non-fluid cell is +0 (its
 //    neighbours' coefficients towards it are zero), and the second sweep
@@ -1744,9 +1747,9 @@ __device__ __forceinline__ float keep_or_zero(float v, bool keep) {
 __device__ __forceinline__ double keep_or_zero(double v, bool keep) {
     return __longlong_as_double(__double_as_longlong(v) & (keep ? -1ll : 0ll));
 }
-// LDS of one workgroup: the x1 ring (3 planes) and the input ring (2 planes),
-// three components each
-constexpr int VS2_LDS_FLOATS = (3 + 2) * 3 * PS2_THREADS;
+// LDS of one workgroup: the x1 ring and the input ring, 3 planes of three
+// components each
+constexpr int VS2_LDS_FLOATS = (3 + 3) * 3 * PS2_THREADS;
 
 template <class T>
 __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* lds) {
@@ -1771,8 +1774,10 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
     // LDS indices of the in-tile neighbours (the thread itself on a tile edge)
     const int iE = eE ? tid : tid + 1, iW = eW ? tid : tid - 1;
     const int iN = eN ? tid : tid + PS2_W, iS = eS ? tid : tid - PS2_W;
-    T* const lx1 = lds;                      // [3 planes][3 components][PS2_THREADS]
-    T* const lvi = lds + 9 * PS2_THREADS;    // [2 planes][3 components][PS2_THREADS]
+    // both rings hold plane z0 - 1 + 3m + k in slot k, so unrolled step k
+    // addresses its slots with constant offsets
+    T* const lx1 = lds;                      // [3 slots][3 components][PS2_THREADS]
+    T* const lvi = lds + 9 * PS2_THREADS;    // [3 slots][3 components][PS2_THREADS]
     const T q = p.q;
     // addresses: uniform base + 32-bit byte offset (column offset plus a
     // uniform plane offset); the W and S coefficients are read at the clamped
@@ -1809,20 +1814,32 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
     // sweep (the thread's own entry on the domain boundary)
     const int jE = hE ? tid + 1 : tid, jW = hW ? tid - 1 : tid;
     const int jN = hN ? tid + PS2_W : tid, jS = hS ? tid - PS2_W : tid;
-    auto ring = [&](int zz) { return lvi + (zz & 1) * (3 * PS2_THREADS); };
+    // A record's boundary masks, applied once before its first use (the
+    // values both sweeps read): W and S on the domain boundary, and the U
+    // coefficients of a plane below the grid, which are the D coefficients of
+    // the plane above
+    auto maskc = [&](VS2Coef<T>& r, int zz) {
+        r.SW = keep_or_zero(r.SW, hW); r.GW = keep_or_zero(r.GW, hW);
+        r.SS = keep_or_zero(r.SS, hS); r.GS = keep_or_zero(r.GS, hS);
+        if (zz < 0) { r.SU = 0; r.GU = 0; }
+    };
     T x1m1[3] = {0, 0, 0}, x1m2[3] = {0, 0, 0};
     T pdS = 0, pdG = 0;  // U coefficients of the plane below the previous one
     // one plane: loads of the coefficients of zz + 1 into nx and of the own
     // iterate at zz + 2 into vn; first sweep at zz (vu: own iterate at zz + 1);
-    // vu into the input ring; second sweep at zz - 1 from pv and the x1 ring
-    auto step = [&](const VS2Coef<T>& pv, const VS2Coef<T>& cu, VS2Coef<T>& nx, const T vu[3], T vn[3], int zz) {
+    // vu into the input ring; second sweep at zz - 1 from pv and the x1 ring.
+    // kc: the step's position in the unrolled loop
+    auto step = [&](auto kc, const VS2Coef<T>& pv, VS2Coef<T>& cu, VS2Coef<T>& nx, const T vu[3], T vn[3],
+                    int zz) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int S0 = K, Sm = (K + 2) % 3, Sp = (K + 1) % 3;  // slots of planes zz, zz - 1, zz + 1
         loadc(nx, zz + 1 <= z1 ? zz + 1 : z1);
         loadv(vn, zz + 2 <= z1 + 1 ? zz + 2 : z1 + 1);
-        const T SW = keep_or_zero(cu.SW, hW), GW = keep_or_zero(cu.GW, hW);
-        const T SS = keep_or_zero(cu.SS, hS), GS = keep_or_zero(cu.GS, hS);
-        const T SD = keep_or_zero(pv.SU, zz > 0), GD = keep_or_zero(pv.GU, zz > 0);
-        const T* Lc = ring(zz);
-        const T* Ld = ring(zz - 1);
+        maskc(cu, zz);
+        const T SW = cu.SW, GW = cu.GW, SS = cu.SS, GS = cu.GS;
+        const T SD = pv.SU, GD = pv.GU;
+        const T* Lc = lvi + S0 * (3 * PS2_THREADS);
+        const T* Ld = lvi + Sm * (3 * PS2_THREADS);
         T x1[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -1847,23 +1864,23 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
             Df += GD * vD;
             const T v = cu.inv[a] * (A + Df + cu.b[a]);
             x1[a] = cu.fl == UWS_CELL_FLUID ? v : (T)0;
-            lx1[((zz + 3) % 3) * (3 * PS2_THREADS) + a * PS2_THREADS + tid] = x1[a];
+            lx1[S0 * (3 * PS2_THREADS) + a * PS2_THREADS + tid] = x1[a];
         }
-        // the own iterate at zz + 1 for the next plane's first sweep (ring slot
-        // of zz - 1: read above by this thread only)
-        T* Ln = ring(zz + 1);
+        // the own iterate at zz + 1 for the next plane's first sweep (the slot
+        // of plane zz - 2, whose last reader was the previous plane)
+        T* Ln = lvi + Sp * (3 * PS2_THREADS);
 #pragma unroll
         for (int a = 0; a < 3; ++a) Ln[a * PS2_THREADS + tid] = vu[a];
         __syncthreads();
         const int zo = zz - 1;
         if (inner && zo >= z0 && zo < z1 && pv.fl == UWS_CELL_FLUID) {
-            const int P = (zo * p.Y + y) * p.X + x;
-            const T SW2 = keep_or_zero(pv.SW, hW), GW2 = keep_or_zero(pv.GW, hW);
-            const T SS2 = keep_or_zero(pv.SS, hS), GS2 = keep_or_zero(pv.GS, hS);
-            const T SD2 = keep_or_zero(pdS, zo > 0), GD2 = keep_or_zero(pdG, zo > 0);
+            // an inner thread's column is in the grid: the offset is its own
+            const uint32_t oP = planeoff(zo);
+            const T SW2 = pv.SW, GW2 = pv.GW, SS2 = pv.SS, GS2 = pv.GS;
+            const T SD2 = pdS, GD2 = pdG;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                const T* L = lx1 + ((zo + 3) % 3) * (3 * PS2_THREADS) + a * PS2_THREADS;
+                const T* L = lx1 + Sm * (3 * PS2_THREADS) + a * PS2_THREADS;
                 const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
                 const T vU = zo + 1 < p.Z ? x1[a] : x1m1[a], vD2 = zo > 0 ? x1m2[a] : x1m1[a];
                 T A = 0;
@@ -1880,7 +1897,7 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
                 Df += GS2 * vS;
                 Df += pv.GU * vU;
                 Df += GD2 * vD2;
-                p.out[a][P] = pv.inv[a] * (A + Df + pv.b[a]);
+                st_at(p.out[a], oP, pv.inv[a] * (A + Df + pv.b[a]));
             }
         }
 #pragma unroll
@@ -1891,16 +1908,17 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
         pdS = pv.SU;
         pdG = pv.GU;
     };
-    // prologue: the input ring holds the own iterate at z0 - 2 and z0 - 1
-    // (the first plane's D and in-plane values), V1 that at z0 (its U value)
+    // prologue: the input ring holds the own iterate at z0 - 2 (slot 2) and
+    // z0 - 1 (slot 0), the first plane's D and in-plane values; V1 that at z0
+    // (its U value)
     {
         T v[3];
         loadv(v, z0 - 2);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) ring(z0 - 2)[a * PS2_THREADS + tid] = v[a];
+        for (int a = 0; a < 3; ++a) lvi[2 * (3 * PS2_THREADS) + a * PS2_THREADS + tid] = v[a];
         loadv(v, z0 - 1);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) ring(z0 - 1)[a * PS2_THREADS + tid] = v[a];
+        for (int a = 0; a < 3; ++a) lvi[a * PS2_THREADS + tid] = v[a];
     }
     // records on entry of step(pv, cu, nx, ...): pv = plane zz - 1, cu = plane zz
     VS2Coef<T> A, B, C;
@@ -1908,13 +1926,17 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
     loadc(A, z0 - 2);
     loadc(B, z0 - 1);
     loadv(V1, z0);
+    maskc(A, z0 - 2);  // B is masked by its first step
     __syncthreads();
+    using K0 = std::integral_constant<int, 0>;
+    using K1 = std::integral_constant<int, 1>;
+    using K2 = std::integral_constant<int, 2>;
     for (int zz = z0 - 1;;) {
-        step(A, B, C, V1, V2, zz);
+        step(K0{}, A, B, C, V1, V2, zz);
         if (++zz > z1) break;
-        step(B, C, A, V2, V3, zz);
+        step(K1{}, B, C, A, V2, V3, zz);
         if (++zz > z1) break;
-        step(C, A, B, V3, V1, zz);
+        step(K2{}, C, A, B, V3, V1, zz);
         if (++zz > z1) break;
     }
 }
