@@ -92,7 +92,7 @@ struct SolverBase {
 //    in 2-plane z chunks against 308 with single sweeps, 128 x 64 x 64 at 587
 //    against 475, the 100 x 50 x 100 stretched case at 12.9 against 18.9 ms
 //    per step), and on large ones they move less.
-//  - Momentum pairs (one 16-wave workgroup per CU, 123 VGPRs) only where the
+//  - Momentum pairs (one 16-wave workgroup per CU, 117 VGPRs) only where the
 //    z chunk has at least 8 planes and the grid fills the CUs: 384 x 128 x 256
 //    and 256 x 96 x 128 (16-plane chunks) run 13 % and 7 % faster with them,
 //    192 x 96 x 64 and 160 x 96 x 64 (4-plane chunks, 1.5 rounds of
