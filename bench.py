@@ -57,10 +57,24 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
             break
     dt = time.perf_counter() - t0
     X, Y, Z = w.grid
+    cf = w.cf
+    C = cf.PISONumCorrectors if int(cf.fvmSolverType) == 1 else 0
+    b_step = bytes_per_cell_step(cf.velMaxNumIter, cf.presMaxNumIter, C, int(cf.turbulenceModel) == 1)
     threads = int(oracle_lib().uwso_num_threads())
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": round(X * Y * Z * n / dt / 1e6, 4), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"{n} PISO steps of {w.name} (V=P=50, C=2), FP32 oracle on {threads} OpenMP threads "
-                      f"({os.cpu_count()} logical CPUs visible)"}
+                      f"({os.cpu_count()} logical CPUs visible)",
+            "ms_per_step": round(dt / n * 1e3, 2), "algorithmic_GBps": round(b_step * X * Y * Z * n / dt / 1e9, 2),
+            "cpu_model": cpu_model}
 
 
 def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
