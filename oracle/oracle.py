@@ -43,7 +43,8 @@ _lib = None
 
 
 def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    # -B: compile on every call, also over an existing library
+    subprocess.run(["make", "-s", "-B", "-C", str(ORACLE_DIR)], check=True)
     return LIB
 
 
