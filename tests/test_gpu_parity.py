@@ -168,7 +168,9 @@ def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters
     bit-identical fields to the one-sweep kernels: counts that do not divide
     end with shorter launches and with norms the last sweep stays single.
     Runs: one-sweep kernels only; pairs only; triples with the grid's z chunk;
-    triples with 5-plane z chunks."""
+    triples with 5-, 2- and 1-plane z chunks, pairs with 1- and 3-plane ones
+    (chunks that start at z = 1 and 2, where the kernels' first plane records
+    reach below the grid)."""
     if name == "tiny":
         w = workloads.tiny(vel_iters=vel_iters, pres_iters=pres_iters)
     else:
@@ -179,10 +181,12 @@ def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters
     w.cf.Init()
     # UWS_SWEEP_FUSION 0: one-sweep kernels only, 2: multi-sweep kernels at any grid size
     runs = [{"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "2"},
-            {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"}]
+            {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "2", "UWS_ZCHUNK": "1"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "1", "UWS_ZCHUNK": "3"}]
     res = []
     for env in runs:
-        for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_ZCHUNK3"):
+        for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_ZCHUNK3", "UWS_ZCHUNK"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
