@@ -79,10 +79,11 @@ def test_low_precision_triples_equal_single_sweeps(gpu, monkeypatch, prec, sweep
     """The three-sweep low-precision kernel (its first two sweeps rounded to 2
     bytes as the stored iterate would be, the last of a round writing p')
     gives the fields and defect norms of single low-precision sweeps bit for
-    bit, with the grid's z chunk and with 5-plane chunks."""
+    bit, with the grid's z chunk and with 5- and 1-plane chunks."""
     w = workloads.tiny(grid=(20, 14, 17))
     w.cf.precision, w.cf.refineRounds, w.cf.refineSweeps, w.cf.computeNorms = prec, 3, sweeps, True
-    runs = [{"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"}]
+    runs = [{"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "1"}]
     res = []
     for env in runs:
         for k in ("UWS_SWEEP_FUSION", "UWS_ZCHUNK3"):
