@@ -70,14 +70,20 @@ def test_ensemble_rejects_mixed_schedules(gpu):
         FvmEnsembleHip([a.cf, b.cf])
 
 
-@pytest.mark.parametrize("group_bytes,vel_iters", [(0, 6), (1, 6), (0, 4), (1, 5)])
-def test_ensemble_two_sweep_kernel_matches_single_cases(gpu, monkeypatch, group_bytes, vel_iters):
+@pytest.mark.parametrize("group_bytes,vel_iters,zchunks", [(0, 6, None), (1, 6, None), (0, 4, None), (1, 5, None),
+                                                             (0, 5, (3, 1))])
+def test_ensemble_two_sweep_kernel_matches_single_cases(gpu, monkeypatch, group_bytes, vel_iters, zchunks):
     """With the multi-sweep kernels forced at every size (momentum pairs,
     pressure pairs and triples), the ensemble's launches over every case's
-    z-marching blocks equal the single cases.  Four momentum sweeps as two
-    pairs end in the other work buffer than four single sweeps would, and five
-    end in a single sweep: the ensemble's graph keeps the roles its step left."""
+    z-marching blocks equal the single cases.  Four momentum sweeps run as
+    two pairs, five as two pairs and a single sweep; either way the step ends
+    in the buffer single sweeps would (the groups of an ensemble share buffer
+    roles).  zchunks: z planes per workgroup of the two- and three-sweep kernels, for
+    every case (chunks that start at z = 1 and 2)."""
     monkeypatch.setenv("UWS_SWEEP_FUSION", "2")
+    if zchunks is not None:
+        monkeypatch.setenv("UWS_ZCHUNK", str(zchunks[0]))
+        monkeypatch.setenv("UWS_ZCHUNK3", str(zchunks[1]))
     ws = small_cases(vel_iters)
     ref = singles(ws, 3)
     got = ensemble_run(ws, 3, 4, False, group_bytes=group_bytes)
