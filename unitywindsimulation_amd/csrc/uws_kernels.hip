@@ -1446,9 +1446,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 // into the output; one barrier per plane.  Own-column values above and below
 // come from registers.  Four plane records (coefficients and the input
 // iterate's neighbours) rotate through named variables (the loop is unrolled
-// four times); the loads of plane zz + 1 are issued first.  Expressions,
-// boundary masking and the handling of non-fluid cells are those of the
-// two-sweep kernel, so the result equals three single sweeps.  A thread
+// four times, the period of the four-slot LDS rings, so LDS addresses are
+// constant offsets); the loads of plane zz + 1 are issued first.  A record's
+// boundary masks are applied once, before its first use.  Expressions and
+// the handling of non-fluid cells are those of the two-sweep kernel, so the
+// result equals three single sweeps.  The barrier also keeps the 16 waves on
+// one plane, so that their neighbours' loads hit the caches.  A thread
 // outside the second sweep's region (the outer halo ring) computes it too, on
 // its own entry, and its value is never used.
 // ---------------------------------------------------------------------------
