@@ -369,13 +369,16 @@ def stretched_report(steps: int = 5, warmup: int = 2) -> dict:
     ms = (time.perf_counter() - t0) / steps * 1e3
     b = bytes_per_cell_step(V, P, 0, les=False)
     t_vel = s.time_kernel(N.KERNEL_VEL_SWEEP, 20)
+    t_vel2 = s.time_kernel(N.KERNEL_VEL_SWEEP2, 20)
     t_pres = s.time_kernel(N.KERNEL_PRES_SWEEP, 20)
+    t_pres3 = s.time_kernel(N.KERNEL_PRES_SWEEP3, 20)
     u = s.velocity(3)
     s.close()
     return {"workload": w.name, "grid_xyz": [X, Y, Z], "cells": n, "vel_sweeps": V, "pres_sweeps": P,
             "steps": steps, "ms_per_step": round(ms, 3), "mlups": round(n / (ms * 1e-3) / 1e6, 2),
             "bytes_per_cell_step": b, "hbm_frac": round(b * n / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
-            "kernel_ms": {"k_vel_sweep (stretched)": round(t_vel, 5), "k_pres_sweep": round(t_pres, 5)},
+            "kernel_ms": {"k_vel_sweep (stretched)": round(t_vel, 5), "k_vel_sweep2 (stretched)": round(t_vel2, 5),
+                          "k_pres_sweep": round(t_pres, 5), "k_pres_sweep3": round(t_pres3, 5)},
             "domain_m": [float(v[-1]) for v in w.face_positions],
             "finite": bool(torch.isfinite(torch.from_numpy(u)).all()),
             "max_abs_u": float(abs(u).max())}

@@ -306,8 +306,8 @@ int uws_get_norms(uws_handle h, double* out, int32_t n);
  * UWS_KERNEL_VEL_SWEEP reads u^n and writes a scratch velocity buffer;
  * UWS_KERNEL_PRES_SWEEP reads and writes the p' scratch pair;
  * UWS_KERNEL_PRES_SWEEP2 is the two-sweep pressure kernel the step uses;
- * UWS_KERNEL_VEL_SWEEP2 the two-sweep momentum kernel (FP32, uniform grid;
- * UWS_ERR_UNSUPPORTED elsewhere), from u^n into a scratch buffer;
+ * UWS_KERNEL_VEL_SWEEP2 the two-sweep momentum kernel (FP32, uniform or
+ * stretched grid; UWS_ERR_UNSUPPORTED elsewhere), from u^n into a scratch buffer;
  * UWS_KERNEL_PRES_SWEEP3 the three-sweep pressure kernel (FP32).  The solver
  * state (u^n, p^n) is not changed.  Call after at least one uws_step. */
 enum { UWS_KERNEL_VEL_SWEEP = 0, UWS_KERNEL_PRES_SWEEP = 1, UWS_KERNEL_PRES_SWEEP2 = 2, UWS_KERNEL_VEL_SWEEP2 = 3,

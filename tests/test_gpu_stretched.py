@@ -114,10 +114,11 @@ def _run(w, steps=3, env=None, monkeypatch=None):
 
 
 def test_stretched_two_sweep_graph_and_norm_pass_bitwise(gpu, monkeypatch):
-    """The two-sweep pressure kernel, eager launches and the divergence-norm
-    pass (which re-runs the pressure precompute on u^{n+1} without touching
-    the stored face velocities) all leave the same fields as the plain
-    graph-replayed step."""
+    """The multi-sweep kernels (the momentum pair with the stretched grid's
+    cell-size factors, the pressure pairs and triples), eager launches and the
+    divergence-norm pass (which re-runs the pressure precompute on u^{n+1}
+    without touching the stored face velocities) all leave the same fields as
+    the plain graph-replayed step of one-sweep kernels."""
     w = workloads.tiny_stretched(pres_iters=9)
     ref = _run(w, env={"UWS_SWEEP_FUSION": "0"}, monkeypatch=monkeypatch)
     for env in ({"UWS_SWEEP_FUSION": "2"}, {"UWS_NO_GRAPH": "1"}):

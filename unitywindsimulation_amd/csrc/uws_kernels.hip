@@ -1721,6 +1721,7 @@ template <class T>
 struct VS2Args {
     int X, Y, Z, zchunk;
     T q;  // ds / 4
+    const T* h[3];  // stretched grid: the cell sizes that replace q per direction (nullptr: uniform)
     const T* S[3];
     const T* G[3];
     const T* invD[3];
@@ -1735,6 +1736,7 @@ __device__ __forceinline__ VS2Args<T> vs2_args(const ParamsT<T>& p, const Fields
     a.X = p.X; a.Y = p.Y; a.Z = p.Z; a.zchunk = p.zchunk;
     a.q = p.ds * (T)0.25f;
     for (int c = 0; c < 3; ++c) {
+        a.h[c] = p.nonuniform ? f.h[c] : nullptr;
         a.S[c] = f.S[c]; a.G[c] = f.G[c]; a.invD[c] = f.invD[c]; a.b[c] = f.b[c];
         a.vin[c] = f.vel[r.vin][c];
         a.out[c] = f.vel[r.vout][c];
@@ -1754,7 +1756,10 @@ __device__ __forceinline__ double keep_or_zero(double v, bool keep) {
 // components each
 constexpr int VS2_LDS_FLOATS = (3 + 3) * 3 * PS2_THREADS;
 
-template <class T>
+// NU: the stretched grid, whose sweep (body_vel_sweep<T, true>) weighs the
+// convective terms of the x, y and z faces by the cell sizes h[0][x],
+// h[1][y], h[2][z] instead of ds / 4
+template <class T, bool NU = false>
 __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* lds) {
     const int tid = (int)threadIdx.x;
     const int lx = tid % PS2_W, ly = tid / PS2_W;
@@ -1782,6 +1787,10 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
     T* const lx1 = lds;                      // [3 slots][3 components][PS2_THREADS]
     T* const lvi = lds + 9 * PS2_THREADS;    // [3 slots][3 components][PS2_THREADS]
     const T q = p.q;
+    // per-direction factors of the convective terms (the plane's z factor is
+    // uniform and read per plane)
+    const T qx = NU ? p.h[0][xc] : q, qy = NU ? p.h[1][yc] : q;
+    auto qz_of = [&](int zz) { return NU ? p.h[2][min(max(zz, 0), p.Z - 1)] : q; };
     // addresses: uniform base + 32-bit byte offset (column offset plus a
     // uniform plane offset); the W and S coefficients are read at the clamped
     // W/S neighbour (the cell itself on the boundary, masked to +0)
@@ -1841,6 +1850,7 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
         maskc(cu, zz);
         const T SW = cu.SW, GW = cu.GW, SS = cu.SS, GS = cu.GS;
         const T SD = pv.SU, GD = pv.GU;
+        const T qz1 = qz_of(zz);
         const T* Lc = lvi + S0 * (3 * PS2_THREADS);
         const T* Ld = lvi + Sm * (3 * PS2_THREADS);
         T x1[3];
@@ -1852,12 +1862,12 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
             const T vN = eN ? cu.vO[a] : lN, vS = eS ? cu.vO[a] : lS;
             const T vU = vu[a], vD = Ld[a * PS2_THREADS + tid];
             T A = 0;
-            A -= q * vE * cu.SE;
-            A += q * vW * SW;
-            A -= q * vN * cu.SN;
-            A += q * vS * SS;
-            A -= q * vU * cu.SU;
-            A += q * vD * SD;
+            A -= qx * vE * cu.SE;
+            A += qx * vW * SW;
+            A -= qy * vN * cu.SN;
+            A += qy * vS * SS;
+            A -= qz1 * vU * cu.SU;
+            A += qz1 * vD * SD;
             T Df = 0;
             Df += cu.GE * vE;
             Df += GW * vW;
@@ -1881,18 +1891,19 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
             const uint32_t oP = planeoff(zo);
             const T SW2 = pv.SW, GW2 = pv.GW, SS2 = pv.SS, GS2 = pv.GS;
             const T SD2 = pdS, GD2 = pdG;
+            const T qz2 = qz_of(zo);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const T* L = lx1 + Sm * (3 * PS2_THREADS) + a * PS2_THREADS;
                 const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
                 const T vU = zo + 1 < p.Z ? x1[a] : x1m1[a], vD2 = zo > 0 ? x1m2[a] : x1m1[a];
                 T A = 0;
-                A -= q * vE * pv.SE;
-                A += q * vW * SW2;
-                A -= q * vN * pv.SN;
-                A += q * vS * SS2;
-                A -= q * vU * pv.SU;
-                A += q * vD2 * SD2;
+                A -= qx * vE * pv.SE;
+                A += qx * vW * SW2;
+                A -= qy * vN * pv.SN;
+                A += qy * vS * SS2;
+                A -= qz2 * vU * pv.SU;
+                A += qz2 * vD2 * SD2;
                 T Df = 0;
                 Df += pv.GE * vE;
                 Df += GW2 * vW;
@@ -1944,10 +1955,11 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
     }
 }
 
+template <bool NU>
 __global__ __launch_bounds__(1024) void k_vel_sweep2_f32(ParamsT<float> p, FieldsT<float> f, Roles r) {
     __shared__ __align__(16) float lds_v[VS2_LDS_FLOATS];
     const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
-    vel_sweep2_block<float>(vs2_args(p, f, r), xcd_tile((int)blockIdx.x, nb), lds_v);
+    vel_sweep2_block<float, NU>(vs2_args(p, f, r), xcd_tile((int)blockIdx.x, nb), lds_v);
 }
 
 // the case owning tile t: the last c with first_tile[c] <= t (binary search)
@@ -2165,12 +2177,11 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
         return;
     }
     if (k == KN_VEL_SWEEP2) {
-        // FP32 on the uniform grid only (the step driver asks for it there)
+        // FP32 only (the step driver asks for it there), uniform or stretched grid
         if constexpr (std::is_same_v<T, float>) {
-            if (!p.nonuniform) {
-                const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
-                hipLaunchKernelGGL(k_vel_sweep2_f32, dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
-            }
+            const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+            if (p.nonuniform) hipLaunchKernelGGL(k_vel_sweep2_f32<true>, dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+            else hipLaunchKernelGGL(k_vel_sweep2_f32<false>, dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
         }
         return;
     }

@@ -65,9 +65,9 @@ struct Solver : SolverBase {
         }
         double ncells() const { return (double)s->p.N; }
         bool fused() const { return use_fusion(s->fuse_mode); }
-        // the two-sweep momentum kernel: FP32 on the uniform grid
+        // the two-sweep momentum kernel: FP32 (uniform or stretched grid)
         bool fused_vel() const {
-            return std::is_same<T, float>::value && !s->p.nonuniform &&
+            return std::is_same<T, float>::value &&
                    use_vel_pairs(s->fuse_mode, pres_sweep2_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk), s->p.zchunk);
         }
         // the three-sweep pressure kernel: FP32
@@ -480,7 +480,7 @@ struct Solver : SolverBase {
         HIP_OK(hipEventCreate(&e0));
         if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return UWS_ERR_DEVICE; }
         const int w0 = (rs.state + 1) % 3;
-        if ((kernel == UWS_KERNEL_VEL_SWEEP2 && !(std::is_same<T, float>::value && !p.nonuniform)) ||
+        if ((kernel == UWS_KERNEL_VEL_SWEEP2 && !std::is_same<T, float>::value) ||
             (kernel == UWS_KERNEL_PRES_SWEEP3 && !std::is_same<T, float>::value)) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
