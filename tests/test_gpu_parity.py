@@ -170,7 +170,8 @@ def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters
     Runs: one-sweep kernels only; pairs only; triples with the grid's z chunk;
     triples with 5-, 2- and 1-plane z chunks, pairs with 1- and 3-plane ones
     (chunks that start at z = 1 and 2, where the kernels' first plane records
-    reach below the grid)."""
+    reach below the grid); pairs in tiles of 15 and 5 rows (the grid's own
+    choice is 14 or 15)."""
     if name == "tiny":
         w = workloads.tiny(vel_iters=vel_iters, pres_iters=pres_iters)
     else:
@@ -183,10 +184,11 @@ def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters
     runs = [{"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "2"},
             {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"},
             {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "2", "UWS_ZCHUNK": "1"},
-            {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "1", "UWS_ZCHUNK": "3"}]
+            {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "1", "UWS_ZCHUNK": "3"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_TILE_ROWS2": "15"}, {"UWS_SWEEP_FUSION": "2", "UWS_TILE_ROWS2": "5"}]
     res = []
     for env in runs:
-        for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_ZCHUNK3", "UWS_ZCHUNK"):
+        for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_ZCHUNK3", "UWS_ZCHUNK", "UWS_TILE_ROWS2"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)

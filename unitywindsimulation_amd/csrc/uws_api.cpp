@@ -85,10 +85,21 @@ StepCfg step_cfg(const uws_config& c) {
 // ragged last chunk.  Measured on 256 x 96 x 128 (k_pres_sweep2): zchunk 16
 // (256 workgroups) 22.8 us, 8 (512) 24.2 us, 12 (352: a second, partial
 // round of workgroups) and 24 (192: idle CUs) 29.5-30 us.
-int choose_zchunk(int X, int Y, int Z) {
+// tile rows of the two-sweep kernels: 15 (13 output rows) or 14 (12),
+// whichever computes fewer thread rows over the grid's Y (ties: 15).  Every
+// workgroup marches the same planes, so a tile row that falls outside the
+// grid costs as much as one inside: 256 x 96 x 128 runs its momentum and
+// pressure pairs in 8 tiles of 12 rows (112 thread rows) rather than 8 of 13
+// (120), 2 % faster a step.
+int choose_by2(int Y) {
+    auto rows = [&](int by) { return ((Y + by - 3) / (by - 2)) * by; };
+    return rows(14) < rows(15) ? 14 : 15;
+}
+
+int choose_zchunk(int X, int Y, int Z, int by) {
     int best = 8, best_score = -1;
     for (int zc : {16, 8, 4}) {
-        const int blocks = pres_sweep2_grid(X, Y, Z, zc);
+        const int blocks = pres_sweep2_grid(X, Y, Z, zc, by);
         if (blocks < FUSE_MIN_BLOCKS) continue;
         const int score = (Z % zc == 0 ? 2 : 0) + (zc == 16 ? 1 : 0);
         if (score > best_score) { best = zc; best_score = score; }
@@ -140,7 +151,12 @@ void fill_params(const uws_config& c, ParamsT<T>& p) {
     // (FvmSolverGpu.cs:1425-1436); its cell (2, Y/2, 2) outside the grid reads 0
     p.normRef = (!anyFixed && 2 < p.X && 2 < p.Z) ? (2 * p.Y + p.Y / 2) * p.X + 2 : -1;
     // z planes per workgroup of the two-sweep kernels (UWS_ZCHUNK overrides, for tuning)
-    p.zchunk = choose_zchunk(p.X, p.Y, p.Z);
+    p.by2 = choose_by2(p.Y);
+    if (const char* e = std::getenv("UWS_TILE_ROWS2")) {
+        const int v = std::atoi(e);
+        if (v >= 3 && v <= 15) p.by2 = v;
+    }
+    p.zchunk = choose_zchunk(p.X, p.Y, p.Z, p.by2);
     if (const char* e = std::getenv("UWS_ZCHUNK")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 1024) p.zchunk = v;

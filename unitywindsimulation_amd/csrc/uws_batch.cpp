@@ -82,6 +82,8 @@ struct uws_batch {
             ParamsT<float>& p = params(c);
             p.ty = ty;
             p.tiles = tiles_of(p.X, p.Y, p.Z, ty);
+            // one launch serves every case: the full two-sweep tile height
+            p.by2 = 15;
         }
         int64_t acc = 0;
         for (int c = 0; c < n; ++c) {
@@ -106,7 +108,7 @@ struct uws_batch {
                     bool divides = true;
                     for (int c : g.cases) {
                         const ParamsT<float>& p = params(c);
-                        blocks += pres_sweep2_grid(p.X, p.Y, p.Z, zc);
+                        blocks += pres_sweep2_grid(p.X, p.Y, p.Z, zc, p.by2);
                         divides = divides && p.Z % zc == 0;
                     }
                     if (blocks < FUSE_MIN_BLOCKS) continue;
@@ -138,7 +140,7 @@ struct uws_batch {
                 ps[i] = params(g.cases[i]);
                 fs[i] = fields(g.cases[i]);
                 first[i + 1] = first[i] + ps[i].tiles;
-                first2[i + 1] = first2[i] + pres_sweep2_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk);
+                first2[i + 1] = first2[i] + pres_sweep2_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk, ps[i].by2);
                 first3[i + 1] = first3[i] + pres_sweep3_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk3);
             }
             if (hipMalloc(&g.d_first, sizeof(int) * (m + 1)) != hipSuccess ||

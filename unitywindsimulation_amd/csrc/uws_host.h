@@ -136,7 +136,8 @@ int init_vel_pres(SolverBase& s, int32_t inflow_type, float wind_speed);
 int flags_from_int(SolverBase& s, const int32_t* flags, int64_t n);
 int normalise_norms(const double* raw, double ncells, double* out, int n);
 int stretched_face_positions(int n, float dx, int uniform, int one_side, float ratio, float* out);
-int choose_zchunk(int X, int Y, int Z);
+int choose_by2(int Y);
+int choose_zchunk(int X, int Y, int Z, int by);
 int choose_zchunk3(int X, int Y, int Z);
 
 // ---------------------------------------------------------------------------

@@ -28,6 +28,7 @@ struct ParamsT {
     T wallZ0;
     int normRef;          // flat index of the pressure normalisation cell, -1 when it is inactive
     int zchunk;           // z planes per workgroup of the two-sweep kernels
+    int by2;              // tile rows of the two-sweep kernels (outputs + 2 halo rows, at most 15)
     int zchunk3;          // z planes per workgroup of the three-sweep pressure kernel
     int nonuniform;       // 1: stretched grid (GridType.CollNonUniform), spacings from FieldsT::h
     // p2 = 1 (float, double) where dx and den are powers of two: the cell
@@ -163,7 +164,7 @@ template <class T>
 void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s);
 
 // workgroups of the two-sweep pressure kernel for one grid
-int pres_sweep2_grid(int X, int Y, int Z, int zchunk);
+int pres_sweep2_grid(int X, int Y, int Z, int zchunk, int by);
 int pres_sweep3_grid(int X, int Y, int Z, int zchunk);
 
 inline int tiles_of(int X, int Y, int Z, int ty) { return ((X + 63) / 64) * ((Y + ty - 1) / ty) * Z; }

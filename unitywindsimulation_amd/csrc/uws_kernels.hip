@@ -1206,8 +1206,8 @@ __device__ __forceinline__ double block_sum(double v) {
 // two single sweeps bit for bit.
 // ---------------------------------------------------------------------------
 constexpr int PS2_W = 66;                   // threads per tile row: 64 outputs + 2 halo
-constexpr int PS2_BY = 15;                  // tile rows: 13 outputs + 2 halo
-constexpr int PS2_THREADS = PS2_W * PS2_BY;  // 990
+constexpr int PS2_BY = 15;                  // most tile rows (13 outputs + 2 halo); a grid uses by2 <= PS2_BY
+constexpr int PS2_THREADS = PS2_W * PS2_BY;  // 990: the LDS stride of a plane
 
 // one plane's inputs of the first sweep at a cell: its coefficients and the
 // previous iterate at its six neighbours (clamped at the domain boundary,
@@ -1219,8 +1219,9 @@ struct PS2Plane {
     T vE, vW, vN, vS, vU;
 };
 
-inline __host__ __device__ int pres_sweep2_blocks(int X, int Y, int Z, int zchunk) {
-    return ((X + 63) / 64) * ((Y + PS2_BY - 3) / (PS2_BY - 2)) * ((Z + zchunk - 1) / zchunk);
+// by: tile rows (outputs + 2 halo rows) of the grid's two-sweep launches
+inline __host__ __device__ int pres_sweep2_blocks(int X, int Y, int Z, int zchunk, int by) {
+    return ((X + 63) / 64) * ((Y + by - 3) / (by - 2)) * ((Z + zchunk - 1) / zchunk);
 }
 
 // the two-sweep block b of one grid (b in [0, pres_sweep2_blocks))
@@ -1252,7 +1253,7 @@ __device__ __forceinline__ void st_at(T* base, uint32_t off, T v) {
 }
 template <class T>
 struct PS2Args {
-    int X, Y, Z, zchunk;
+    int X, Y, Z, zchunk, by;
     PresCoef<T> k;
     const T* x0;
     T* out;
@@ -1265,7 +1266,7 @@ struct PS2Args {
 template <class T>
 __device__ __forceinline__ PS2Args<T> ps2_args(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r) {
     PS2Args<T> a;
-    a.X = p.X; a.Y = p.Y; a.Z = p.Z; a.zchunk = p.zchunk;
+    a.X = p.X; a.Y = p.Y; a.Z = p.Z; a.zchunk = p.zchunk; a.by = p.by2;
     a.k = pres_coef(f, r.coef);
     a.x0 = r.pin >= 0 ? f.pc[r.pin] : nullptr;
     a.out = f.pc[r.pout];
@@ -1280,14 +1281,14 @@ template <class T, bool ZIN, bool OFF = (sizeof(T) <= 8)>
 __device__ __forceinline__ void pres_sweep2_block(const PS2Args<T>& p, int b, T* lds) {
     const int tid = (int)threadIdx.x;
     const int lx = tid % PS2_W, ly = tid / PS2_W;
-    const int ntx = (p.X + 63) / 64, nty = (p.Y + PS2_BY - 3) / (PS2_BY - 2);
+    const int ntx = (p.X + 63) / 64, nty = (p.Y + p.by - 3) / (p.by - 2);
     const int bx = b % ntx;
     b /= ntx;
     const int by = b % nty, bz = b / nty;
-    const int x = bx * 64 - 1 + lx, y = by * (PS2_BY - 2) - 1 + ly;
+    const int x = bx * 64 - 1 + lx, y = by * (p.by - 2) - 1 + ly;
     const int z0 = bz * p.zchunk, z1 = min(z0 + p.zchunk, p.Z);
     const bool cell = x >= 0 && x < p.X && y >= 0 && y < p.Y;
-    const bool inner = cell && lx >= 1 && lx <= 64 && ly >= 1 && ly <= PS2_BY - 2;
+    const bool inner = cell && lx >= 1 && lx <= 64 && ly >= 1 && ly <= p.by - 2;
     const PresCoef<T> k = p.k;
     const T* __restrict__ x0 = ZIN ? nullptr : p.x0;
     T* __restrict__ out = p.out;
@@ -1420,7 +1421,7 @@ __device__ __forceinline__ void pres_sweep2_block(const PS2Args<T>& p, int b, T*
 template <class T, bool ZIN>
 __global__ __launch_bounds__(1024) void k_pres_sweep2(ParamsT<T> p, FieldsT<T> f, Roles r) {
     __shared__ __align__(16) unsigned char lds_raw[3 * PS2_THREADS * sizeof(T)];
-    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk, p.by2);
     pres_sweep2_block<T, ZIN>(ps2_args(p, f, r), xcd_tile((int)blockIdx.x, nb), reinterpret_cast<T*>(lds_raw));
 }
 // FP32: held to 64 VGPRs, so that 8 waves run per SIMD and two 16-wave
@@ -1430,7 +1431,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
                                                                                                   FieldsT<float> f,
                                                                                                   Roles r) {
     __shared__ __align__(16) float lds_f[3 * PS2_THREADS];
-    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk, p.by2);
     pres_sweep2_block<float, ZIN>(ps2_args(p, f, r), xcd_tile((int)blockIdx.x, nb), lds_f);
 }
 
@@ -1719,7 +1720,7 @@ struct VS2Coef {
 };
 template <class T>
 struct VS2Args {
-    int X, Y, Z, zchunk;
+    int X, Y, Z, zchunk, by;
     T q;  // ds / 4
     const T* h[3];  // stretched grid: the cell sizes that replace q per direction (nullptr: uniform)
     const T* S[3];
@@ -1733,7 +1734,7 @@ struct VS2Args {
 template <class T>
 __device__ __forceinline__ VS2Args<T> vs2_args(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r) {
     VS2Args<T> a;
-    a.X = p.X; a.Y = p.Y; a.Z = p.Z; a.zchunk = p.zchunk;
+    a.X = p.X; a.Y = p.Y; a.Z = p.Z; a.zchunk = p.zchunk; a.by = p.by2;
     a.q = p.ds * (T)0.25f;
     for (int c = 0; c < 3; ++c) {
         a.h[c] = p.nonuniform ? f.h[c] : nullptr;
@@ -1763,20 +1764,20 @@ template <class T, bool NU = false>
 __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* lds) {
     const int tid = (int)threadIdx.x;
     const int lx = tid % PS2_W, ly = tid / PS2_W;
-    const int ntx = (p.X + 63) / 64, nty = (p.Y + PS2_BY - 3) / (PS2_BY - 2);
+    const int ntx = (p.X + 63) / 64, nty = (p.Y + p.by - 3) / (p.by - 2);
     const int bx = b % ntx;
     b /= ntx;
     const int by = b % nty, bz = b / nty;
-    const int x = bx * 64 - 1 + lx, y = by * (PS2_BY - 2) - 1 + ly;
+    const int x = bx * 64 - 1 + lx, y = by * (p.by - 2) - 1 + ly;
     const int z0 = bz * p.zchunk, z1 = min(z0 + p.zchunk, p.Z);
     const bool cell = x >= 0 && x < p.X && y >= 0 && y < p.Y;
-    const bool inner = cell && lx >= 1 && lx <= 64 && ly >= 1 && ly <= PS2_BY - 2;
+    const bool inner = cell && lx >= 1 && lx <= 64 && ly >= 1 && ly <= p.by - 2;
     const int sy = p.X;
     const bool hW = x > 0, hE = x + 1 < p.X, hS = y > 0, hN = y + 1 < p.Y;
     const int xc = min(max(x, 0), p.X - 1), yc = min(max(y, 0), p.Y - 1);
     const bool gE = xc + 1 < p.X, gW = xc > 0, gN = yc + 1 < p.Y, gS = yc > 0;
     // tile edges: the neighbour outside the tile comes from the thread's own load
-    const bool eW = lx == 0, eE = lx == PS2_W - 1, eS = ly == 0, eN = ly == PS2_BY - 1;
+    const bool eW = lx == 0, eE = lx == PS2_W - 1, eS = ly == 0, eN = ly == p.by - 1;
     const bool edge = eW || eE || eS || eN;
     const int oO = eW ? (gW ? -1 : 0) : eE ? (gE ? 1 : 0) : eS ? (gS ? -sy : 0) : eN ? (gN ? sy : 0) : 0;
     // LDS indices of the in-tile neighbours (the thread itself on a tile edge)
@@ -1958,7 +1959,7 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
 template <bool NU>
 __global__ __launch_bounds__(1024) void k_vel_sweep2_f32(ParamsT<float> p, FieldsT<float> f, Roles r) {
     __shared__ __align__(16) float lds_v[VS2_LDS_FLOATS];
-    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+    const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk, p.by2);
     vel_sweep2_block<float, NU>(vs2_args(p, f, r), xcd_tile((int)blockIdx.x, nb), lds_v);
 }
 
@@ -2179,20 +2180,21 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
     if (k == KN_VEL_SWEEP2) {
         // FP32 only (the step driver asks for it there), uniform or stretched grid
         if constexpr (std::is_same_v<T, float>) {
-            const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
-            if (p.nonuniform) hipLaunchKernelGGL(k_vel_sweep2_f32<true>, dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
-            else hipLaunchKernelGGL(k_vel_sweep2_f32<false>, dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+            const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk, p.by2);
+            const dim3 tb(PS2_W * p.by2);
+            if (p.nonuniform) hipLaunchKernelGGL(k_vel_sweep2_f32<true>, dim3(nb), tb, 0, s, p, f, r);
+            else hipLaunchKernelGGL(k_vel_sweep2_f32<false>, dim3(nb), tb, 0, s, p, f, r);
         }
         return;
     }
     if (k == KN_PRES_SWEEP2) {
-        const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk);
+        const int nb = pres_sweep2_blocks(p.X, p.Y, p.Z, p.zchunk, p.by2);
         if constexpr (std::is_same_v<T, float>) {
-            if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep2_f32<true>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
-            else hipLaunchKernelGGL((k_pres_sweep2_f32<false>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+            if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep2_f32<true>), dim3(nb), dim3(PS2_W * p.by2), 0, s, p, f, r);
+            else hipLaunchKernelGGL((k_pres_sweep2_f32<false>), dim3(nb), dim3(PS2_W * p.by2), 0, s, p, f, r);
         } else {
-            if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep2<T, true>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
-            else hipLaunchKernelGGL((k_pres_sweep2<T, false>), dim3(nb), dim3(PS2_THREADS), 0, s, p, f, r);
+            if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep2<T, true>), dim3(nb), dim3(PS2_W * p.by2), 0, s, p, f, r);
+            else hipLaunchKernelGGL((k_pres_sweep2<T, false>), dim3(nb), dim3(PS2_W * p.by2), 0, s, p, f, r);
         }
         return;
     }
@@ -2267,7 +2269,7 @@ __global__ __launch_bounds__(256) void k_mean_accumulate(const T* u0, const T* u
 }
 }  // namespace
 
-int pres_sweep2_grid(int X, int Y, int Z, int zchunk) { return pres_sweep2_blocks(X, Y, Z, zchunk); }
+int pres_sweep2_grid(int X, int Y, int Z, int zchunk, int by) { return pres_sweep2_blocks(X, Y, Z, zchunk, by); }
 int pres_sweep3_grid(int X, int Y, int Z, int zchunk) { return pres_sweep3_blocks(X, Y, Z, zchunk); }
 
 // ---------------------------------------------------------------------------

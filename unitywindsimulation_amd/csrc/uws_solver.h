@@ -68,7 +68,7 @@ struct Solver : SolverBase {
         // the two-sweep momentum kernel: FP32 (uniform or stretched grid)
         bool fused_vel() const {
             return std::is_same<T, float>::value &&
-                   use_vel_pairs(s->fuse_mode, pres_sweep2_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk), s->p.zchunk);
+                   use_vel_pairs(s->fuse_mode, pres_sweep2_grid(s->p.X, s->p.Y, s->p.Z, s->p.zchunk, s->p.by2), s->p.zchunk);
         }
         // the three-sweep pressure kernel: FP32
         bool fused3() const { return std::is_same<T, float>::value && use_fusion3(s->fuse_mode); }
