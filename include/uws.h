@@ -43,7 +43,9 @@ enum {
     UWS_ERR_ALLOC = -3,         /* device allocation failed */
     UWS_ERR_DEVICE = -4,        /* HIP runtime error / no device */
     UWS_ERR_UNSUPPORTED = -5,   /* combination the reference also rejects */
-    UWS_ERR_NONFINITE = -6      /* NaN/Inf detected by a norm check */
+    UWS_ERR_NONFINITE = -6,     /* NaN/Inf detected by a norm check */
+    UWS_ERR_TOO_LARGE = -7      /* input larger than the fixed staging array of an
+                                   allocation-free entry point */
 };
 
 /* ---- enums: identical integer values to the reference ---- */
@@ -211,6 +213,22 @@ typedef struct uws_background {
 int uws_init_vel_pres_fields_from_background(uws_handle h, const float phys_field_pos[3], float field_rot_deg,
                                              const uws_background* bg, const float* bg_vel, int32_t ncomp,
                                              const float* bg_pres);
+
+/* Allocation-free variant of uws_init_vel_pres_fields_from_background.  No
+ * reference counterpart: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and the same fields, bit for bit; instead of a
+ * device buffer allocated per call, the caller's arrays are staged as they
+ * are in a fixed device array of UWS_BG_STAGING_MAX_BYTES per GPU, held until
+ * the process ends.  Needs cells * (ncomp + (bg_pres ? 1 : 0)) * 4 bytes
+ * <= UWS_BG_STAGING_MAX_BYTES (at most 1,677,721 background cells with
+ * ncomp 4 and a pressure), else UWS_ERR_TOO_LARGE.  The call synchronises
+ * the handle's stream before it returns.  Every handle on one GPU shares the
+ * array: calls that overlap in time on one GPU (from several threads) race. */
+#define UWS_BG_STAGING_MAX_BYTES (32u << 20)
+int uws_init_vel_pres_fields_from_background_noalloc(uws_handle h, const float phys_field_pos[3],
+                                                     float field_rot_deg, const uws_background* bg,
+                                                     const float* bg_vel, int32_t ncomp, const float* bg_pres);
 
 /* Upload one boundary texture (face UWS_X0..UWS_ZN) as [j][i][3] float where
  * (i, j) are the reference texture coordinates: X faces (y, z), Y faces (x, z),

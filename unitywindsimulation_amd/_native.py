@@ -16,6 +16,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libuws_hip.so"
 # ---- constants mirrored from uws.h ----
 UWS_OK = 0
 UWS_ERR_INVALID_ARG, UWS_ERR_CONFIG, UWS_ERR_ALLOC, UWS_ERR_DEVICE, UWS_ERR_UNSUPPORTED, UWS_ERR_NONFINITE = -1, -2, -3, -4, -5, -6
+UWS_ERR_TOO_LARGE = -7
+BG_STAGING_MAX_BYTES = 32 << 20
 VEL_FIXED_VALUE, VEL_ZERO_GRAD, VEL_SYMMETRY = 0, 1, 2
 PRES_FIXED_VALUE, PRES_ZERO_GRAD, PRES_SYMMETRY = 0, 1, 2
 SIMPLE, PISO = 0, 1
@@ -139,6 +141,8 @@ SIGNATURES = {
     "uws_batch_autotune": (c_int, [c_void_p, c_int32, c_void_p, POINTER(c_double)]),
     "uws_init_vel_pres_fields_from_background": (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_int32,
                                                          c_void_p]),
+    "uws_init_vel_pres_fields_from_background_noalloc": (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_void_p,
+                                                                 c_int32, c_void_p]),
     "uws_mean_reset": (c_int, [c_void_p]),
     "uws_mean_accumulate": (c_int, [c_void_p]),
     "uws_get_mean_velocity": (c_int, [c_void_p, c_void_p, c_int64, POINTER(c_int64)]),

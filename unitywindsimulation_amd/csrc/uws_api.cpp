@@ -302,6 +302,8 @@ int create_solver(const uws_config& cfg, bool allow_residual_break, hipStream_t 
     s->cfg = cfg;
     s->sc = step_cfg(cfg);
     if (hipGetDevice(&s->device) != hipSuccess) return UWS_ERR_DEVICE;
+    // the staging array of the allocation-free warm start: its address on this device
+    if (int rc = bg_staging_setup()) return rc;
     if (shared_stream) {
         s->stream = shared_stream;
         s->owns_stream = false;
@@ -516,6 +518,7 @@ const char* uws_strerror(int status) {
     case UWS_ERR_DEVICE: return "HIP runtime error";
     case UWS_ERR_UNSUPPORTED: return "unsupported combination";
     case UWS_ERR_NONFINITE: return "non-finite value detected";
+    case UWS_ERR_TOO_LARGE: return "input exceeds the fixed staging array of an allocation-free entry point";
     default: return "unknown status";
     }
 }
@@ -675,6 +678,23 @@ int uws_init_vel_pres_fields_from_background(uws_handle h, const float phys_fiel
         return UWS_ERR_INVALID_ARG;
     return with_solver(h, [&](SolverBase& s) {
         return s.init_from_background(phys_field_pos, field_rot_deg, *bg, bg_vel, ncomp, bg_pres);
+    });
+}
+
+int uws_init_vel_pres_fields_from_background_noalloc(uws_handle h, const float phys_field_pos[3],
+                                                     float field_rot_deg, const uws_background* bg,
+                                                     const float* bg_vel, int32_t ncomp, const float* bg_pres) {
+    if (!phys_field_pos || !bg || !bg_vel || (ncomp != 3 && ncomp != 4) || !std::isfinite(field_rot_deg))
+        return UWS_ERR_INVALID_ARG;
+    for (int i = 0; i < 3; ++i)
+        if (bg->grid_res[i] < 1 || !std::isfinite(bg->phys_field_pos[i]) || !std::isfinite(phys_field_pos[i]))
+            return UWS_ERR_INVALID_ARG;
+    if (!(bg->dx > 0.f)) return UWS_ERR_INVALID_ARG;
+    const long long nb = (long long)bg->grid_res[0] * bg->grid_res[1] * bg->grid_res[2];
+    if (nb * (ncomp + (bg_pres ? 1 : 0)) * (long long)sizeof(float) > (long long)UWS_BG_STAGING_MAX_BYTES)
+        return UWS_ERR_TOO_LARGE;
+    return with_solver(h, [&](SolverBase& s) {
+        return s.init_from_background_noalloc(phys_field_pos, field_rot_deg, *bg, bg_vel, ncomp, bg_pres);
     });
 }
 

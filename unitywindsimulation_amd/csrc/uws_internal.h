@@ -145,19 +145,31 @@ template <class T>
 void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot = 0);
 
 // A background flow field resident on the device, for the warm start of
-// InitVelPresFieldsFromBackGroundFlow (geometry in R, values in T).
-template <class T, class R>
+// InitVelPresFieldsFromBackGroundFlow (geometry in R, values in T).  The
+// stored elements are of type S (T itself, or the caller's float arrays
+// staged as they are), converted to T where they are read; velocity
+// component a of background cell i is vel[a][i * vstride] (vstride 1 for
+// planes, ncomp for the caller's [cell][ncomp] layout).
+template <class T, class R, class S = T>
 struct BgField {
     int X, Y, Z;
     R dx;
     R pos[3];        // physFieldPos of the background grid (its centre)
     R fgPos[3];      // physFieldPos of this grid
     R angle;         // flow-field orientation [rad]
-    const T* vel[3];
-    const T* pres;   // may be null (zero)
+    const S* vel[3];
+    const S* pres;   // may be null (zero)
+    int vstride;
 };
-template <class T, class R>
-void init_from_background(const ParamsT<T>& p, const FieldsT<T>& f, const BgField<T, R>& bg, hipStream_t s);
+template <class T, class R, class S>
+void init_from_background(const ParamsT<T>& p, const FieldsT<T>& f, const BgField<T, R, S>& bg, hipStream_t s);
+
+// Fixed staging array of the allocation-free warm start, one per GPU (a
+// file-scope device array of UWS_BG_STAGING_MAX_BYTES).  bg_staging_setup()
+// resolves its address on the current device once (hipGetSymbolAddress) and
+// keeps it; bg_staging(dev) returns the kept address, nullptr before setup.
+int bg_staging_setup();
+void* bg_staging(int device);
 
 // sum[c][i] += u[c][i] in FP64 (time averaging of the velocity)
 template <class T>

@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "uws_internal.h"
@@ -2284,8 +2285,8 @@ int pres_sweep3_grid(int X, int Y, int Z, int zchunk) { return pres_sweep3_block
 // flags are not looked at.
 // ---------------------------------------------------------------------------
 namespace {
-template <class T, class R>
-__device__ __forceinline__ T sample_trilinear(const T* a, int X, int Y, int Z, R u, R v, R w) {
+template <class T, class R, class S>
+__device__ __forceinline__ T sample_trilinear(const S* a, int stride, int X, int Y, int Z, R u, R v, R w) {
     const R tx = u * (R)X - (R)0.5f, ty = v * (R)Y - (R)0.5f, tz = w * (R)Z - (R)0.5f;
     const R fx0 = floor(tx), fy0 = floor(ty), fz0 = floor(tz);
     const R ax = tx - fx0, ay = ty - fy0, az = tz - fz0;
@@ -2293,7 +2294,7 @@ __device__ __forceinline__ T sample_trilinear(const T* a, int X, int Y, int Z, R
     const int x0 = min(max(ix, 0), X - 1), x1 = min(max(ix + 1, 0), X - 1);
     const int y0 = min(max(iy, 0), Y - 1), y1 = min(max(iy + 1, 0), Y - 1);
     const int z0 = min(max(iz, 0), Z - 1), z1 = min(max(iz + 1, 0), Z - 1);
-    auto at = [&](int x, int y, int z) { return a[((size_t)z * Y + y) * X + x]; };
+    auto at = [&](int x, int y, int z) { return (T)a[(((size_t)z * Y + y) * X + x) * (size_t)stride]; };
     const T bx = (T)((R)1 - ax), cx = (T)ax, by = (T)((R)1 - ay), cy = (T)ay, bz = (T)((R)1 - az), cz = (T)az;
     const T c00 = at(x0, y0, z0) * bx + at(x1, y0, z0) * cx;
     const T c10 = at(x0, y1, z0) * bx + at(x1, y1, z0) * cx;
@@ -2303,8 +2304,8 @@ __device__ __forceinline__ T sample_trilinear(const T* a, int X, int Y, int Z, R
     return c0 * bz + c1 * cz;
 }
 
-template <class T, class R>
-__global__ __launch_bounds__(256) void k_init_from_bg(ParamsT<T> p, FieldsT<T> f, BgField<T, R> bg) {
+template <class T, class R, class S>
+__global__ __launch_bounds__(256) void k_init_from_bg(ParamsT<T> p, FieldsT<T> f, BgField<T, R, S> bg) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= p.N) return;
     const int x = i % p.X, t = i / p.X, y = t % p.Y, z = t / p.Y;
@@ -2320,8 +2321,8 @@ __global__ __launch_bounds__(256) void k_init_from_bg(ParamsT<T> p, FieldsT<T> f
             w = bzp / ((R)bg.Z * bg.dx) + (R)0.5f;
     T vel[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) vel[a] = sample_trilinear<T, R>(bg.vel[a], bg.X, bg.Y, bg.Z, u, v, w);
-    const T pr = bg.pres ? sample_trilinear<T, R>(bg.pres, bg.X, bg.Y, bg.Z, u, v, w) : (T)0;
+    for (int a = 0; a < 3; ++a) vel[a] = sample_trilinear<T, R, S>(bg.vel[a], bg.vstride, bg.X, bg.Y, bg.Z, u, v, w);
+    const T pr = bg.pres ? sample_trilinear<T, R, S>(bg.pres, 1, bg.X, bg.Y, bg.Z, u, v, w) : (T)0;
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -2334,16 +2335,53 @@ __global__ __launch_bounds__(256) void k_init_from_bg(ParamsT<T> p, FieldsT<T> f
 }
 }  // namespace
 
-template <class T, class R>
-void init_from_background(const ParamsT<T>& p, const FieldsT<T>& f, const BgField<T, R>& bg, hipStream_t s) {
-    hipLaunchKernelGGL((k_init_from_bg<T, R>), dim3((p.N + 255) / 256), dim3(256), 0, s, p, f, bg);
+template <class T, class R, class S>
+void init_from_background(const ParamsT<T>& p, const FieldsT<T>& f, const BgField<T, R, S>& bg, hipStream_t s) {
+    hipLaunchKernelGGL((k_init_from_bg<T, R, S>), dim3((p.N + 255) / 256), dim3(256), 0, s, p, f, bg);
 }
-template void init_from_background<float, float>(const ParamsT<float>&, const FieldsT<float>&,
-                                                 const BgField<float, float>&, hipStream_t);
-template void init_from_background<double, double>(const ParamsT<double>&, const FieldsT<double>&,
-                                                   const BgField<double, double>&, hipStream_t);
-template void init_from_background<CStep, double>(const ParamsT<CStep>&, const FieldsT<CStep>&,
-                                                  const BgField<CStep, double>&, hipStream_t);
+template void init_from_background<float, float, float>(const ParamsT<float>&, const FieldsT<float>&,
+                                                        const BgField<float, float, float>&, hipStream_t);
+template void init_from_background<double, double, double>(const ParamsT<double>&, const FieldsT<double>&,
+                                                           const BgField<double, double, double>&, hipStream_t);
+template void init_from_background<CStep, double, CStep>(const ParamsT<CStep>&, const FieldsT<CStep>&,
+                                                         const BgField<CStep, double, CStep>&, hipStream_t);
+template void init_from_background<double, double, float>(const ParamsT<double>&, const FieldsT<double>&,
+                                                          const BgField<double, double, float>&, hipStream_t);
+template void init_from_background<CStep, double, float>(const ParamsT<CStep>&, const FieldsT<CStep>&,
+                                                         const BgField<CStep, double, float>&, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Staging array of the allocation-free warm start
+// (uws_init_vel_pres_fields_from_background_noalloc): the caller's
+// background arrays are copied here as they are instead of into a buffer
+// allocated per call.  Each GPU holds its own copy of the array until the
+// process ends; host code never uses the array's name as an address, only
+// the per-device address hipGetSymbolAddress returns.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[UWS_BG_STAGING_MAX_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_staging_mu;
+void* g_staging[kMaxDevices] = {};
+}  // namespace
+
+int bg_staging_setup() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return UWS_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(g_staging_mu);
+    if (g_staging[dev]) return UWS_OK;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return UWS_ERR_DEVICE;
+    g_staging[dev] = ptr;
+    return UWS_OK;
+}
+
+void* bg_staging(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_staging_mu);
+    return g_staging[device];
+}
 
 template <class T>
 void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s) {

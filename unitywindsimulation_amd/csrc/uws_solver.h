@@ -381,6 +381,7 @@ struct Solver : SolverBase {
                 bg.vel[a] = planes + (size_t)a * nb;
             }
             bg.pres = pres ? planes + 3 * (size_t)nb : nullptr;
+            bg.vstride = 1;
             // the reference forms the angle in float: flowFieldOrientation * (float)(3.1415926 / 180)
             bg.angle = (R)(rot_deg * (float)(3.1415926 / 180));
             uws::init_from_background<T, R>(p, f, bg, stream);
@@ -388,6 +389,42 @@ struct Solver : SolverBase {
             if (rc == UWS_OK) rc = sync();
         }
         (void)hipFree(tmp);
+        rs.cur = -1;
+        rs.pc_latest = -1;
+        return rc;
+    }
+    // The same warm start with the caller's float arrays staged as they are
+    // in the GPU's fixed staging array ([cell][ncomp] velocity, then the
+    // pressure), converted to T where the kernel reads them: the same values
+    // as the upload above, which converts on the host.  No device allocation.
+    int init_from_background_noalloc(const float fg_pos[3], float rot_deg, const uws_background& bgc,
+                                     const float* vel, int ncomp, const float* pres) override {
+        using R = typename std::conditional<std::is_same<T, float>::value, float, double>::type;
+        const size_t nb = (size_t)bgc.grid_res[0] * bgc.grid_res[1] * bgc.grid_res[2];
+        const size_t vbytes = sizeof(float) * nb * (size_t)ncomp, pbytes = pres ? sizeof(float) * nb : 0;
+        if (vbytes + pbytes > (size_t)UWS_BG_STAGING_MAX_BYTES) return UWS_ERR_TOO_LARGE;
+        char* st = static_cast<char*>(bg_staging(device));
+        if (!st) {
+            if (int rc = bg_staging_setup()) return rc;
+            st = static_cast<char*>(bg_staging(device));
+            if (!st) return UWS_ERR_DEVICE;
+        }
+        HIP_OK(hipMemcpyAsync(st, vel, vbytes, hipMemcpyHostToDevice, stream));
+        if (pres) HIP_OK(hipMemcpyAsync(st + vbytes, pres, pbytes, hipMemcpyHostToDevice, stream));
+        BgField<T, R, float> bg{};
+        bg.X = bgc.grid_res[0]; bg.Y = bgc.grid_res[1]; bg.Z = bgc.grid_res[2];
+        bg.dx = (R)bgc.dx;
+        for (int a = 0; a < 3; ++a) {
+            bg.pos[a] = (R)bgc.phys_field_pos[a];
+            bg.fgPos[a] = (R)fg_pos[a];
+            bg.vel[a] = reinterpret_cast<const float*>(st) + a;
+        }
+        bg.vstride = ncomp;
+        bg.pres = pres ? reinterpret_cast<const float*>(st + vbytes) : nullptr;
+        bg.angle = (R)(rot_deg * (float)(3.1415926 / 180));
+        uws::init_from_background<T, R, float>(p, f, bg, stream);
+        int rc = check_launch();
+        if (rc == UWS_OK) rc = sync();
         rs.cur = -1;
         rs.pc_latest = -1;
         return rc;
