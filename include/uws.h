@@ -142,6 +142,20 @@ int uws_abi_version(void);
  * the inflow it wants (uws_init_vel_pres_fields, uws_set_fixed_value_vel_bnd_cond). */
 int uws_create(const uws_config* cfg, uws_handle* out);
 int uws_destroy(uws_handle h);
+
+/* IFvmSolver.ChangePhysDomainSize() (FvmSolverGpu.cs:1034-1046, called by
+ * FvmSolverController.cs:863-874 after FluidSimConfig.Init() re-derived
+ * gridRes and dx from a new physDomainSize): the handle takes cfg (same
+ * precision and grid type, else UWS_ERR_CONFIG).  Every field is
+ * reallocated for cfg's grid and zeroed as by uws_create; where gridRes is
+ * unchanged the flags and the boundary textures are kept (the reference
+ * keeps its textures), otherwise flags are all fluid and textures zero.
+ * The step counter and time are kept.  The caller then re-initialises the
+ * fields as the reference does (uws_init_vel_pres_fields) and, after a grid
+ * change, the mask and inflow.  Adjoint handles made from h must be
+ * destroyed first.  On an allocation failure the handle can only be
+ * destroyed. */
+int uws_change_phys_domain_size(uws_handle h, const uws_config* cfg);
 int uws_get_config(uws_handle h, uws_config* out);
 int uws_num_cells(uws_handle h, int64_t* n);
 
@@ -313,7 +327,9 @@ int uws_get_field_f64(uws_handle h, int32_t field, double* dst, int64_t n_floats
  *  [3] number of cells whose u or p is not finite
  *  [4 + 8*s + r] low-precision modes: rms of the defect of pressure solve s
  *      (0 = SIMPLE, 1.. = PISO correctors) at the start of round r.
- * Accumulation is in FP64 with a fixed summation order (deterministic). */
+ * Accumulation is in FP64 with a fixed summation order (deterministic).
+ * Returns UWS_ERR_NONFINITE, with out[] filled as usual, when the last step
+ * left a non-finite u or p somewhere ([3] > 0). */
 enum { UWS_NORM_VEL_RESIDUAL = 0, UWS_NORM_PRES_RESIDUAL = 1, UWS_NORM_DIVERGENCE = 2, UWS_NORM_NONFINITE = 3,
        UWS_NORM_REFINE0 = 4, UWS_NORM_COUNT = 64 };
 int uws_get_norms(uws_handle h, double* out, int32_t n);

@@ -134,3 +134,34 @@ def test_time_mean_accumulator(gpu):
     assert cnt == 4
     ref = np.mean(frames, axis=0)
     assert np.max(np.abs(um - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_nonfinite_detection(gpu):
+    """One NaN planted in u^n of a fluid cell: after a step with norms the
+    FP64 count of non-finite cells is at least one and uws_get_norms returns
+    UWS_ERR_NONFINITE (with the norms filled); a clean step returns 0 cells
+    and UWS_OK."""
+    w = workloads.tiny()
+    w.cf.computeNorms = True
+    w.cf.Init()
+    from unitywindsimulation_amd.geometry import noise_velocity
+
+    u0 = noise_velocity(w.flags, w.noise_seed, w.noise_amp).reshape(-1, 3).copy()
+    for plant in (False, True):
+        u = u0.copy()
+        if plant:
+            cell = int(np.flatnonzero(w.flags == 0)[w.ncells // 3])
+            u[cell, 1] = np.nan
+        with solver_for(w, u0=u) as s:
+            s.step(1)
+            out = np.zeros(N.NORM_COUNT)
+            rc = s._lib.uws_get_norms(s._h, N.ptr(out), N.NORM_COUNT)
+            nrm = s.norms()
+            print(f"planted NaN {plant}: status {rc}, non-finite cells {nrm[N.NORM_NONFINITE]:.0f}")
+            if plant:
+                assert rc == N.UWS_ERR_NONFINITE and nrm[N.NORM_NONFINITE] >= 1
+                assert out[N.NORM_NONFINITE] == nrm[N.NORM_NONFINITE]
+                with pytest.raises(N.UwsError):
+                    s.norms(strict=True)
+            else:
+                assert rc == N.UWS_OK and nrm[N.NORM_NONFINITE] == 0

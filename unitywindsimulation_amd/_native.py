@@ -96,6 +96,7 @@ SIGNATURES = {
     "uws_abi_version": (c_int, []),
     "uws_create": (c_int, [POINTER(uws_config), POINTER(c_void_p)]),
     "uws_destroy": (c_int, [c_void_p]),
+    "uws_change_phys_domain_size": (c_int, [c_void_p, POINTER(uws_config)]),
     "uws_get_config": (c_int, [c_void_p, POINTER(uws_config)]),
     "uws_num_cells": (c_int, [c_void_p, POINTER(c_int64)]),
     "uws_init_flags": (c_int, [c_void_p, c_void_p, c_int64]),

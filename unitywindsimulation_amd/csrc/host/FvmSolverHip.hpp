@@ -251,8 +251,26 @@ public:
               "uws_set_fixed_value_vel_bnd_cond");
     }
     void ChangePhysFieldPos() override { InitVelPresFields(); }
+    // FvmSolverGpu.ChangePhysDomainSize (FvmSolverGpu.cs:1034-1046): the config's
+    // re-derived gridRes and dx, then InitVelPresFields.  The fields are
+    // reallocated for the new grid; flags and boundary textures survive where
+    // gridRes is unchanged; after a grid change the mask is all fluid (the box
+    // is re-applied for SolidType.Box, a model mask comes from InitFlags as in
+    // FvmSolverController.cs:869-874), the stretched mesh is rebuilt and the
+    // inflow texture refilled.
     void ChangePhysDomainSize() override {
-        throw Error(UWS_ERR_UNSUPPORTED, "ChangePhysDomainSize (the grid of a handle is fixed)");
+        const int32_t old[3] = {cf_.gridRes.x, cf_.gridRes.y, cf_.gridRes.z};
+        cf_.Init();
+        const uws_config c = cf_.to_c();
+        check(uws_change_phys_domain_size(h_, &c), "uws_change_phys_domain_size");
+        n_ = (int64_t)cf_.gridRes.x * cf_.gridRes.y * cf_.gridRes.z;
+        const bool changed = old[0] != cf_.gridRes.x || old[1] != cf_.gridRes.y || old[2] != cf_.gridRes.z;
+        if (cf_.gridType == GridType::CollNonUniform) InitMeshBuffers();
+        InitVelPresFields();
+        if (changed) {
+            if (cf_.solidType == SolidType::Box) check(uws_init_box(h_), "uws_init_box");
+            SetFixedValueVelBndCond();
+        }
     }
     std::vector<float> GetVelField() override {
         std::vector<float> v((size_t)n_ * 4);

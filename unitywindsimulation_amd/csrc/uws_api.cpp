@@ -568,6 +568,12 @@ int uws_create(const uws_config* cfg, uws_handle* out) {
     return UWS_OK;
 }
 
+int uws_change_phys_domain_size(uws_handle h, const uws_config* cfg) {
+    if (!cfg) return UWS_ERR_INVALID_ARG;
+    if (int rc = validate(cfg, true)) return rc;
+    return with_solver(h, [&](SolverBase& s) { return s.reconfigure(*cfg); });
+}
+
 int uws_destroy(uws_handle h) {
     if (!h) return UWS_ERR_INVALID_ARG;
     if (h->impl) (void)hipSetDevice(h->impl->device);
@@ -842,7 +848,8 @@ int uws_get_norms(uws_handle h, double* out, int32_t n) {
     return with_solver(h, [&](SolverBase& s) {
         double raw[UWS_NORM_COUNT];
         if (int rc = s.get_norms(raw)) return rc;
-        return normalise_norms(raw, (double)s.ncells(), out, n);
+        if (int rc = normalise_norms(raw, (double)s.ncells(), out, n)) return rc;
+        return raw[UWS_NORM_NONFINITE] > 0.0 ? (int)UWS_ERR_NONFINITE : (int)UWS_OK;
     });
 }
 

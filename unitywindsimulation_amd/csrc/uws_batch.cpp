@@ -373,7 +373,8 @@ int uws_batch_get_norms(uws_batch_handle b, int32_t c, double* out, int32_t n) {
     return with_case(b, c, [&](SolverBase& s) {
         double raw[UWS_NORM_COUNT];
         if (int rc = s.get_norms(raw)) return rc;
-        return normalise_norms(raw, (double)s.ncells(), out, n);
+        if (int rc = normalise_norms(raw, (double)s.ncells(), out, n)) return rc;
+        return raw[UWS_NORM_NONFINITE] > 0.0 ? (int)UWS_ERR_NONFINITE : (int)UWS_OK;
     });
 }
 

@@ -78,6 +78,8 @@ struct SolverBase {
     // the same from the per-GPU staging array (no device allocation); the caller checked the size
     virtual int init_from_background_noalloc(const float fg_pos[3], float rot_deg, const uws_background& bg,
                                              const float* vel, int ncomp, const float* pres) = 0;
+    // ChangePhysDomainSize: reallocate for a new configuration (same precision and grid type)
+    virtual int reconfigure(const uws_config& c) = 0;
     virtual int mean_reset() = 0;
     virtual int mean_accumulate() = 0;
     virtual int get_mean(double* dst, int64_t n, int64_t* count) = 0;

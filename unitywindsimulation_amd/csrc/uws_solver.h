@@ -430,6 +430,51 @@ struct Solver : SolverBase {
         return rc;
     }
 
+    // ---- ChangePhysDomainSize: a new configuration of the same arithmetic ----
+    // The fields are reallocated for the new grid and zeroed (as at creation);
+    // where the grid is unchanged the flags and the boundary textures are kept,
+    // as the reference keeps its textures.  The clock is kept.
+    int reconfigure(const uws_config& c) override {
+        if (c.precision != cfg.precision || c.grid_type != cfg.grid_type) return UWS_ERR_CONFIG;
+        const bool same_grid = c.grid_res[0] == cfg.grid_res[0] && c.grid_res[1] == cfg.grid_res[1] &&
+                               c.grid_res[2] == cfg.grid_res[2];
+        std::vector<uint8_t> fl;
+        std::vector<T> bnd[6][3];
+        if (same_grid) {
+            fl.resize((size_t)p.N);
+            HIP_OK(hipMemcpyAsync(fl.data(), f.flag, fl.size(), hipMemcpyDeviceToHost, stream));
+            for (int face = 0; face < 6; ++face)
+                for (int k = 0; k < 3; ++k) {
+                    bnd[face][k].resize((size_t)bnd_w(p.X, p.Y, face) * bnd_h(p.Y, p.Z, face));
+                    HIP_OK(hipMemcpyAsync(bnd[face][k].data(), f.bnd[face][k], sizeof(T) * bnd[face][k].size(),
+                                          hipMemcpyDeviceToHost, stream));
+                }
+        }
+        if (int rc = sync()) return rc;
+        drop_graphs();
+        if (mean_sum[0]) (void)hipFree(mean_sum[0]);
+        for (auto& m : mean_sum) m = nullptr;
+        mean_count = 0;
+        (void)hipFree(slab);
+        slab = nullptr;
+        cfg = c;
+        sc = step_cfg(c);
+        rs = RoleState{};
+        fuse_mode = fusion_mode();
+        for (auto& fp : face_pos) fp.clear();
+        p = ParamsT<T>{};
+        f = FieldsT<T>{};
+        if (int rc = init(c)) return rc;
+        if (same_grid) {
+            for (int face = 0; face < 6; ++face)
+                for (int k = 0; k < 3; ++k)
+                    HIP_OK(hipMemcpyAsync(f.bnd[face][k], bnd[face][k].data(), sizeof(T) * bnd[face][k].size(),
+                                          hipMemcpyHostToDevice, stream));
+            return set_flags_u8(fl.data());
+        }
+        return sync();
+    }
+
     // ---- time mean of the velocity ----
     int mean_reset() override {
         if (!mean_sum[0]) {

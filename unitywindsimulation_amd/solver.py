@@ -199,7 +199,30 @@ class FvmSolverHip(IFvmSolver):
         self.InitVelPresFields()
 
     def ChangePhysDomainSize(self) -> None:
-        raise NotImplementedError("the grid of a handle is fixed; create a new FvmSolverHip")
+        """FvmSolverGpu.ChangePhysDomainSize (FvmSolverGpu.cs:1034-1046): take
+        the configuration's re-derived gridRes and dx (cf.Init()) and
+        re-initialise the fields (InitVelPresFields).  The handle's fields are
+        reallocated for the new grid (uws_change_phys_domain_size); flags and
+        boundary textures survive where gridRes is unchanged.  After a grid
+        change the mask is all fluid (the controller re-voxelizes and calls
+        InitFlags, FvmSolverController.cs:869-874; SolidType.Box re-applies
+        the hard-coded box), the stretched mesh is rebuilt as in the
+        constructor, and the inflow texture is refilled."""
+        old_grid = self.cf.gridRes
+        self.cf.Init()
+        self._cfg = self.cf.to_c()
+        N.check(self._lib.uws_change_phys_domain_size(self._h, byref(self._cfg)), "uws_change_phys_domain_size")
+        X, Y, Z = self.cf.gridRes
+        self.shape = (Z, Y, X)
+        self.n = X * Y * Z
+        changed = tuple(self.cf.gridRes) != tuple(old_grid)
+        if int(self.cf.gridType) == int(GridType.CollNonUniform):
+            self.set_face_positions(*reference_mesh(self.cf))
+        self.InitVelPresFields()
+        if changed:
+            if int(self.cf.solidType) == 1:
+                N.check(self._lib.uws_init_box(self._h), "uws_init_box")
+            self.SetFixedValueVelBndCond()
 
     def GetVelField(self) -> np.ndarray:
         return self.velocity(4)
@@ -322,10 +345,14 @@ class FvmSolverHip(IFvmSolver):
         N.check(self._lib.uws_time_kernel(self._h, int(kernel), int(reps), byref(ms)), "uws_time_kernel")
         return float(ms.value)
 
-    def norms(self) -> np.ndarray:
-        """FP64 norms of the last step (needs ``computeNorms``): see ``uws_get_norms``."""
+    def norms(self, strict: bool = False) -> np.ndarray:
+        """FP64 norms of the last step (needs ``computeNorms``): see ``uws_get_norms``.
+        A step that left a non-finite value returns the norms all the same
+        (``out[NORM_NONFINITE]`` counts the cells); ``strict`` raises instead."""
         out = np.zeros(N.NORM_COUNT, dtype=np.float64)
-        N.check(self._lib.uws_get_norms(self._h, N.ptr(out), N.NORM_COUNT), "uws_get_norms")
+        rc = self._lib.uws_get_norms(self._h, N.ptr(out), N.NORM_COUNT)
+        if rc != N.UWS_ERR_NONFINITE or strict:
+            N.check(rc, "uws_get_norms")
         return out
 
     def field64(self, fid: int) -> np.ndarray:
