@@ -334,6 +334,12 @@ enum { UWS_NORM_VEL_RESIDUAL = 0, UWS_NORM_PRES_RESIDUAL = 1, UWS_NORM_DIVERGENC
        UWS_NORM_REFINE0 = 4, UWS_NORM_COUNT = 64 };
 int uws_get_norms(uws_handle h, double* out, int32_t n);
 
+/* Jacobi sweeps the last step ran: momentum (VelPredict) and SIMPLE pressure
+ * (PresCorrect).  velMaxNumIter / presMaxNumIter, or k + 1 where the
+ * residual-based exit (calResidual) stopped at sweep k (FvmSolverGpu.cs
+ * 1250-1262, 1386-1398); 0 before the first step. */
+int uws_get_sweep_counts(uws_handle h, int32_t* vel_sweeps, int32_t* pres_sweeps);
+
 /* Diagnostics (no reference counterpart): average duration in milliseconds of
  * one launch of a hot-loop kernel, timed with HIP events on the handle's
  * stream over `reps` back-to-back launches on the current step data.

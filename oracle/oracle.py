@@ -34,6 +34,7 @@ _SIG = {
     "uwso_get_bnd_field": (c_int, [c_void_p, c_int, c_void_p]),
     "uwso_log_law_wall_func": (c_double, [c_void_p, c_double]),
     "uwso_residuals": (None, [c_void_p, c_void_p]),
+    "uwso_sweep_counts": (None, [c_void_p, c_void_p]),
     "uwso_time": (None, [c_void_p, POINTER(c_int64), POINTER(c_double)]),
     "uwso_set_face_positions": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "uwso_get_face_flux": (c_int, [c_void_p, c_void_p]),
@@ -180,6 +181,12 @@ class Oracle:
         out = np.zeros(2)
         self.L.uwso_residuals(self.h, out.ctypes.data_as(c_void_p))
         return out
+
+    def sweep_counts(self):
+        """(momentum, SIMPLE pressure) sweeps run by the last step."""
+        out = np.zeros(2, dtype=np.int32)
+        self.L.uwso_sweep_counts(self.h, out.ctypes.data_as(c_void_p))
+        return int(out[0]), int(out[1])
 
     def time(self):
         s, t = c_int64(), c_double()

@@ -84,6 +84,7 @@ template <class T> struct Oracle {
     int64_t step_count = 0;
     double phys_time = 0.0;
     T last_vel_residual = 0, last_pres_residual = 0;
+    int last_vel_sweeps = 0, last_pres_sweeps = 0;  // sweeps run by the last VelPredict / PresCorrect
 
     int64_t idx(int x, int y, int z) const { return (int64_t)z * X * Y + (int64_t)y * X + x; }
     bool InRegion(int x, int y, int z) const {
@@ -549,11 +550,15 @@ template <class T> struct Oracle {
     }
     // VelPredict() (FvmSolverGpu.cs:1238-1282) without time slicing
     void VelPredictStage() {
+        last_vel_sweeps = cf.vel_max_num_iter;
         for (int k = 0; k < cf.vel_max_num_iter; ++k) {
             VelPredict();
             if (cf.cal_residual && cf.vel_residual_check_interval > 0 && k % cf.vel_residual_check_interval == 0) {
                 last_vel_residual = VelPredictResidual();
-                if (last_vel_residual < (T)cf.vel_tolerance) break;
+                if (last_vel_residual < (T)cf.vel_tolerance) {
+                    last_vel_sweeps = k + 1;
+                    break;
+                }
             }
             std::swap(velFieldLastIter, velField);
         }
@@ -682,11 +687,15 @@ template <class T> struct Oracle {
     }
     // PresCorrect() sweeps (FvmSolverGpu.cs:1367-1415)
     void PresSweepsStage() {
+        last_pres_sweeps = cf.pres_max_num_iter;
         for (int k = 0; k < cf.pres_max_num_iter; ++k) {
             PresCorrectSweep();
             if (cf.cal_residual && cf.pres_residual_check_interval > 0 && k % cf.pres_residual_check_interval == 0) {
                 last_pres_residual = PresCorrectResidual();
-                if (last_pres_residual < (T)cf.pres_tolerance) break;
+                if (last_pres_residual < (T)cf.pres_tolerance) {
+                    last_pres_sweeps = k + 1;
+                    break;
+                }
             }
             std::swap(presCorrectFieldLastIter, presCorrectField);
         }
@@ -1408,6 +1417,9 @@ double uwso_log_law_wall_func(void* hv, double uP) {
 }
 void uwso_residuals(void* hv, double* out2) {
     dispatch((Handle*)hv, [&](auto& o) { out2[0] = o.last_vel_residual; out2[1] = o.last_pres_residual; });
+}
+void uwso_sweep_counts(void* hv, int* out2) {
+    dispatch((Handle*)hv, [&](auto& o) { out2[0] = o.last_vel_sweeps; out2[1] = o.last_pres_sweeps; });
 }
 void uwso_time(void* hv, int64_t* step, double* t) {
     dispatch((Handle*)hv, [&](auto& o) { *step = o.step_count; *t = o.phys_time; });

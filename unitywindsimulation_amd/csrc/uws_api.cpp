@@ -853,6 +853,15 @@ int uws_get_norms(uws_handle h, double* out, int32_t n) {
     });
 }
 
+int uws_get_sweep_counts(uws_handle h, int32_t* vel_sweeps, int32_t* pres_sweeps) {
+    if (!vel_sweeps || !pres_sweeps) return UWS_ERR_INVALID_ARG;
+    return with_solver(h, [&](SolverBase& s) {
+        *vel_sweeps = s.rs.vel_sweeps;
+        *pres_sweeps = s.rs.pres_sweeps;
+        return (int)UWS_OK;
+    });
+}
+
 int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms) {
     if (!avg_ms || reps <= 0 || kernel < UWS_KERNEL_VEL_SWEEP || kernel > UWS_KERNEL_PRES_SWEEP3)
         return UWS_ERR_INVALID_ARG;

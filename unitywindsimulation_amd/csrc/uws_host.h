@@ -32,6 +32,10 @@ struct RoleState {
     int cur = -1;
     int pc_latest = -1;
     int piso_done = 0;
+    // sweeps the last momentum solve and the last SIMPLE pressure solve ran
+    // (the configured counts unless the residual-based exit stopped earlier)
+    int vel_sweeps = 0;
+    int pres_sweeps = 0;
 };
 
 // host<->device plane selectors: p, u' (comp 0..2), boundary textures, velocity buffer b comp c
@@ -159,6 +163,7 @@ int stage_vel_sweeps(Lr& l, const StepCfg& c, RoleState& rs) {
     const int s = rs.state, w0 = (s + 1) % 3, w1 = (s + 2) % 3;
     int in = s, last_in = s, result = -1;
     bool broke = false;
+    rs.vel_sweeps = c.V;
     // pairs of sweeps run as one two-sweep launch where the launcher has it
     // (the first sweep's result stays on chip: a pair writes one buffer); the
     // residual-based exit and the last sweep's residual need single sweeps
@@ -195,6 +200,7 @@ int stage_vel_sweeps(Lr& l, const StepCfg& c, RoleState& rs) {
                 // the reference keeps the previous iterate after a break (the swap is skipped, :1256-1279)
                 result = in;
                 broke = true;
+                rs.vel_sweeps = k + 1;
                 break;
             }
         }
@@ -222,7 +228,8 @@ int stage_vel_sweeps(Lr& l, const StepCfg& c, RoleState& rs) {
 // FvmSolverGpu.cs:1367-1415) or a PISO corrector (coef 1, :1493-1514).
 // Returns the buffer of the latest iterate in *latest (-1: zero).
 template <class Lr>
-int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest) {
+int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest, int* sweeps_run = nullptr) {
+    if (sweeps_run) *sweeps_run = c.lowp ? c.R * c.K : c.P;
     if (c.lowp) {
         // FP32 defect correction with 2-byte correction storage
         int x = -1;
@@ -277,7 +284,11 @@ int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest) {
             l(KN_SUM_PARTIALS, r, UWS_NORM_PRES_RESIDUAL);
             double sum = 0.0;
             if (int rc = l.read_norm(UWS_NORM_PRES_RESIDUAL, &sum)) return rc;
-            if ((float)std::sqrt(sum / l.ncells()) < c.ptol) { broke = true; break; }
+            if ((float)std::sqrt(sum / l.ncells()) < c.ptol) {
+                broke = true;
+                if (sweeps_run) *sweeps_run = k + 1;
+                break;
+            }
         }
         last_in = in;
         in = out;
@@ -327,7 +338,7 @@ int run_stage(Lr& l, const StepCfg& c, RoleState& rs, int stage) {
         r.vout = rs.cur;
         l(KN_PRES_PRECOMP, r);
         return UWS_OK;
-    case UWS_STAGE_PRES_SWEEPS: return pres_solve(l, c, 0, 0, &rs.pc_latest);
+    case UWS_STAGE_PRES_SWEEPS: return pres_solve(l, c, 0, 0, &rs.pc_latest, &rs.pres_sweeps);
     case UWS_STAGE_APPLY_PRES:
         ensure_pc(l, rs);
         r.pin = rs.pc_latest;

@@ -355,6 +355,12 @@ class FvmSolverHip(IFvmSolver):
             N.check(rc, "uws_get_norms")
         return out
 
+    def sweep_counts(self):
+        """(momentum, SIMPLE pressure) Jacobi sweeps the last step ran (uws_get_sweep_counts)."""
+        v, p = ctypes.c_int32(), ctypes.c_int32()
+        N.check(self._lib.uws_get_sweep_counts(self._h, byref(v), byref(p)), "uws_get_sweep_counts")
+        return v.value, p.value
+
     def field64(self, fid: int) -> np.ndarray:
         nc = N.FIELD_NCOMP[fid]
         out = np.empty(self.n * nc, dtype=np.float64)
