@@ -34,6 +34,33 @@ def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
     return 263 - (0 if les else 83) + 77 * V + 29 * P + C * (123 + 33 * P)
 
 
+def compulsory_bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
+    """Bytes per cell one step of this build must move, each array once per
+    launch: wall function + eddy viscosity with the gradient store (78), the
+    momentum precompute with the LES term and the gradient read (146), V / 2
+    momentum pairs (73 each, a leftover single sweep 77), the Rhie-Chow
+    precompute (37), per pressure solve P / 3 triples (28 each; a leftover
+    pair 28, a single 29), p += p' (13), the velocity correction (53) and per
+    PISO corrector A_od u' (37), the corrector's right-hand side (4) and the
+    correction (78).  3,818 B/cell at V = P = 50, C = 2."""
+    solve = (P // 3) * 28 + {0: 0, 1: 29, 2: 28}[P % 3]
+    one_shot = (78 + 146) if les else 69
+    return one_shot + (V // 2) * 73 + (V % 2) * 77 + 37 + solve + 13 + 53 + C * (37 + 4 + solve + 78)
+
+
+def stream_ceiling(cells: int, nin: int, reps: int = 20) -> float:
+    """GB/s of the library's plain streaming kernel over nin + 1 float arrays
+    of `cells` values (uws_stream_bandwidth), launched back to back on the
+    same arrays as the sweeps are."""
+    import ctypes
+
+    from unitywindsimulation_amd import _native as N
+
+    g = ctypes.c_double()
+    N.check(N.lib().uws_stream_bandwidth(int(cells), int(nin), int(reps), ctypes.byref(g)), "uws_stream_bandwidth")
+    return g.value
+
+
 def cpu_baseline(budget_s: float = 15.0) -> dict:
     """The CPU oracle (a restatement of the reference kernels, its cell loops
     parallel over the host cores with OpenMP like the reference's Parallel.For
@@ -70,14 +97,21 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
                     break
     except OSError:
         pass
+    import shutil
+
+    dotnet = {t: shutil.which(t) is not None for t in ("dotnet", "mono", "csc")}
     return {"value": round(X * Y * Z * n / dt / 1e6, 4), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"{n} PISO steps of {w.name} (V=P=50, C=2), FP32 oracle on {threads} OpenMP threads "
                       f"({os.cpu_count()} logical CPUs visible)",
+            "stand_in_for": "the C# Parallel.For transcription of the reference kernels, which this build does not "
+                            "have (the image it is built in has no .NET toolchain): the C++ OpenMP oracle, the same "
+                            "kernels, order and FP32 arithmetic, runs instead",
+            "dotnet_toolchain_on_host": dotnet,
             "ms_per_step": round(dt / n * 1e3, 2), "algorithmic_GBps": round(b_step * X * Y * Z * n / dt / 1e9, 2),
             "cpu_model": cpu_model}
 
 
-def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
+def ensemble_report(steps: int = 50, check_steps: int = 500) -> dict:
     """The 24-case uneven ensemble: cases/s of the batched ensemble (one graph
     per step over all cases) against the same cases stepped one after another,
     each with its own graph per step ("plain") and with one host launch per
@@ -145,7 +179,24 @@ def ensemble_report(steps: int = 5, check_steps: int = 10) -> dict:
             got = e.velocity(c, 3).reshape(-1, 3).astype(np.float64)
             errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
     cells = sum(w.ncells for w in ws)
-    return {"cases": len(ws), "cells": cells, "steps_per_case": full_steps,
+    # bound: every step must move each case's compulsory bytes; no launch shape moves them faster than
+    # the plain streaming kernel does on any working set these cases form (one case's arrays, which the
+    # last-level cache holds, or all cases' arrays at once)
+    cf0 = ws[0].cf
+    C0 = cf0.PISONumCorrectors if int(cf0.fvmSolverType) == 1 else 0
+    b_comp = compulsory_bytes_per_cell_step(cf0.velMaxNumIter, cf0.presMaxNumIter, C0,
+                                            int(cf0.turbulenceModel) == 1)
+    sizes = sorted({w.ncells for w in ws})
+    ceilings = {f"{n}_cells_nin{k}": stream_ceiling(n, k) for n in sizes for k in (6, 16)}
+    ceilings[f"{cells}_cells_nin16"] = stream_ceiling(cells, 16, 5)
+    best = max(ceilings.values())
+    t_min = b_comp * cells / (best * 1e9)
+    bound = {"compulsory_B_per_cell_step": b_comp, "compulsory_GB_per_step": round(b_comp * cells / 1e9, 3),
+             "stream_ceilings_GBps": {k: round(v, 1) for k, v in ceilings.items()},
+             "min_ms_per_step_at_best_ceiling": round(t_min * 1e3, 3),
+             "max_speedup_vs_plain": round(t_plain / t_min, 3),
+             "batched_frac_of_bound": round(t_min / t_batch, 4)}
+    return {"cases": len(ws), "cells": cells, "steps_per_case": full_steps, "bound": bound,
             "plain_cases_per_s": round(len(ws) / (t_plain * full_steps), 4),
             "batched_cases_per_s": round(len(ws) / (t_batch * full_steps), 4),
             "speedup": round(t_plain / t_batch, 3),
@@ -460,6 +511,15 @@ def main() -> int:
     u = s.velocity(3)
     finite = bool(torch.isfinite(torch.from_numpy(u)).all())
     s.close()
+    # what a plain streaming kernel reaches on the same working sets (6 arrays read + 1 written for the
+    # pressure launches' 28 B/cell, 16 + 1 for the momentum pairs' 73 B/cell) and on one far larger than
+    # the last-level cache (HBM itself)
+    ceil = {"pressure_like_28B_per_cell": round(stream_ceiling(ncell, 6), 1),
+            "momentum_like_68B_per_cell": round(stream_ceiling(ncell, 16), 1),
+            "hbm_1792MiB": round(stream_ceiling(64 << 20, 6, 10), 1)}
+    kceil = ceil["momentum_like_68B_per_cell"] if kname == "k_vel_sweep2" else ceil["pressure_like_28B_per_cell"]
+    b_comp = compulsory_bytes_per_cell_step(V, P, C, les)
+    comp_gbps = b_comp * ncell / (ms_step * 1e-3) / 1e9
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -488,8 +548,16 @@ def main() -> int:
         "hbm_frac_basis": "B_step: every Jacobi sweep counted as a separate pass over its arrays; the two-sweep "
                           "launches move less, so this can exceed 1",
         "algorithmic_GBps": round(b_step * ncell / (ms_step * 1e-3) / 1e9, 1),
+        "step_bytes": {"compulsory_B_per_cell": b_comp, "compulsory_GBps": round(comp_gbps, 1),
+                       "compulsory_frac_of_peak": round(comp_gbps / HBM_PEAK_GBS, 4),
+                       "compulsory_frac_of_stream_ceiling": round(comp_gbps / ceil["momentum_like_68B_per_cell"], 4),
+                       "basis": "each array once per launch of the kernels the step runs (multi-sweep launches "
+                                "counted as one pass); the stream ceiling is the plain streaming kernel on a "
+                                "working set of the momentum pair's size"},
+        "stream_ceiling_GBps": ceil,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac_of_stream_ceiling": round(achieved / kceil, 4),
                      "algorithmic_bytes": kbytes, "single_sweeps_bytes": kbytes_sweeps,
                      "kernel_ms": {"k_vel_sweep": round(t_vel, 5), "k_vel_sweep2": round(t_vel2, 5),
                                    "k_pres_sweep": round(t_pres, 5), "k_pres_sweep2": round(t_pres2, 5),

@@ -354,6 +354,15 @@ enum { UWS_KERNEL_VEL_SWEEP = 0, UWS_KERNEL_PRES_SWEEP = 1, UWS_KERNEL_PRES_SWEE
        UWS_KERNEL_PRES_SWEEP3 = 4 };
 int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms);
 
+/* Diagnostics (no reference counterpart): the bandwidth ceiling a plain
+ * streaming kernel reaches on the current device, in GB/s of bytes read and
+ * written.  Each launch reads nin (1..16) float arrays of `cells` values and
+ * writes one (out = sum of the inputs, 16-byte accesses), reps launches back
+ * to back on the same arrays, as the Jacobi sweeps re-read theirs: a working
+ * set (nin + 1) * 4 * cells bytes that fits the last-level cache is served
+ * from it.  Allocates and frees its own arrays. */
+int uws_stream_bandwidth(int64_t cells, int32_t nin, int32_t reps, double* gbps);
+
 /* ---- ensemble of uneven cases (extension, no reference counterpart) ----
  * Several independent solvers (different grids, buildings, inflow) stepped
  * together: every kernel of the step runs once for all cases, over one tile

@@ -124,6 +124,7 @@ SIGNATURES = {
     "uws_get_field_f64": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
     "uws_get_norms": (c_int, [c_void_p, c_void_p, c_int32]),
     "uws_get_sweep_counts": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
+    "uws_stream_bandwidth": (c_int, [c_int64, c_int32, c_int32, POINTER(c_double)]),
     "uws_time_kernel": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_double)]),
     "uws_batch_create": (c_int, [c_void_p, c_int32, POINTER(c_void_p)]),
     "uws_batch_destroy": (c_int, [c_void_p]),

@@ -853,6 +853,11 @@ int uws_get_norms(uws_handle h, double* out, int32_t n) {
     });
 }
 
+int uws_stream_bandwidth(int64_t cells, int32_t nin, int32_t reps, double* gbps) {
+    if (!gbps) return UWS_ERR_INVALID_ARG;
+    return stream_bandwidth(cells, nin, reps, gbps);
+}
+
 int uws_get_sweep_counts(uws_handle h, int32_t* vel_sweeps, int32_t* pres_sweeps) {
     if (!vel_sweeps || !pres_sweeps) return UWS_ERR_INVALID_ARG;
     return with_solver(h, [&](SolverBase& s) {

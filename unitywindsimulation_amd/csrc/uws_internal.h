@@ -171,6 +171,10 @@ void init_from_background(const ParamsT<T>& p, const FieldsT<T>& f, const BgFiel
 int bg_staging_setup();
 void* bg_staging(int device);
 
+// bandwidth of a plain streaming kernel (nin float arrays of `cells` read,
+// one written, back to back on the same arrays), GB/s on the current device
+int stream_bandwidth(int64_t cells, int nin, int reps, double* gbps);
+
 // sum[c][i] += u[c][i] in FP64 (time averaging of the velocity)
 template <class T>
 void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s);

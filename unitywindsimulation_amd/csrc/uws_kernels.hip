@@ -2383,6 +2383,91 @@ void* bg_staging(int device) {
     return g_staging[device];
 }
 
+// ---------------------------------------------------------------------------
+// Streaming reference for the bandwidth ceiling: out[i] = sum_k in_k[i] over
+// nin float arrays and one written array, 16-byte loads and stores, a
+// grid-stride loop over a fixed grid of 16 workgroups per CU.  Launched
+// back to back on the same arrays, as the Jacobi sweeps re-read theirs, so a
+// working set that fits the last-level cache is served from it.
+// ---------------------------------------------------------------------------
+namespace {
+struct StreamArgs {
+    const float4* in[16];
+    float4* out;
+    int nin;
+    int64_t n4;
+};
+// each thread keeps four 16-byte accesses per array in flight (four
+// workgroup-wide rows of 256 vectors per iteration)
+__global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
+    constexpr int U = 4;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * U;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x * U + threadIdx.x; i0 < a.n4; i0 += stride) {
+        float4 acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            acc[u] = i < a.n4 ? a.in[0][i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        for (int k = 1; k < a.nin; ++k) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = i0 + (int64_t)u * blockDim.x;
+                if (i < a.n4) {
+                    const float4 v = a.in[k][i];
+                    acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            if (i < a.n4) a.out[i] = acc[u];
+        }
+    }
+}
+}  // namespace
+
+int stream_bandwidth(int64_t cells, int nin, int reps, double* gbps) {
+    if (cells < 1024 || nin < 1 || nin > 16 || reps < 1) return UWS_ERR_INVALID_ARG;
+    const int64_t n4 = (cells + 3) / 4;
+    const size_t plane = (size_t)n4 * sizeof(float4);
+    void* mem = nullptr;
+    if (hipMalloc(&mem, plane * (size_t)(nin + 1)) != hipSuccess) return UWS_ERR_ALLOC;
+    StreamArgs a{};
+    for (int k = 0; k < nin; ++k) a.in[k] = reinterpret_cast<const float4*>(static_cast<char*>(mem) + plane * k);
+    a.out = reinterpret_cast<float4*>(static_cast<char*>(mem) + plane * nin);
+    a.nin = nin;
+    a.n4 = n4;
+    int rc = UWS_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        rc = UWS_ERR_DEVICE;
+    const int grid = (int)std::min<int64_t>((n4 + 1023) / 1024, (int64_t)cus * 16);
+    if (rc == UWS_OK && (hipMemsetAsync(mem, 0, plane * (size_t)(nin + 1), nullptr) != hipSuccess ||
+                         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = UWS_ERR_DEVICE;
+    if (rc == UWS_OK) {
+        hipLaunchKernelGGL(k_stream, dim3(grid), dim3(256), 0, nullptr, a);  // warm-up
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream, dim3(grid), dim3(256), 0, nullptr, a);
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.f)
+            rc = UWS_ERR_DEVICE;
+        else
+            *gbps = (double)plane * (nin + 1) * reps / ((double)ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(mem);
+    return rc;
+}
+
 template <class T>
 void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s) {
     const int grid = std::min((n + 255) / 256, 8192);
