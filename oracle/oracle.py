@@ -39,8 +39,15 @@ _SIG = {
     "uwso_set_face_positions": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "uwso_get_face_flux": (c_int, [c_void_p, c_void_p]),
     "uwso_set_face_flux": (c_int, [c_void_p, c_void_p]),
+    "uwso_run_kernel": (c_int, [c_void_p, c_int]),
 }
 _lib = None
+_variants = {}
+
+# single kernels and extra field ids of the pinning tests (uws_oracle.cpp)
+KERNEL_VEL_SWEEP, KERNEL_PISO_AODU, KERNEL_PISO_PRES_SWEEP, KERNEL_PISO_VEL_CORRECT = 0, 1, 2, 3
+FIELD_VEL_ITER, FIELD_PC_ITER = 100, 101
+_NCOMP = {**N.FIELD_NCOMP, FIELD_VEL_ITER: 3, FIELD_PC_ITER: 1}
 
 
 def build() -> Path:
@@ -49,28 +56,48 @@ def build() -> Path:
     return LIB
 
 
-def lib():
+def _load(path):
+    d = ctypes.CDLL(str(path))
+    for k, (r, a) in _SIG.items():
+        fn = getattr(d, k)
+        fn.restype = r
+        fn.argtypes = a
+    return d
+
+
+def lib(path=None):
+    """The oracle library; `path` loads another build of the same source
+    (the planted-mistake builds of the pinning tests)."""
     global _lib
+    if path is not None:
+        key = str(path)
+        if key not in _variants:
+            _variants[key] = _load(path)
+        return _variants[key]
     if _lib is None:
         if not LIB.exists():
             build()
-        d = ctypes.CDLL(str(LIB))
-        for k, (r, a) in _SIG.items():
-            fn = getattr(d, k)
-            fn.restype = r
-            fn.argtypes = a
-        _lib = d
+        _lib = _load(LIB)
     return _lib
+
+
+def build_variant(mutant: int, out_dir) -> Path:
+    """The oracle compiled with planted mistake number `mutant` (-DUWSO_MUTANT)."""
+    out = Path(out_dir) / f"liboracle_mutant{mutant}.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+                    f"-DUWSO_MUTANT={int(mutant)}", "-shared", "-o", str(out), str(ORACLE_DIR / "uws_oracle.cpp")],
+                   check=True)
+    return out
 
 
 class Oracle:
     """The reference step on the CPU; precision 0 = FP32, 1 = FP64."""
 
-    def __init__(self, cf, precision: int = 0):
+    def __init__(self, cf, precision: int = 0, libpath=None):
         self.cf = cf
         cf.Init()
         self._cfg = cf.to_c()
-        self.L = lib()
+        self.L = lib(libpath)
         self.h = self.L.uwso_create(ctypes.byref(self._cfg), precision)
         X, Y, Z = cf.gridRes
         self.grid = (X, Y, Z)
@@ -136,14 +163,14 @@ class Oracle:
         self.L.uwso_run_stage(self.h, int(s))
 
     def field(self, fid):
-        nc = N.FIELD_NCOMP[fid]
+        nc = _NCOMP[fid]
         out = np.empty(self.n * nc, dtype=np.float64)
         assert self.L.uwso_get_field(self.h, fid, out.ctypes.data_as(c_void_p)) == 0
         return out.reshape(self.n, nc) if nc == 3 else out
 
     def set_field(self, fid, a):
         a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
-        assert a.size == self.n * N.FIELD_NCOMP[fid]
+        assert a.size == self.n * _NCOMP[fid]
         assert self.L.uwso_set_field(self.h, fid, a.ctypes.data_as(c_void_p)) == 0
 
     def velocity(self):
@@ -181,6 +208,10 @@ class Oracle:
         out = np.zeros(2)
         self.L.uwso_residuals(self.h, out.ctypes.data_as(c_void_p))
         return out
+
+    def run_kernel(self, k: int):
+        """One kernel alone (KERNEL_*), for the pinning tests."""
+        assert self.L.uwso_run_kernel(self.h, int(k)) == 0
 
     def sweep_counts(self):
         """(momentum, SIMPLE pressure) sweeps run by the last step."""

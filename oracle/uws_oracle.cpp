@@ -53,6 +53,16 @@
 
 namespace {
 
+enum { UWSO_FIELD_VEL_ITER = 100, UWSO_FIELD_PC_ITER = 101 };
+
+// Planted mistakes, one per pinned function, compiled in only with
+// -DUWSO_MUTANT=<n> by the test that shows each pin fails on its mistake
+// (tests/test_oracle_assembled.py); the oracle proper is built without it.
+#ifndef UWSO_MUTANT
+#define UWSO_MUTANT 0
+#endif
+#define MUT(n) (UWSO_MUTANT == (n))
+
 template <class T> struct V3 { T x, y, z; };
 
 template <class T> static inline T sq(T v) { return v * v; }
@@ -241,14 +251,15 @@ template <class T> struct Oracle {
                 const V3<T>& uP = u[idx(x, y, z)];
                 if (InRegionAndIsSolid(x - 1, y, z))
                     fev.x += LogLawWallFunc(std::sqrt(sq(uP.y) + sq(uP.z)));
-                if (InRegionAndIsSolid(x, y - 1, z) || (!InRegion(x, y - 1, z) && velBnd[UWS_Y0] == UWS_VEL_FIXED_VALUE))
+                if (InRegionAndIsSolid(x, y - 1, z) ||
+                    (!MUT(5) && !InRegion(x, y - 1, z) && velBnd[UWS_Y0] == UWS_VEL_FIXED_VALUE))
                     fev.y += LogLawWallFunc(std::sqrt(sq(uP.x) + sq(uP.z)));
                 if (InRegionAndIsSolid(x, y, z - 1))
                     fev.z += LogLawWallFunc(std::sqrt(sq(uP.x) + sq(uP.y)));
             } else if (IsSolid(x, y, z)) {
                 if (InRegionAndIsFluid(x - 1, y, z)) {
                     const V3<T>& uW = u[idx(x - 1, y, z)];
-                    fev.x += LogLawWallFunc(std::sqrt(sq(uW.y) + sq(uW.z)));
+                    fev.x += LogLawWallFunc(std::sqrt(sq(MUT(4) ? uW.x : uW.y) + sq(uW.z)));
                 }
                 if (InRegionAndIsFluid(x, y - 1, z)) {
                     const V3<T>& uS = u[idx(x, y - 1, z)];
@@ -292,6 +303,7 @@ template <class T> struct Oracle {
             return uP;
         } else if (t == UWS_VEL_SYMMETRY) {
             V3<T> v = uP;
+            if (MUT(14)) { if (dir < 2) v.y = 0; else if (dir < 4) v.z = 0; else v.x = 0; return v; }
             if (dir < 2) v.x = 0; else if (dir < 4) v.y = 0; else v.z = 0;
             return v;
         }
@@ -329,7 +341,7 @@ template <class T> struct Oracle {
         if (InRegionAndIsFluid(x + 1, y, z)) vis[0] = nu + (T)0.5f * (eddyVis[P] + eddyVis[idx(x + 1, y, z)]);
         else if (InRegionAndIsSolid(x + 1, y, z)) vis[0] = nu + faceEddyVis[idx(x + 1, y, z)].x;
         if (InRegionAndIsFluid(x - 1, y, z)) vis[1] = nu + (T)0.5f * (eddyVis[P] + eddyVis[idx(x - 1, y, z)]);
-        else if (InRegionAndIsSolid(x - 1, y, z)) vis[1] = nu + faceEddyVis[P].x;
+        else if (InRegionAndIsSolid(x - 1, y, z)) vis[1] = nu + faceEddyVis[MUT(3) ? idx(x - 1, y, z) : P].x;
         if (InRegionAndIsFluid(x, y + 1, z)) vis[2] = nu + (T)0.5f * (eddyVis[P] + eddyVis[idx(x, y + 1, z)]);
         else if (InRegionAndIsSolid(x, y + 1, z)) vis[2] = nu + faceEddyVis[idx(x, y + 1, z)].y;
         if (InRegionAndIsFluid(x, y - 1, z)) vis[3] = nu + (T)0.5f * (eddyVis[P] + eddyVis[idx(x, y - 1, z)]);
@@ -364,9 +376,10 @@ template <class T> struct Oracle {
                     if (IsFluid(nx, ny, nz)) {
                         T gn[3][3];
                         CellGrad(nx, ny, nz, gn);
-                        for (int j = 0; j < 3; ++j) fg[f][j] = (T)0.5f * (g[c][j] + gn[c][j]);
+                        for (int j = 0; j < 3; ++j)
+                            fg[f][j] = MUT(9) ? (T)0.5f * (g[j][c] + gn[j][c]) : (T)0.5f * (g[c][j] + gn[c][j]);
                     } else {
-                        fg[f][c] = plus ? -uPc / hdx : uPc / hdx;
+                        fg[f][c] = (plus != MUT(15)) ? -uPc / hdx : uPc / hdx;
                     }
                 } else {
                     int t = velBnd[bface[f]];
@@ -378,7 +391,7 @@ template <class T> struct Oracle {
                         T ubc = c == 0 ? ub.x : (c == 1 ? ub.y : ub.z);
                         fg[f][c] = plus ? (ubc - uPc) / hdx : (uPc - ubc) / hdx;
                     } else if (t == UWS_VEL_ZERO_GRAD || t == UWS_VEL_SYMMETRY) {
-                        for (int j = 0; j < 3; ++j) if (j != c) fg[f][j] = g[c][j];
+                        for (int j = 0; j < 3; ++j) if (j != c || MUT(16)) fg[f][j] = g[c][j];
                     }
                 }
             }
@@ -387,7 +400,7 @@ template <class T> struct Oracle {
             V3<T> r;
             T* rr[3] = {&r.x, &r.y, &r.z};
             for (int i = 0; i < 3; ++i) {
-                *rr[i] = ds * (vis[0] * fg[0][i] - vis[1] * fg[1][i] + vis[2] * fg[2][i] -
+                *rr[i] = ds * (vis[0] * fg[0][i] - (MUT(10) ? -vis[1] : vis[1]) * fg[1][i] + vis[2] * fg[2][i] -
                                vis[3] * fg[3][i] + vis[4] * fg[4][i] - vis[5] * fg[5][i]);
             }
             bfield[idx(x, y, z)] = r;
@@ -435,7 +448,7 @@ template <class T> struct Oracle {
                         else { bU += ds * ub.x * ubn; bV += ds * ub.y * ubn; bW += ds * ub.z * ubn; }
                     } else if (velBnd[bf] == UWS_VEL_ZERO_GRAD) {
                         T t = ds * uPn;
-                        if (plus) { Dx += t; Dy += t; Dz += t; } else { Dx -= t; Dy -= t; Dz -= t; }
+                        if (plus != MUT(11)) { Dx += t; Dy += t; Dz += t; } else { Dx -= t; Dy -= t; Dz -= t; }
                     }
                 }
             }
@@ -466,7 +479,7 @@ template <class T> struct Oracle {
                         T t = vis[f] * ds / dx;
                         Dx += t; Dy += t; Dz += t;
                     } else {
-                        T t = (T)2 * vis[f] * ds / dx;
+                        T t = (T)(MUT(2) ? 1 : 2) * vis[f] * ds / dx;
                         Dx += t; Dy += t; Dz += t;
                     }
                 } else {
@@ -475,14 +488,14 @@ template <class T> struct Oracle {
                         int i, j;
                         if (bf < 2) { i = y; j = z; } else if (bf < 4) { i = x; j = z; } else { i = x; j = y; }
                         const V3<T>& ub = B(bf, i, j);
-                        bU += (T)2 * vis[f] * ds / dx * ub.x;
+                        if (!MUT(12)) bU += (T)2 * vis[f] * ds / dx * ub.x;
                         bV += (T)2 * vis[f] * ds / dx * ub.y;
                         bW += (T)2 * vis[f] * ds / dx * ub.z;
                         T t = (T)2 * vis[f] * ds / dx;
                         Dx += t; Dy += t; Dz += t;
                     } else if (velBnd[bf] == UWS_VEL_SYMMETRY) {
                         T t = (T)2 * vis[f] * ds / dx;
-                        int c = f / 2;
+                        int c = MUT(13) ? (f / 2 + 1) % 3 : f / 2;
                         if (c == 0) Dx += t; else if (c == 1) Dy += t; else Dz += t;
                     }
                 }
@@ -519,6 +532,7 @@ template <class T> struct Oracle {
                 T tU = ds * (T)0.25f * uk[Q].x * sum;
                 T tV = ds * (T)0.25f * uk[Q].y * sum;
                 T tW = ds * (T)0.25f * uk[Q].z * sum;
+                if (MUT(1)) plus = !plus;
                 if (plus) { UA -= tU; VA -= tV; WA -= tW; } else { UA += tU; VA += tV; WA += tW; }
             }
             T vis[6];
@@ -782,6 +796,7 @@ template <class T> struct Oracle {
                 int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
                 if (!InRegionAndIsFluid(nx, ny, nz)) continue;
                 const int64_t Q = idx(nx, ny, nz);
+                if (MUT(6)) UD += nu * ds / dx * velCorrect[Q].x; else
                 UD -= nu * ds / dx * velCorrect[Q].x;
                 VD -= nu * ds / dx * velCorrect[Q].y;
                 WD -= nu * ds / dx * velCorrect[Q].z;
@@ -815,7 +830,7 @@ template <class T> struct Oracle {
                     presCoeff -= (dv / den) * AdInv / dx;
                     presTerm -= (dv / den) * AdInv / dx * pk[Q];
                 }
-                if (plus) velTerm -= AdInv * AodUCorrect; else velTerm += AdInv * AodUCorrect;
+                if (plus != MUT(7)) velTerm -= AdInv * AodUCorrect; else velTerm += AdInv * AodUCorrect;
             }
             presCorrectField[P] = ((T)1 / presCoeff) * (presTerm + velTerm);
         }
@@ -838,7 +853,7 @@ template <class T> struct Oracle {
             for (int f = 0; f < 6; ++f) {
                 int nx = x + off[f][0], ny = y + off[f][1], nz = z + off[f][2];
                 if (InRegionAndIsFluid(nx, ny, nz)) pf[f] = (T)0.5f * (pc[P] + pc[idx(nx, ny, nz)]);
-                else pf[f] = pc[P];
+                else pf[f] = (MUT(8) && InRegion(nx, ny, nz)) ? (T)0 : pc[P];
             }
             T gx = (pf[0] - pf[1]) / dx, gy = (pf[2] - pf[3]) / dx, gz = (pf[4] - pf[5]) / dx;
             T cx = -DxInv * (AodU[P].x + dv / den * gx);
@@ -1243,9 +1258,13 @@ template <class T> struct Oracle {
         FinishStage();
     }
 
-    // field access for the C API; V3 fields as [N][3]
+    // field access for the C API; V3 fields as [N][3].  Two ids beyond the
+    // library's, for the pinning tests' single kernels: UWSO_FIELD_VEL_ITER
+    // (velFieldLastIter, a momentum sweep's input) and UWSO_FIELD_PC_ITER
+    // (presCorrectFieldLastIter, a pressure sweep's input)
     V3<T>* vfield(int f) {
         switch (f) {
+        case UWSO_FIELD_VEL_ITER: return velFieldLastIter;
         case UWS_FIELD_VEL: return velFieldLastTime;
         case UWS_FIELD_VEL_WORK: return velField;
         case UWS_FIELD_VEL_CORR: return velCorrect.data();
@@ -1258,6 +1277,7 @@ template <class T> struct Oracle {
     }
     T* sfield(int f) {
         switch (f) {
+        case UWSO_FIELD_PC_ITER: return presCorrectFieldLastIter;
         case UWS_FIELD_PRES: return presLastTime.data();
         case UWS_FIELD_PRES_CORR: return presCorrectField;
         case UWS_FIELD_EDDY_VIS: return eddyVis.data();
@@ -1414,6 +1434,23 @@ double uwso_log_law_wall_func(void* hv, double uP) {
     double r = 0;
     dispatch((Handle*)hv, [&](auto& o) { using TT = std::remove_reference_t<decltype(o.dx)>; r = o.LogLawWallFunc((TT)uP); });
     return r;
+}
+/* single kernels, for the oracle's own pinning tests: 0 one momentum Jacobi
+ * sweep (velFieldLastIter -> velField), 1 PISO A_od u' (velCorrect -> AodU),
+ * 2 one PISO pressure sweep (presCorrectFieldLastIter -> presCorrectField),
+ * 3 PISO velocity correction (presCorrectField, AodU -> velCorrect, velField) */
+int uwso_run_kernel(void* hv, int k) {
+    int rc = 0;
+    dispatch((Handle*)hv, [&](auto& o) {
+        switch (k) {
+        case 0: o.VelPredict(); break;
+        case 1: o.PisoCalAodUCorrect(); break;
+        case 2: o.PisoPresCorrection(); break;
+        case 3: o.PisoVelCorrection(); break;
+        default: rc = -1;
+        }
+    });
+    return rc;
 }
 void uwso_residuals(void* hv, double* out2) {
     dispatch((Handle*)hv, [&](auto& o) { out2[0] = o.last_vel_residual; out2[1] = o.last_pres_residual; });
