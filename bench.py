@@ -208,7 +208,7 @@ def ensemble_report(steps: int = 50, check_steps: int = 500) -> dict:
             "max_rel_l2_u_vs_fp64": max(errs), "compared_after_steps": check_steps}
 
 
-def precision_report(steps: int = 500, mean_last: int = 250) -> dict:
+def precision_report(steps: int = 500, mean_last: int = 250, only=None) -> dict:
     """The precision case (384 x 128 high x 256 span, 120 buildings): the
     same run in FP64, FP32 and FP32 with FP16 / BF16 correction storage
     (3 rounds x 20 sweeps of defect correction per pressure solve; the plain
@@ -226,6 +226,8 @@ def precision_report(steps: int = 500, mean_last: int = 250) -> dict:
     X, Y, Z = w.grid
     modes = [("fp64", N.PREC_FP64), ("fp32", N.PREC_FP32), ("fp32_fp16", N.PREC_FP32_FP16),
              ("fp32_bf16", N.PREC_FP32_BF16)]
+    if only:
+        modes = [m for m in modes if m[0] in only]
     out = {"workload": w.name, "grid_xyz": [X, Y, Z], "cells": w.ncells, "buildings": len(w.boxes),
            "steps": steps, "mean_over_last": mean_last, "refine": [w.cf.refineRounds, w.cf.refineSweeps]}
     ref = None
@@ -264,9 +266,9 @@ def precision_report(steps: int = 500, mean_last: int = 250) -> dict:
             del rec["norms_last_step"]["pres_residual"]
             rec["norms_last_step"]["refine_defect_simple_rounds"] = [float(nrm[N.NORM_REFINE0 + r])
                                                                      for r in range(w.cf.refineRounds)]
-        if ref is None:
+        if prec == N.PREC_FP64:
             ref = (u, p, um, rec["norms_last_step"])
-        else:
+        elif ref is not None:
             rec["rel_l2_vs_fp64"] = {"u": rel(u, ref[0]), "p": rel(p, ref[1]), "u_time_mean": rel(um, ref[2])}
             # the momentum residual of the last sweep is rounding-limited (1e-15 in FP64), so only the
             # divergence, and for the plain modes the pressure residual, are compared
@@ -450,6 +452,9 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the precision leg (FP32 / FP16 / BF16 correction storage vs an FP64 run, 500 steps)")
     ap.add_argument("--precision-steps", type=int, default=500)
+    ap.add_argument("--precision-modes", default="",
+                    help="comma list of the precision leg's modes (fp64, fp32, fp32_fp16, fp32_bf16); default all; "
+                         "errors against FP64 need fp64 in the list")
     ap.add_argument("--no-stretched", action="store_true",
                     help="skip the stretched-grid leg (the reference's wind-tunnel box on its stretched mesh)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -569,7 +574,8 @@ def main() -> int:
     if not args.no_stretched:
         out["stretched"] = stretched_report()
     if not args.no_precision:
-        out["precision"] = precision_report(args.precision_steps, args.precision_steps // 2)
+        only = [m for m in args.precision_modes.split(",") if m] or None
+        out["precision"] = precision_report(args.precision_steps, args.precision_steps // 2, only)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_report(args.adjoint_dirs)
     if not args.no_cpu_baseline:

@@ -33,6 +33,8 @@ struct Solver : SolverBase {
     int64_t mean_count = 0;
 
     ~Solver() override {
+        // the stream may still run a replay of a graph: finish it before its executable goes
+        if (stream) (void)hipStreamSynchronize(stream);
         drop_graphs();
         if (mean_sum[0]) (void)hipFree(mean_sum[0]);
     }
