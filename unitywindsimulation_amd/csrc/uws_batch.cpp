@@ -78,8 +78,12 @@ struct uws_batch {
         for (auto& g : groups) g.release();
         groups.clear();
         const int n = (int)cases.size();
+        // the ensemble's launch shape lives in these copies only: each case's
+        // own parameters keep the shape its handle chose for its own launches
+        std::vector<ParamsT<float>> shaped(n);
         for (int c = 0; c < n; ++c) {
-            ParamsT<float>& p = params(c);
+            ParamsT<float>& p = shaped[c];
+            p = params(c);
             p.ty = ty;
             p.tiles = tiles_of(p.X, p.Y, p.Z, ty);
             // one launch serves every case: the full two-sweep tile height
@@ -107,7 +111,7 @@ struct uws_batch {
                     int blocks = 0;
                     bool divides = true;
                     for (int c : g.cases) {
-                        const ParamsT<float>& p = params(c);
+                        const ParamsT<float>& p = shaped[c];
                         blocks += pres_sweep2_grid(p.X, p.Y, p.Z, zc, p.by2);
                         divides = divides && p.Z % zc == 0;
                     }
@@ -115,7 +119,7 @@ struct uws_batch {
                     const int score = (divides ? 2 : 0) + (zc == 16 ? 1 : 0);
                     if (score > sbest) { zbest = zc; sbest = score; }
                 }
-                for (int c : g.cases) params(c).zchunk = zbest;
+                for (int c : g.cases) shaped[c].zchunk = zbest;
             }
             // z chunk of the group's three-sweep launches: the one that
             // minimises rounds of 256 workgroups times planes per workgroup
@@ -125,19 +129,19 @@ struct uws_batch {
                 for (int zc = 4; zc <= 64; ++zc) {
                     int blocks = 0;
                     for (int c : g.cases) {
-                        const ParamsT<float>& p = params(c);
+                        const ParamsT<float>& p = shaped[c];
                         blocks += pres_sweep3_grid(p.X, p.Y, p.Z, zc);
                     }
                     const int cost = ((blocks + 255) / 256) * (zc + 4);
                     if (cbest < 0 || cost < cbest) { zbest = zc; cbest = cost; }
                 }
-                for (int c : g.cases) params(c).zchunk3 = zbest;
+                for (int c : g.cases) shaped[c].zchunk3 = zbest;
             }
             std::vector<int> first(m + 1, 0), first2(m + 1, 0), first3(m + 1, 0);
             std::vector<ParamsT<float>> ps(m);
             std::vector<FieldsT<float>> fs(m);
             for (int i = 0; i < m; ++i) {
-                ps[i] = params(g.cases[i]);
+                ps[i] = shaped[g.cases[i]];
                 fs[i] = fields(g.cases[i]);
                 first[i + 1] = first[i] + ps[i].tiles;
                 first2[i + 1] = first2[i] + pres_sweep2_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk, ps[i].by2);

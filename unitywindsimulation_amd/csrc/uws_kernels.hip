@@ -1521,6 +1521,22 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
             }
         }
     };
+    // the two records below the first computed plane are read only in part:
+    // plane z0 - 4 for the input iterate above it (the D value of plane
+    // z0 - 2), plane z0 - 3 for that and its U coefficient (the D coefficient
+    // of plane z0 - 2); loading nothing else saves their other planes' traffic
+    auto load_below = [&](PS2Plane<T>& q, int zz, bool with_cU) {
+        const uint32_t oP = (uint32_t)min(max(zz, 0), p.Z - 1) * planeB + colB;
+        const uint32_t oU = (uint32_t)min(max(zz + 1, 0), p.Z - 1) * planeB + colB;
+        q.cE = q.cW = q.cN = q.cS = q.inv = q.rhs = 0;
+        q.vE = q.vW = q.vN = q.vS = 0;
+        q.cU = with_cU ? ld_at(k.cU, oP) : (T)0;
+        q.vU = 0;
+        if (!ZIN) {
+            if constexpr (LP == 0) q.vU = ld_at(x0, oU);
+            else q.vU = ld_lp<LP == 2>(p.ein, (int)(oU / (uint32_t)sizeof(T)));
+        }
+    };
     const T sc = LP == 0 ? (T)0 : p.s;
     // one sweep's value from its term, and the value the next sweep reads
     auto comb = [&](T inv, T t, T rhs) {
@@ -1626,15 +1642,17 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
         s2m2 = s2m1;
         s2m1 = s2;
     };
+    // A = plane z0 - 4, B = z0 - 3 (partial records), C = z0 - 2; D is the
+    // record the first step loads plane z0 - 1 into (its plane z0 - 5 value
+    // would only feed a third sweep below the chunk, which never runs)
     PS2Plane<T> A, B, C, D;
-    load(A, z0 - 4);
-    load(B, z0 - 3);
+    load_below(A, z0 - 4, false);
+    load_below(B, z0 - 3, true);
     load(C, z0 - 2);
-    load(D, z0 - 5);
+    D = A;
     // C is masked by its first step
     mask(A, z0 - 4);
     mask(B, z0 - 3);
-    mask(D, z0 - 5);
     using K0 = std::integral_constant<int, 0>;
     using K1 = std::integral_constant<int, 1>;
     using K2 = std::integral_constant<int, 2>;
@@ -1939,7 +1957,17 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
     // records on entry of step(pv, cu, nx, ...): pv = plane zz - 1, cu = plane zz
     VS2Coef<T> A, B, C;
     T V1[3], V2[3], V3[3];
-    loadc(A, z0 - 2);
+    // plane z0 - 2 is read only for its U coefficients (the D coefficients of
+    // plane z0 - 1): its other arrays are not loaded
+    {
+        const uint32_t oP = planeoff(z0 - 2);
+        A.SE = A.SN = A.GE = A.GN = A.SW = A.GW = A.SS = A.GS = 0;
+        A.SU = ld_at(p.S[2], oP);
+        A.GU = ld_at(p.G[2], oP);
+        A.fl = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) A.inv[a] = A.b[a] = A.vO[a] = 0;
+    }
     loadc(B, z0 - 1);
     loadv(V1, z0);
     maskc(A, z0 - 2);  // B is masked by its first step
