@@ -16,13 +16,16 @@
 //   C x   aodu(u^n, u'), PISO coefficients, P sweeps, p += p', u += u'(p', 1/D, aodu)
 //
 // and its reverse walks this list backwards.  The sweeps are reversed by the
-// transposed-sweep kernels; every one-shot kernel by colouring + Dual runs
-// of its own body (uws_adjoint.h).  In-place updates (u += u', p += p') are
-// identities handled here directly.
+// transposed-sweep kernels, every one-shot kernel by a hand-written gather
+// kernel (uws_adjoint_op1.h, uws_adjoint_ops.h) or, with
+// UWS_ADJOINT_REVERSE=colour, by colouring + Dual runs of its own forward body
+// (uws_adjoint.h).  In-place updates (u += u', p += p') are identities
+// handled here directly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -97,6 +100,13 @@ struct Adjoint : AdjointBase {
     V3a lamInvD{}, lamD{}, lamS{}, lamG{}, lamB{}, lamCP{}, lamU0{}, lamUc{}, lamUcNew{}, lamUT{}, lamAO{};
     A *lamCPE = nullptr, *lamInvDp = nullptr, *lamBp = nullptr, *lamIDPI = nullptr, *lamBPI = nullptr;
     A* tcon = nullptr;
+    // the one-shot kernels are reversed by hand (adj_op1, adj_rhie_chow,
+    // adj_correct, adj_piso_coef, adj_aodu) unless UWS_ADJOINT_REVERSE=colour
+    // selects the colouring and Dual runs of their forward bodies (the two
+    // are compared in the tests)
+    bool hand = true;
+    A* op1s[51] = {};  // scratch of the hand-written reverses
+    double* sum_part = nullptr;
     V3a lamBnd[6]{};
     double* lamCs = nullptr;
     // forward-mode mirror of the fields, in the adjoint arithmetic
@@ -160,9 +170,11 @@ struct Adjoint : AdjointBase {
         lamU0 = a3(); lamUc = a3(); lamUcNew = a3(); lamUT = a3(); lamAO = a3();
         lamCPE = pa(); lamInvDp = pa(); lamBp = pa(); lamIDPI = pa(); lamBPI = pa();
         tcon = pa();
+        for (auto& q : op1s) q = pa();
         for (int f = 0; f < 6; ++f)
             for (int c = 0; c < 3; ++c) lamBnd[f][c] = (A*)take((size_t)bw(f) * bh(f) * sizeof(A));
         lamCs = (double*)take(sizeof(double));
+        sum_part = (double*)take(ADJ_SUM_PARTS * sizeof(double));
         return off + 256;
     }
 
@@ -172,6 +184,7 @@ struct Adjoint : AdjointBase {
         K = std::min(k, n);
         nseg = (n + K - 1) / K;
         V = s->sc.V; P = s->sc.P; C = s->sc.C; les = s->sc.les;
+        if (const char* e = std::getenv("UWS_ADJOINT_REVERSE")) hand = std::strcmp(e, "colour") != 0;
         X = s->p.X; Y = s->p.Y; Z = s->p.Z; N = s->p.N;
         st = s->stream;
         device = s->device;
@@ -346,7 +359,7 @@ struct Adjoint : AdjointBase {
             q.cs.d = (A)1;
             go(q);
             dual_contract(pr, tcon, N, st);
-            adj_sum(tcon, N, lamCs, st);
+            adj_sum(tcon, N, lamCs, sum_part, st);
         }
     }
     void pack(const T* src, Dual<A>* dst) { dual_pack<T, A>(src, dst, N, st); }
@@ -387,6 +400,32 @@ struct Adjoint : AdjointBase {
         return r;
     }
 
+    Op1Args<A> op1_args() {
+        Op1Args<A> a{};
+        a.X = X; a.Y = Y; a.Z = Z; a.N = N;
+        a.slots = s->cfg.wall_slots;
+        a.les = les;
+        int k = 0;
+        for (int c = 0; c < 3; ++c) {
+            a.lamS[c] = lamS[c]; a.lamG[c] = lamG[c]; a.lamD[c] = lamD[c];
+            a.lamInvD[c] = lamInvD[c]; a.lamB[c] = lamB[c];
+            a.lamU[c] = lamU0[c];
+        }
+        for (int d = 0; d < 6; ++d) a.lvis[d] = op1s[k++];
+        for (int d = 0; d < 18; ++d) a.lfg[d] = op1s[k++];
+        for (int d = 0; d < 6; ++d) a.ladv[d] = op1s[k++];
+        for (int d = 0; d < 6; ++d) a.lpf[d] = op1s[k++];
+        for (int c = 0; c < 3; ++c) a.lu[c] = op1s[k++];
+        a.lp = op1s[k++];
+        for (int c = 0; c < 9; ++c) a.lg[c] = op1s[k++];
+        a.lnut = op1s[k++];
+        a.lcs = op1s[k++];
+        a.lamP = lamP;
+        for (int f : fixed_faces())
+            for (int c = 0; c < 3; ++c) a.lamBnd[f][c] = lamBnd[f][c];
+        return a;
+    }
+
     int reverse_step(const Tape& tp) {
         const uint8_t* flag = s->f.flag;
         for (auto* v : {&lamInvD, &lamD, &lamS, &lamG, &lamB, &lamCP, &lamU0, &lamUc})
@@ -400,10 +439,21 @@ struct Adjoint : AdjointBase {
             for (int a = 0; a < 3; ++a) adj_add_fluid(flag, lamUc[a], lu[a], lamUT[a], N, st);
             adj_add_fluid<A>(flag, nullptr, lamP, lamPC[0], N, st);
             if (int rc = clear3(lamAO)) return rc;
-            pack(pc_final(tp, c), fd.pc[0]);
-            pack3(tp.invD, fd.invD);
-            pack3(tp.AO[c], fd.aodu);
-            {
+            if (hand) {
+                AdjCorrect<T, A> a{};
+                a.piso = true;
+                a.flag = flag;
+                a.pc = pc_final(tp, c);
+                a.lam_pc = lamPC[0];
+                for (int k = 0; k < 3; ++k) {
+                    a.coef[k] = tp.invD[k]; a.ao[k] = tp.AO[c][k]; a.lam_uc[k] = lamUT[k];
+                    a.lam_coef[k] = lamInvD[k]; a.lam_ao[k] = lamAO[k];
+                }
+                adj_correct(pd, a, st);
+            } else {
+                pack(pc_final(tp, c), fd.pc[0]);
+                pack3(tp.invD, fd.invD);
+                pack3(tp.AO[c], fd.aodu);
                 Pairs<A> pr{};
                 for (int a = 0; a < 3; ++a) { pr.lam[pr.n] = lamUT[a]; pr.d[pr.n++] = fd.ucorr[a]; }
                 std::vector<Seed> sd{{fd.pc[0], lamPC[0], ST_STAR7, -1}};
@@ -415,9 +465,21 @@ struct Adjoint : AdjointBase {
             }
             reverse_pres(tp, c);
             // PISO coefficients (1/diag, rhs) of this corrector
-            pack(tp.cpe, fd.cpisoE);
-            pack3(tp.cp, fd.cp);
-            {
+            if (hand) {
+                AdjPisoCoef<T, A> a{};
+                a.flag = flag;
+                a.cpe = tp.cpe;
+                a.lam_idp = lamIDPI; a.lam_bp = lamBPI;
+                a.lcf = op1s[0];
+                a.lam_cpe = lamCPE;
+                for (int k = 0; k < 3; ++k) {
+                    a.invD[k] = tp.invD[k]; a.ao[k] = tp.AO[c][k]; a.cp[k] = tp.cp[k];
+                    a.lam_invD[k] = lamInvD[k]; a.lam_ao[k] = lamAO[k]; a.lam_cp[k] = lamCP[k];
+                }
+                adj_piso_coef(pd, a, st);
+            } else {
+                pack(tp.cpe, fd.cpisoE);
+                pack3(tp.cp, fd.cp);
                 Pairs<A> pr{};
                 pr.lam[pr.n] = lamIDPI; pr.d[pr.n++] = fd.invDpiso;
                 pr.lam[pr.n] = lamBPI; pr.d[pr.n++] = fd.bpiso;
@@ -433,10 +495,18 @@ struct Adjoint : AdjointBase {
             if (int rc = clear(lamIDPI)) return rc;
             if (int rc = clear(lamBPI)) return rc;
             // aodu(u^n, u' of the previous correction)
-            pack3(tp.U[0], fd.vel[0]);
-            pack3(tp.UC[c - 1], fd.ucorr);
             if (int rc = clear3(lamUcNew)) return rc;
-            {
+            if (hand) {
+                AdjAodu<T, A> a{};
+                a.flag = flag;
+                for (int k = 0; k < 3; ++k) {
+                    a.u[k] = tp.U[0][k]; a.uc[k] = tp.UC[c - 1][k]; a.lam_ao[k] = lamAO[k];
+                    a.lam_u[k] = lamU0[k]; a.lam_uc[k] = lamUcNew[k];
+                }
+                adj_aodu(pd, a, st);
+            } else {
+                pack3(tp.U[0], fd.vel[0]);
+                pack3(tp.UC[c - 1], fd.ucorr);
                 Pairs<A> pr{};
                 for (int a = 0; a < 3; ++a) { pr.lam[pr.n] = lamAO[a]; pr.d[pr.n++] = fd.aodu[a]; }
                 std::vector<Seed> sd;
@@ -451,9 +521,19 @@ struct Adjoint : AdjointBase {
         // ---- SIMPLE correction: u = u* + u'(p', D), p = p^n + p' ----
         for (int a = 0; a < 3; ++a) adj_add_fluid(flag, lamUc[a], lu[a], lamUT[a], N, st);
         if (int rc = clear(lamPC[0])) return rc;
-        pack(pc_final(tp, 0), fd.pc[0]);
-        pack3(tp.D, fd.D);
-        {
+        if (hand) {
+            AdjCorrect<T, A> a{};
+            a.piso = false;
+            a.flag = flag;
+            a.pc = pc_final(tp, 0);
+            a.lam_pc = lamPC[0];
+            for (int k = 0; k < 3; ++k) {
+                a.coef[k] = tp.D[k]; a.lam_uc[k] = lamUT[k]; a.lam_coef[k] = lamD[k];
+            }
+            adj_correct(pd, a, st);
+        } else {
+            pack(pc_final(tp, 0), fd.pc[0]);
+            pack3(tp.D, fd.D);
             Pairs<A> pr{};
             for (int a = 0; a < 3; ++a) { pr.lam[pr.n] = lamUT[a]; pr.d[pr.n++] = fd.ucorr[a]; }
             std::vector<Seed> sd{{fd.pc[0], lamPC[0], ST_STAR7, -1}};
@@ -465,10 +545,24 @@ struct Adjoint : AdjointBase {
         adj_add_fluid(flag, lamPC[0], lamP, lamPC[0], N, st);
         reverse_pres(tp, 0);
         // ---- Rhie-Chow coefficients ----
-        pack3(tp.U[V], fd.vel[1]);
-        pack(tp.P0, fd.p);
-        pack3(tp.invD, fd.invD);
-        {
+        if (hand) {
+            AdjRhieChow<T, A> a{};
+            a.flag = flag;
+            a.p = tp.P0;
+            a.lam_cpe = lamCPE; a.lam_idp = lamInvDp; a.lam_bp = lamBp;
+            a.lam_p = lamP;
+            for (int k = 0; k < 3; ++k) {
+                a.invD[k] = tp.invD[k]; a.D[k] = tp.D[k]; a.lam_cp[k] = lamCP[k];
+                a.lam_u[k] = lu[k]; a.lam_invD[k] = lamInvD[k]; a.lam_D[k] = lamD[k];
+            }
+            for (int d = 0; d < 6; ++d) { a.w[d] = op1s[d]; a.ldi[d] = op1s[6 + d]; }
+            for (int f : fixed_faces())
+                for (int k = 0; k < 3; ++k) a.lam_bnd[f][k] = lamBnd[f][k];
+            adj_rhie_chow(pd, a, st);
+        } else {
+            pack3(tp.U[V], fd.vel[1]);
+            pack(tp.P0, fd.p);
+            pack3(tp.invD, fd.invD);
             Pairs<A> pr{};
             for (int a = 0; a < 3; ++a) { pr.lam[pr.n] = lamCP[a]; pr.d[pr.n++] = fd.cp[a]; }
             pr.lam[pr.n] = lamCPE; pr.d[pr.n++] = fd.cpisoE;
@@ -510,7 +604,21 @@ struct Adjoint : AdjointBase {
         // ---- Op1: wall function, eddy viscosity, LES term, momentum coefficients ----
         pack3(tp.U[0], fd.vel[0]);
         pack(tp.P0, fd.p);
-        {
+        if (hand) {
+            // the primal values in the adjoint arithmetic: the forward bodies
+            // run once on the Dual mirror, every tangent zero
+            Roles r{};
+            r.state = 0;
+            if (les) {
+                r.flags = K_WALL | K_EDDY;
+                dlaunch(KN_WALL_EDDY, pd, r);
+                r.flags = K_FUSE_LES;
+            }
+            dlaunch(KN_VEL_PRECOMP, pd, r);
+            const Op1Args<A> oa = op1_args();
+            adj_op1<A>(pd, fd, oa, st);
+            if (les) adj_sum(oa.lcs, N, lamCs, sum_part, st);
+        } else {
             Pairs<A> pr{};
             for (int a = 0; a < 3; ++a) {
                 pr.lam[pr.n] = lamS[a]; pr.d[pr.n++] = fd.S[a];

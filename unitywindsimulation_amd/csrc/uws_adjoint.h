@@ -7,9 +7,11 @@
 //    also accumulate the adjoints of the coefficients, and
 //  - one-shot local kernels (wall function + eddy viscosity + LES term +
 //    momentum coefficients, Rhie-Chow coefficients, corrections, PISO
-//    coefficients).  The output of such a kernel at cell o depends only on its
-//    inputs in a small stencil around o, so its transposed Jacobian is built
-//    exactly from forward-mode runs of the same kernel body in Dual<T>: the
+//    coefficients), reversed by hand-written gather kernels.  The output of
+//    such a kernel at cell o depends only on its inputs in a small stencil
+//    around o, so its transposed Jacobian can also be built exactly from
+//    forward-mode runs of the same kernel body in Dual<T> (the colouring path
+//    the hand-written reverses are tested against): the
 //    input cells are coloured so that no two cells of one colour reach the
 //    same output, each colour is seeded at once, the outputs' derivatives are
 //    contracted with the output adjoints (t_o = sum_k lam_k(o) dout_k(o)),
@@ -91,6 +93,110 @@ struct Pairs {
     const Dual<T>* d[16];
 };
 
+// Reverse of the momentum coefficients by hand (kernels in uws_adjoint_op1.h):
+// the adjoints of S, G, D, 1/D and b go to u^n, p^n, the boundary textures and
+// (per cell, to be summed) Cs.  The primal values are read from the Dual
+// mirror after one forward run with zero tangents.
+template <class A>
+struct Op1Args {
+    int X, Y, Z, N, slots;
+    bool les;
+    // adjoints of the outputs
+    const A* lamS[3];
+    const A* lamG[3];
+    const A* lamD[3];
+    const A* lamInvD[3];
+    const A* lamB[3];
+    // per-cell intermediate adjoints (written by k_op1_local, completed by k_op1_grad)
+    A* lvis[6];     // face viscosities
+    A* lfg[18];     // LES face gradients [face][component]
+    A* ladv[6];     // advection weight of the neighbour across the face
+    A* lpf[6];      // face pressure
+    A* lu[3];       // own u
+    A* lp;          // own p
+    A* lg[9];       // cell gradient (own part, then the total)
+    A* lnut;        // eddy viscosity (own faces)
+    A* lcs;         // per-cell share of dJ/dCs
+    // accumulated results
+    A* lamU[3];
+    A* lamP;
+    A* lamBnd[6][3];
+};
+template <class A>
+void adj_op1(const ParamsT<Dual<A>>& p, const FieldsT<Dual<A>>& f, const Op1Args<A>& a, hipStream_t s);
+
+// Reverses by hand of the remaining one-shot kernels (uws_adjoint_ops.h), in
+// gather form: one thread per cell R accumulates the adjoints of R's inputs
+// from R's own outputs and from the outputs of the neighbours that read R.
+// Primal records are read in T and used in A; parameters come from the Dual
+// parameter block (.v parts).
+//
+// Velocity correction: SIMPLE u' = -(dv/den) (1/D) (pf+ - pf-) / dx (coef = D)
+// or PISO u' = -(1/D) (aodu + (dv/den) (pf+ - pf-) / dx) (coef = 1/D), with
+// the face values pf of p'.
+template <class T, class A>
+struct AdjCorrect {
+    bool piso;
+    const uint8_t* flag;
+    const T* pc;          // p' the correction reads
+    const T* coef[3];     // D (SIMPLE) or 1/D (PISO)
+    const T* ao[3];       // aodu (PISO)
+    const A* lam_uc[3];   // adjoint of u'
+    A* lam_pc;            // accumulated
+    A* lam_coef[3];
+    A* lam_ao[3];         // PISO
+};
+// PISO coefficients: invDpiso = 1 / presCoeff, bpiso = velTerm
+template <class T, class A>
+struct AdjPisoCoef {
+    const uint8_t* flag;
+    const T* invD[3];
+    const T* ao[3];
+    const T* cpe;
+    const T* cp[3];       // [1], [2]: the N and U face coefficients
+    const A* lam_idp;     // adjoint of invDpiso
+    const A* lam_bp;      // adjoint of bpiso
+    A* lcf;               // scratch: d invDpiso / d(face coefficient) times lam_idp, per cell
+    A* lam_invD[3];       // accumulated
+    A* lam_ao[3];
+    A* lam_cpe;
+    A* lam_cp[3];         // [1], [2]
+};
+// PISO aodu(u^n, u')
+template <class T, class A>
+struct AdjAodu {
+    const uint8_t* flag;
+    const T* u[3];
+    const T* uc[3];
+    const A* lam_ao[3];
+    A* lam_u[3];          // accumulated
+    A* lam_uc[3];
+};
+// Rhie-Chow coefficients (u*, p^n, 1/D, D, boundary textures) -> cp, cpE,
+// 1/Dp, bp
+template <class T, class A>
+struct AdjRhieChow {
+    const uint8_t* flag;
+    const T* p;
+    const T* invD[3];
+    const T* D[3];
+    const A* lam_cp[3];
+    const A* lam_cpe;
+    const A* lam_idp;
+    const A* lam_bp;
+    A* w[6];              // scratch: per face, the adjoint of its (g - ga) scaled by sign / dx
+    A* ldi[6];            // scratch: per face, the adjoint of its mean 1/D
+    A* lam_u[3];          // accumulated
+    A* lam_p;
+    A* lam_invD[3];
+    A* lam_D[3];
+    A* lam_bnd[6][3];     // nullptr: the face's texture is not an input
+};
+template <class T, class A> void adj_correct(const ParamsT<Dual<A>>& p, const AdjCorrect<T, A>& a, hipStream_t s);
+template <class T, class A> void adj_piso_coef(const ParamsT<Dual<A>>& p, const AdjPisoCoef<T, A>& a, hipStream_t s);
+template <class T, class A> void adj_aodu(const ParamsT<Dual<A>>& p, const AdjAodu<T, A>& a, hipStream_t s);
+template <class T, class A> void adj_rhie_chow(const ParamsT<Dual<A>>& p, const AdjRhieChow<T, A>& a, hipStream_t s);
+
 template <class T, class A> void adj_vel_sweep(const AdjVelSweep<T, A>& a, int N, hipStream_t s);
 template <class T, class A> void adj_pres_sweep(const AdjPresSweep<T, A>& a, int N, hipStream_t s);
 // dst[i] = ((A)src[i] or 0, 0)
@@ -110,7 +216,9 @@ void adj_gather_bnd(const T* t, int X, int Y, int Z, int face, const Stencil& st
 // out[i] = (a ? a[i] : 0) + (b and (flag == nullptr or flag[i] is fluid) ? b[i] : 0)
 template <class T>
 void adj_add_fluid(const uint8_t* flag, const T* a, const T* b, T* out, int n, hipStream_t s);
-// acc[0] += sum_i t[i] (one workgroup, fixed order)
-template <class T> void adj_sum(const T* t, int n, double* acc, hipStream_t s);
+// acc[0] += sum_i t[i] in a fixed order: per-workgroup partials into
+// part (ADJ_SUM_PARTS elements), then one workgroup sums them
+constexpr int ADJ_SUM_PARTS = 512;
+template <class T> void adj_sum(const T* t, int n, double* acc, double* part, hipStream_t s);
 
 }  // namespace uws

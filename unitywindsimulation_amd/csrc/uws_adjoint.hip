@@ -1,6 +1,8 @@
 // Kernels of the discrete adjoint (see uws_adjoint.h).  Every kernel is one
 // thread per cell (or boundary element) in gather form: each thread writes
 // only its own cell, so the adjoint is deterministic without atomics.
+#include <algorithm>
+
 #include "uws_adjoint.h"
 
 namespace uws {
@@ -230,18 +232,31 @@ __global__ __launch_bounds__(BLK) void k_adj_add_fluid(const uint8_t* flag, cons
     out[i] = v;
 }
 
+// fixed-order sum: workgroup b of nb sums t[b*256 + j*nb*256 + lane] into
+// part[b], then one workgroup sums part[0..nb) and adds the total to acc
 template <class T>
-__global__ __launch_bounds__(1024) void k_adj_sum(const T* t, int n, double* acc) {
-    __shared__ double part[1024];
+__global__ __launch_bounds__(BLK) void k_adj_sum_part(const T* t, int n, double* part) {
+    __shared__ double red[BLK];
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) s += (double)t[i];
-    part[threadIdx.x] = s;
+    const int stride = (int)gridDim.x * BLK;
+    for (int i = (int)blockIdx.x * BLK + (int)threadIdx.x; i < n; i += stride) s += (double)t[i];
+    red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    for (int w = BLK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) acc[0] += part[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(ADJ_SUM_PARTS) void k_adj_sum_final(const double* part, int nb, double* acc) {
+    __shared__ double red[ADJ_SUM_PARTS];
+    red[threadIdx.x] = (int)threadIdx.x < nb ? part[threadIdx.x] : 0.0;
+    __syncthreads();
+    for (int w = ADJ_SUM_PARTS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += red[0];
 }
 
 }  // namespace
@@ -322,8 +337,10 @@ void adj_add_fluid(const uint8_t* flag, const T* a, const T* b, T* out, int n, h
     hipLaunchKernelGGL(k_adj_add_fluid<T>, dim3(blocks(n)), dim3(BLK), 0, s, flag, a, b, out, n);
 }
 template <class T>
-void adj_sum(const T* t, int n, double* acc, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_sum<T>, dim3(1), dim3(1024), 0, s, t, n, acc);
+void adj_sum(const T* t, int n, double* acc, double* part, hipStream_t s) {
+    const int nb = std::max(1, std::min(ADJ_SUM_PARTS, (n + 8 * BLK - 1) / (8 * BLK)));
+    hipLaunchKernelGGL(k_adj_sum_part<T>, dim3(nb), dim3(BLK), 0, s, t, n, part);
+    hipLaunchKernelGGL(k_adj_sum_final, dim3(1), dim3(ADJ_SUM_PARTS), 0, s, (const double*)part, nb, acc);
 }
 
 // records in float or double, adjoint in double
@@ -339,6 +356,6 @@ template void adj_gather<double>(const double*, int, int, int, const Stencil&, C
 template void adj_gather_bnd<double>(const double*, int, int, int, int, const Stencil&, Colour, int, double*,
                                      hipStream_t);
 template void adj_add_fluid<double>(const uint8_t*, const double*, const double*, double*, int, hipStream_t);
-template void adj_sum<double>(const double*, int, double*, hipStream_t);
+template void adj_sum<double>(const double*, int, double*, double*, hipStream_t);
 
 }  // namespace uws

@@ -22,6 +22,7 @@
 #include <mutex>
 #include <type_traits>
 
+#include "uws_adjoint.h"
 #include "uws_internal.h"
 
 namespace uws {
@@ -2150,7 +2151,52 @@ void go_batch_reduce(const BatchTable<T>& tab, int grid, const Roles& r, hipStre
     hipLaunchKernelGGL((k_batch_reduce<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r);
 }
 
+#include "uws_adjoint_op1.h"
+#include "uws_adjoint_ops.h"
+
 }  // namespace
+
+template <class A>
+void adj_op1(const ParamsT<Dual<A>>& p, const FieldsT<Dual<A>>& f, const Op1Args<A>& a, hipStream_t s) {
+    const int g = (a.N + 255) / 256;
+    hipLaunchKernelGGL(k_op1_local<A>, dim3(g), dim3(256), 0, s, p, f, a);
+    if (a.les) hipLaunchKernelGGL(k_op1_grad<A>, dim3(g), dim3(256), 0, s, p, f, a);
+    hipLaunchKernelGGL(k_op1_state<A>, dim3(g), dim3(256), 0, s, p, f, a);
+}
+template void adj_op1<double>(const ParamsT<Dual<double>>&, const FieldsT<Dual<double>>&, const Op1Args<double>&,
+                              hipStream_t);
+
+template <class T, class A>
+void adj_correct(const ParamsT<Dual<A>>& p, const AdjCorrect<T, A>& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_correct<T, A>), dim3((p.N + 255) / 256), dim3(256), 0, s, p, a);
+}
+template <class T, class A>
+void adj_piso_coef(const ParamsT<Dual<A>>& p, const AdjPisoCoef<T, A>& a, hipStream_t s) {
+    const int g = (p.N + 255) / 256;
+    hipLaunchKernelGGL((k_adj_piso_lcf<T, A>), dim3(g), dim3(256), 0, s, p, a);
+    hipLaunchKernelGGL((k_adj_piso_coef<T, A>), dim3(g), dim3(256), 0, s, p, a);
+}
+template <class T, class A>
+void adj_aodu(const ParamsT<Dual<A>>& p, const AdjAodu<T, A>& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_aodu<T, A>), dim3((p.N + 255) / 256), dim3(256), 0, s, p, a);
+}
+template <class T, class A>
+void adj_rhie_chow(const ParamsT<Dual<A>>& p, const AdjRhieChow<T, A>& a, hipStream_t s) {
+    const int g = (p.N + 255) / 256;
+    hipLaunchKernelGGL((k_adj_rc_local<T, A>), dim3(g), dim3(256), 0, s, p, a);
+    hipLaunchKernelGGL((k_adj_rc_gather<T, A>), dim3(g), dim3(256), 0, s, p, a);
+}
+#define UWS_ADJ_OPS(T)                                                                                     \
+    template void adj_correct<T, double>(const ParamsT<Dual<double>>&, const AdjCorrect<T, double>&,       \
+                                         hipStream_t);                                                     \
+    template void adj_piso_coef<T, double>(const ParamsT<Dual<double>>&, const AdjPisoCoef<T, double>&,   \
+                                           hipStream_t);                                                   \
+    template void adj_aodu<T, double>(const ParamsT<Dual<double>>&, const AdjAodu<T, double>&, hipStream_t); \
+    template void adj_rhie_chow<T, double>(const ParamsT<Dual<double>>&, const AdjRhieChow<T, double>&,   \
+                                           hipStream_t);
+UWS_ADJ_OPS(float)
+UWS_ADJ_OPS(double)
+#undef UWS_ADJ_OPS
 
 #define UWS_KERNEL_CASES(GO, GOR)                                  \
     switch (k) {                                                   \
