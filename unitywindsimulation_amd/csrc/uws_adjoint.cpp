@@ -100,6 +100,9 @@ struct Adjoint : AdjointBase {
     V3a lamInvD{}, lamD{}, lamS{}, lamG{}, lamB{}, lamCP{}, lamU0{}, lamUc{}, lamUcNew{}, lamUT{}, lamAO{};
     A *lamCPE = nullptr, *lamInvDp = nullptr, *lamBp = nullptr, *lamIDPI = nullptr, *lamBPI = nullptr;
     A* tcon = nullptr;
+    // adjoints of the iterates of a window of transposed sweeps
+    A* pwin[ADJ_WIN + 1] = {};
+    V3a vwin[ADJ_WIN + 1]{};
     // the one-shot kernels are reversed by hand (adj_op1, adj_rhie_chow,
     // adj_correct, adj_piso_coef, adj_aodu) unless UWS_ADJOINT_REVERSE=colour
     // selects the colouring and Dual runs of their forward bodies (the two
@@ -170,6 +173,8 @@ struct Adjoint : AdjointBase {
         lamU0 = a3(); lamUc = a3(); lamUcNew = a3(); lamUT = a3(); lamAO = a3();
         lamCPE = pa(); lamInvDp = pa(); lamBp = pa(); lamIDPI = pa(); lamBPI = pa();
         tcon = pa();
+        for (auto& w : pwin) w = pa();
+        for (auto& w : vwin) w = a3();
         for (auto& q : op1s) q = pa();
         for (int f = 0; f < 6; ++f)
             for (int c = 0; c < 3; ++c) lamBnd[f][c] = (A*)take((size_t)bw(f) * bh(f) * sizeof(A));
@@ -368,7 +373,10 @@ struct Adjoint : AdjointBase {
     }
     void dlaunch(Kernel k, const ParamsT<Dual<A>>& q, const Roles& r) { launch<Dual<A>>(k, q, fd, r, st); }
 
-    // transposed pressure solve from the adjoint of its last iterate in lamPC[0]
+    // transposed pressure solve from the adjoint of its last iterate in
+    // lamPC[0]: sweeps P-1 .. 0 in windows of ADJ_WIN; within a window each
+    // sweep's input adjoint goes to a fresh slot, then one coefficient pass
+    // covers the window; the last slot carries over to the next window
     void reverse_pres(const Tape& tp, int solve) {
         AdjPresSweep<T, A> a{};
         a.X = X; a.Y = Y; a.Z = Z;
@@ -382,13 +390,24 @@ struct Adjoint : AdjointBase {
         }
         a.cN = tp.cp[1]; a.cU = tp.cp[2];
         a.lam_cN = lamCP[1]; a.lam_cU = lamCP[2];
-        int pc = 0;
-        for (int k = P - 1; k >= 0; --k) {
-            a.in = k > 0 ? tp.PS[solve][k - 1] : nullptr;
-            a.lam_out = lamPC[pc];
-            a.lam_in = lamPC[1 - pc];
-            adj_pres_sweep(a, N, st);
-            pc = 1 - pc;
+        const A* cur = lamPC[0];
+        int ci = ADJ_WIN;  // slot of cur (lamPC[0] stands outside the slots)
+        for (int k = P - 1; k >= 0;) {
+            SweepWindow<T, A> w{};
+            w.n = std::min(ADJ_WIN, k + 1);
+            for (int j = 0; j < w.n; ++j, --k) {
+                const T* xin = k > 0 ? tp.PS[solve][k - 1] : nullptr;
+                w.lo[j][0] = cur;
+                w.in[j][0] = xin;
+                if (!xin) continue;  // the zero first iterate: its adjoint is not needed
+                const int slot = (ci + 1 + j) % (ADJ_WIN + 1);
+                a.lam_out = cur;
+                a.lam_in = pwin[slot];
+                adj_pres_lam(a, N, st);
+                cur = pwin[slot];
+            }
+            adj_pres_coef(a, w, N, st);
+            ci = (ci + w.n) % (ADJ_WIN + 1);
         }
     }
 
@@ -589,15 +608,28 @@ struct Adjoint : AdjointBase {
                 a.S[c] = tp.S[c]; a.G[c] = tp.G[c]; a.invD[c] = tp.invD[c]; a.b[c] = tp.b[c];
                 a.lam_S[c] = lamS[c]; a.lam_G[c] = lamG[c]; a.lam_invD[c] = lamInvD[c]; a.lam_b[c] = lamB[c];
             }
-            for (int k = V - 1; k >= 0; --k) {
-                for (int c = 0; c < 3; ++c) {
-                    a.in[c] = tp.U[k][c];
-                    a.lam_out[c] = lamU[cur][c];
-                    a.lam_in[c] = lamU[1 - cur][c];
+            // sweeps V-1 .. 0 in windows, as for the pressure; the adjoint of
+            // u^0 (the last sweep's input) goes to lamU[1 - cur]
+            const V3a* lo = &lamU[cur];
+            int ci = ADJ_WIN;
+            for (int k = V - 1; k >= 0;) {
+                SweepWindow<T, A> w{};
+                w.n = std::min(ADJ_WIN, k + 1);
+                for (int j = 0; j < w.n; ++j, --k) {
+                    const V3a* li = k > 0 ? &vwin[(ci + 1 + j) % (ADJ_WIN + 1)] : &lamU[1 - cur];
+                    for (int c = 0; c < 3; ++c) {
+                        w.lo[j][c] = (*lo)[c];
+                        w.in[j][c] = tp.U[k][c];
+                        a.lam_out[c] = (*lo)[c];
+                        a.lam_in[c] = (*li)[c];
+                    }
+                    adj_vel_lam(a, N, st);
+                    lo = li;
                 }
-                adj_vel_sweep(a, N, st);
-                cur = 1 - cur;
+                adj_vel_coef(a, w, N, st);
+                ci = (ci + w.n) % (ADJ_WIN + 1);
             }
+            if (V > 0) cur = 1 - cur;
         }
         // lamU0 += adjoint of u^0 from the sweeps (every cell)
         for (int a = 0; a < 3; ++a) adj_add_fluid<A>(nullptr, lamU0[a], lamU[cur][a], lamU0[a], N, st);

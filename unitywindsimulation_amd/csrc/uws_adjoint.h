@@ -42,10 +42,9 @@ struct AdjVelSweep {
     const T* G[3];
     const T* invD[3];
     const T* b[3];
-    const T* in[3];               // the sweep's input iterate
-    const A* lam_out[3];          // adjoint of the sweep's output
-    A* lam_in[3];                 // adjoint of the input iterate (written)
-    A* lam_S[3];                  // accumulated
+    const A* lam_out[3];          // adj_vel_lam: adjoint of the sweep's output
+    A* lam_in[3];                 // adj_vel_lam: adjoint of the input iterate (written)
+    A* lam_S[3];                  // accumulated by adj_vel_coef
     A* lam_G[3];
     A* lam_invD[3];
     A* lam_b[3];
@@ -62,10 +61,9 @@ struct AdjPresSweep {
     const T* cU;
     const T* invDp;
     const T* rhs;
-    const T* in;
-    const A* lam_out;
-    A* lam_in;                    // written when in != nullptr
-    A* lam_cE;                    // accumulated
+    const A* lam_out;             // adj_pres_lam's input
+    A* lam_in;                    // adj_pres_lam's output
+    A* lam_cE;                    // accumulated by adj_pres_coef
     A* lam_cN;
     A* lam_cU;
     A* lam_invDp;
@@ -197,8 +195,24 @@ template <class T, class A> void adj_piso_coef(const ParamsT<Dual<A>>& p, const 
 template <class T, class A> void adj_aodu(const ParamsT<Dual<A>>& p, const AdjAodu<T, A>& a, hipStream_t s);
 template <class T, class A> void adj_rhie_chow(const ParamsT<Dual<A>>& p, const AdjRhieChow<T, A>& a, hipStream_t s);
 
-template <class T, class A> void adj_vel_sweep(const AdjVelSweep<T, A>& a, int N, hipStream_t s);
-template <class T, class A> void adj_pres_sweep(const AdjPresSweep<T, A>& a, int N, hipStream_t s);
+// A transposed sweep runs in two parts: adj_*_lam writes lam_in from
+// lam_out (the adjoint of the iterate, every cell); adj_*_coef adds the
+// coefficient adjoints of up to ADJ_WIN consecutive sweeps at once (their
+// output adjoints and input iterates in the window), accumulated in registers
+// so that each coefficient adjoint is read and written once per window.
+constexpr int ADJ_WIN = 16;
+template <class T, class A>
+struct SweepWindow {
+    int n;
+    const A* lo[ADJ_WIN][3];      // adjoint of each sweep's output ([0] only: pressure)
+    const T* in[ADJ_WIN][3];      // each sweep's input iterate (pressure: nullptr is the zero iterate)
+};
+template <class T, class A> void adj_vel_lam(const AdjVelSweep<T, A>& a, int N, hipStream_t s);
+template <class T, class A>
+void adj_vel_coef(const AdjVelSweep<T, A>& a, const SweepWindow<T, A>& w, int N, hipStream_t s);
+template <class T, class A> void adj_pres_lam(const AdjPresSweep<T, A>& a, int N, hipStream_t s);
+template <class T, class A>
+void adj_pres_coef(const AdjPresSweep<T, A>& a, const SweepWindow<T, A>& w, int N, hipStream_t s);
 // dst[i] = ((A)src[i] or 0, 0)
 template <class T, class A> void dual_pack(const T* src, Dual<A>* dst, int n, hipStream_t s);
 // dst[i].d = 1 on the cells of colour k, 0 elsewhere (k < 0: 0 everywhere);

@@ -12,31 +12,39 @@ constexpr int BLK = 256;
 inline int blocks(int n) { return (n + BLK - 1) / BLK; }
 
 __device__ __forceinline__ int gid() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
+// the cell of a thread of a stencil kernel: workgroup ids are remapped so
+// that XCD j (ids j, j+8, ...) walks a contiguous range of cells, and the
+// neighbouring planes a workgroup reads stay in that XCD's L2
+__device__ __forceinline__ int gid_xcd() {
+    const int B = (int)gridDim.x, q = B >> 3, r = B & 7;
+    const int i = (int)blockIdx.x, xcd = i & 7;
+    return (xcd * q + min(xcd, r) + (i >> 3)) * (int)blockDim.x + (int)threadIdx.x;
+}
 
 // ---------------------------------------------------------------------------
-// Transposed momentum sweep.  For cell Q and component a:
+// Transposed momentum sweep, in two parts.  k_adj_vel_lam: for cell Q and
+// component a,
 //   lam_in(Q) = sum over the cells P that read Q of dout(P)/du(Q) lam_out(P)
 // with dout(P)/du(E of P) = invD(P) (-q S_E(P) + G_E(P)) and
 // dout(P)/du(W of P) = invD(P) (q S_E(W) + G_E(W)) (and alike in y, z);
 // coefficients of faces that are not fluid-fluid are zero, and invD is zero
-// off the fluid, so no flags are needed for this part.
+// off the fluid, so no flags are needed for this part.  k_adj_vel_coef: the
+// adjoints of S, G, 1/D and b summed over a window of sweeps in registers
+// and added once.
 // ---------------------------------------------------------------------------
 template <class T, class A>
-__global__ __launch_bounds__(BLK) void k_adj_vel_sweep(AdjVelSweep<T, A> a, int N) {
-    const int i = gid();
+__global__ __launch_bounds__(BLK) void k_adj_vel_lam(AdjVelSweep<T, A> a, int N) {
+    const int i = gid_xcd();
     if (i >= N) return;
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int x = i % X, t = i / X, y = t % Y, z = t / Y;
     const int sy = X, sz = X * Y;
     const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
-    const bool isF = a.flag[i] == UWS_CELL_FLUID;
     const A q = a.q;
-    A rP[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const A* lo = a.lam_out[c];
         const T* iv = a.invD[c];
-        rP[c] = lo[i] * (A)iv[i];
         A acc = 0;
         if (hW) acc += lo[i - 1] * (A)iv[i - 1] * (-q * (A)a.S[0][i - 1] + (A)a.G[0][i - 1]);
         if (hE) acc += lo[i + 1] * (A)iv[i + 1] * (q * (A)a.S[0][i] + (A)a.G[0][i]);
@@ -46,64 +54,103 @@ __global__ __launch_bounds__(BLK) void k_adj_vel_sweep(AdjVelSweep<T, A> a, int 
         if (hU) acc += lo[i + sz] * (A)iv[i + sz] * (q * (A)a.S[2][i] + (A)a.G[2][i]);
         a.lam_in[c][i] = acc;
     }
-    if (isF) {
-        // the bracket A + Df + b of the forward sweep, on the input iterate
-        const int iE = hE ? i + 1 : i, iW = hW ? i - 1 : i, iN = hN ? i + sy : i, iS = hS ? i - sy : i;
-        const int iU = hU ? i + sz : i, iD = hD ? i - sz : i;
-        const A SE = a.S[0][i], SN = a.S[1][i], SU = a.S[2][i];
-        const A GE = a.G[0][i], GN = a.G[1][i], GU = a.G[2][i];
-        const A SW = hW ? (A)a.S[0][iW] : (A)0, GW = hW ? (A)a.G[0][iW] : (A)0;
-        const A SS = hS ? (A)a.S[1][iS] : (A)0, GS = hS ? (A)a.G[1][iS] : (A)0;
-        const A SD = hD ? (A)a.S[2][iD] : (A)0, GD = hD ? (A)a.G[2][iD] : (A)0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const T* k = a.in[c];
-            const A vE = k[iE], vW = k[iW], vN = k[iN], vS = k[iS], vU = k[iU], vD = k[iD];
-            A Av = 0;
-            Av -= q * vE * SE;
-            Av += q * vW * SW;
-            Av -= q * vN * SN;
-            Av += q * vS * SS;
-            Av -= q * vU * SU;
-            Av += q * vD * SD;
-            A Df = 0;
-            Df += GE * vE;
-            Df += GW * vW;
-            Df += GN * vN;
-            Df += GS * vS;
-            Df += GU * vU;
-            Df += GD * vD;
-            a.lam_invD[c][i] += a.lam_out[c][i] * (Av + Df + (A)a.b[c][i]);
-            a.lam_b[c][i] += rP[c];
-        }
-    }
-    // owned faces E, N, U: S and G enter out(P) and out(P + e)
+}
+
+template <class T, class A>
+__global__ __launch_bounds__(BLK) void k_adj_vel_coef(AdjVelSweep<T, A> a, SweepWindow<T, A> w, int N) {
+    const int i = gid_xcd();
+    if (i >= N) return;
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const int x = i % X, t = i / X, y = t % Y, z = t / Y;
+    const int sy = X, sz = X * Y;
+    const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
+    const bool isF = a.flag[i] == UWS_CELL_FLUID;
+    const A q = a.q;
+    const int iE = hE ? i + 1 : i, iW = hW ? i - 1 : i, iN = hN ? i + sy : i, iS = hS ? i - sy : i;
+    const int iU = hU ? i + sz : i, iD = hD ? i - sz : i;
+    const A SE = a.S[0][i], SN = a.S[1][i], SU = a.S[2][i];
+    const A GE = a.G[0][i], GN = a.G[1][i], GU = a.G[2][i];
+    const A SW = hW ? (A)a.S[0][iW] : (A)0, GW = hW ? (A)a.G[0][iW] : (A)0;
+    const A SS = hS ? (A)a.S[1][iS] : (A)0, GS = hS ? (A)a.G[1][iS] : (A)0;
+    const A SD = hD ? (A)a.S[2][iD] : (A)0, GD = hD ? (A)a.G[2][iD] : (A)0;
     const int offs[3] = {1, sy, sz};
     const bool has[3] = {hE, hN, hU};
+    A ivP[3], ivQ[3][3], bP[3];
 #pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        if (!has[e]) continue;
-        const int j = i + offs[e];
-        A sS = 0, sG = 0;
+    for (int c = 0; c < 3; ++c) {
+        ivP[c] = a.invD[c][i];
+        bP[c] = a.b[c][i];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) ivQ[e][c] = has[e] ? (A)a.invD[c][i + offs[e]] : (A)0;
+    }
+    A lInv[3] = {0, 0, 0}, lB[3] = {0, 0, 0}, lS[3] = {0, 0, 0}, lG[3] = {0, 0, 0};
+#pragma unroll 4
+    for (int j = 0; j < w.n; ++j) {
+        A rP[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const A rQ = a.lam_out[c][j] * (A)a.invD[c][j];
-            const A uP = a.in[c][i], uQ = a.in[c][j];
-            sS += rP[c] * (-q * uQ) + rQ * (q * uP);
-            sG += rP[c] * uQ + rQ * uP;
+            const A lo = w.lo[j][c][i];
+            rP[c] = lo * ivP[c];
+            if (isF) {
+                // the bracket A + Df + b of the forward sweep, on its input iterate
+                const T* k = w.in[j][c];
+                const A vE = k[iE], vW = k[iW], vN = k[iN], vS = k[iS], vU = k[iU], vD = k[iD];
+                A Av = 0;
+                Av -= q * vE * SE;
+                Av += q * vW * SW;
+                Av -= q * vN * SN;
+                Av += q * vS * SS;
+                Av -= q * vU * SU;
+                Av += q * vD * SD;
+                A Df = 0;
+                Df += GE * vE;
+                Df += GW * vW;
+                Df += GN * vN;
+                Df += GS * vS;
+                Df += GU * vU;
+                Df += GD * vD;
+                lInv[c] += lo * (Av + Df + bP[c]);
+                lB[c] += rP[c];
+            }
         }
-        a.lam_S[e][i] += sS;
-        a.lam_G[e][i] += sG;
+        // owned faces E, N, U: S and G enter out(P) and out(P + e)
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            if (!has[e]) continue;
+            const int jq = i + offs[e];
+            A sS = 0, sG = 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const A rQ = w.lo[j][c][jq] * ivQ[e][c];
+                const A uP = w.in[j][c][i], uQ = w.in[j][c][jq];
+                sS += rP[c] * (-q * uQ) + rQ * (q * uP);
+                sG += rP[c] * uQ + rQ * uP;
+            }
+            lS[e] += sS;
+            lG[e] += sG;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (isF) {
+            a.lam_invD[c][i] += lInv[c];
+            a.lam_b[c][i] += lB[c];
+        }
+        if (has[c]) {
+            a.lam_S[c][i] += lS[c];
+            a.lam_G[c][i] += lG[c];
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Transposed pressure sweep.  out(P) = invDp(P) (term(P) + rhs(P)) with
-// term(P) = -sum_f c_f x(nb); c_f of a face is stored on the cell owning it.
+// Transposed pressure sweep, in the same two parts.  out(P) = invDp(P)
+// (term(P) + rhs(P)) with term(P) = -sum_f c_f x(nb); c_f of a face is
+// stored on the cell owning it.
 // ---------------------------------------------------------------------------
 template <class T, class A>
-__global__ __launch_bounds__(BLK) void k_adj_pres_sweep(AdjPresSweep<T, A> a, int N) {
-    const int i = gid();
+__global__ __launch_bounds__(BLK) void k_adj_pres_lam(AdjPresSweep<T, A> a, int N) {
+    const int i = gid_xcd();
     if (i >= N) return;
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int x = i % X, t = i / X, y = t % Y, z = t / Y;
@@ -111,39 +158,64 @@ __global__ __launch_bounds__(BLK) void k_adj_pres_sweep(AdjPresSweep<T, A> a, in
     const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
     const A* lo = a.lam_out;
     const T* iv = a.invDp;
-    const A rP = lo[i] * (A)iv[i];
-    const T* xin = a.in;
-    if (xin) {
-        A acc = 0;
-        if (hW) acc -= lo[i - 1] * (A)iv[i - 1] * (A)a.cE[i - 1];
-        if (hE) acc -= lo[i + 1] * (A)iv[i + 1] * (A)a.cE[i];
-        if (hS) acc -= lo[i - sy] * (A)iv[i - sy] * (A)a.cN[i - sy];
-        if (hN) acc -= lo[i + sy] * (A)iv[i + sy] * (A)a.cN[i];
-        if (hD) acc -= lo[i - sz] * (A)iv[i - sz] * (A)a.cU[i - sz];
-        if (hU) acc -= lo[i + sz] * (A)iv[i + sz] * (A)a.cU[i];
-        a.lam_in[i] = acc;
-    }
-    if (a.flag[i] == UWS_CELL_FLUID) {
-        A term = 0;
-        if (xin) {
-            const int iE = hE ? i + 1 : i, iW = hW ? i - 1 : i, iN = hN ? i + sy : i, iS = hS ? i - sy : i;
-            const int iU = hU ? i + sz : i, iD = hD ? i - sz : i;
-            const A cW = hW ? (A)a.cE[iW] : (A)0, cS = hS ? (A)a.cN[iS] : (A)0, cD = hD ? (A)a.cU[iD] : (A)0;
-            term -= (A)a.cE[i] * (A)xin[iE];
-            term -= cW * (A)xin[iW];
-            term -= (A)a.cN[i] * (A)xin[iN];
-            term -= cS * (A)xin[iS];
-            term -= (A)a.cU[i] * (A)xin[iU];
-            term -= cD * (A)xin[iD];
+    A acc = 0;
+    if (hW) acc -= lo[i - 1] * (A)iv[i - 1] * (A)a.cE[i - 1];
+    if (hE) acc -= lo[i + 1] * (A)iv[i + 1] * (A)a.cE[i];
+    if (hS) acc -= lo[i - sy] * (A)iv[i - sy] * (A)a.cN[i - sy];
+    if (hN) acc -= lo[i + sy] * (A)iv[i + sy] * (A)a.cN[i];
+    if (hD) acc -= lo[i - sz] * (A)iv[i - sz] * (A)a.cU[i - sz];
+    if (hU) acc -= lo[i + sz] * (A)iv[i + sz] * (A)a.cU[i];
+    a.lam_in[i] = acc;
+}
+
+template <class T, class A>
+__global__ __launch_bounds__(BLK) void k_adj_pres_coef(AdjPresSweep<T, A> a, SweepWindow<T, A> w, int N) {
+    const int i = gid_xcd();
+    if (i >= N) return;
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const int x = i % X, t = i / X, y = t % Y, z = t / Y;
+    const int sy = X, sz = X * Y;
+    const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
+    const bool isF = a.flag[i] == UWS_CELL_FLUID;
+    const int iE = hE ? i + 1 : i, iW = hW ? i - 1 : i, iN = hN ? i + sy : i, iS = hS ? i - sy : i;
+    const int iU = hU ? i + sz : i, iD = hD ? i - sz : i;
+    const A cE = a.cE[i], cN = a.cN[i], cU = a.cU[i];
+    const A cW = hW ? (A)a.cE[iW] : (A)0, cS = hS ? (A)a.cN[iS] : (A)0, cD = hD ? (A)a.cU[iD] : (A)0;
+    const A ivP = a.invDp[i], rhs = a.rhs[i];
+    const A ivE = hE ? (A)a.invDp[iE] : (A)0, ivN = hN ? (A)a.invDp[iN] : (A)0, ivU = hU ? (A)a.invDp[iU] : (A)0;
+    A lInv = 0, lRhs = 0, lE = 0, lN = 0, lU = 0;
+#pragma unroll 4
+    for (int j = 0; j < w.n; ++j) {
+        const A* lo = w.lo[j][0];
+        const T* xin = w.in[j][0];
+        const A l = lo[i], rP = l * ivP;
+        if (isF) {
+            A term = 0;
+            if (xin) {
+                term -= cE * (A)xin[iE];
+                term -= cW * (A)xin[iW];
+                term -= cN * (A)xin[iN];
+                term -= cS * (A)xin[iS];
+                term -= cU * (A)xin[iU];
+                term -= cD * (A)xin[iD];
+            }
+            lInv += l * (term + rhs);
+            lRhs += rP;
         }
-        a.lam_invDp[i] += lo[i] * (term + (A)a.rhs[i]);
-        a.lam_rhs[i] += rP;
+        if (xin) {
+            const A xP = xin[i];
+            if (hE) lE -= rP * (A)xin[iE] + lo[iE] * ivE * xP;
+            if (hN) lN -= rP * (A)xin[iN] + lo[iN] * ivN * xP;
+            if (hU) lU -= rP * (A)xin[iU] + lo[iU] * ivU * xP;
+        }
     }
-    if (xin) {
-        if (hE) a.lam_cE[i] -= rP * (A)xin[i + 1] + lo[i + 1] * (A)iv[i + 1] * (A)xin[i];
-        if (hN) a.lam_cN[i] -= rP * (A)xin[i + sy] + lo[i + sy] * (A)iv[i + sy] * (A)xin[i];
-        if (hU) a.lam_cU[i] -= rP * (A)xin[i + sz] + lo[i + sz] * (A)iv[i + sz] * (A)xin[i];
+    if (isF) {
+        a.lam_invDp[i] += lInv;
+        a.lam_rhs[i] += lRhs;
     }
+    if (hE) a.lam_cE[i] += lE;
+    if (hN) a.lam_cN[i] += lN;
+    if (hU) a.lam_cU[i] += lU;
 }
 
 // ---------------------------------------------------------------------------
@@ -302,12 +374,20 @@ Colour colour_of(int kind) {
 Colour colour_bnd(int kind) { return kind == ST_OWN ? Colour{1, 0, 0} : Colour{5, 2, 0}; }
 
 template <class T, class A>
-void adj_vel_sweep(const AdjVelSweep<T, A>& a, int N, hipStream_t s) {
-    hipLaunchKernelGGL((k_adj_vel_sweep<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+void adj_vel_lam(const AdjVelSweep<T, A>& a, int N, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_vel_lam<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, N);
 }
 template <class T, class A>
-void adj_pres_sweep(const AdjPresSweep<T, A>& a, int N, hipStream_t s) {
-    hipLaunchKernelGGL((k_adj_pres_sweep<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+void adj_vel_coef(const AdjVelSweep<T, A>& a, const SweepWindow<T, A>& w, int N, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_vel_coef<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, w, N);
+}
+template <class T, class A>
+void adj_pres_lam(const AdjPresSweep<T, A>& a, int N, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_pres_lam<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+}
+template <class T, class A>
+void adj_pres_coef(const AdjPresSweep<T, A>& a, const SweepWindow<T, A>& w, int N, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_pres_coef<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, w, N);
 }
 template <class T, class A>
 void dual_pack(const T* src, Dual<A>* dst, int n, hipStream_t s) {
@@ -344,10 +424,16 @@ void adj_sum(const T* t, int n, double* acc, double* part, hipStream_t s) {
 }
 
 // records in float or double, adjoint in double
-template void adj_vel_sweep<float, double>(const AdjVelSweep<float, double>&, int, hipStream_t);
-template void adj_vel_sweep<double, double>(const AdjVelSweep<double, double>&, int, hipStream_t);
-template void adj_pres_sweep<float, double>(const AdjPresSweep<float, double>&, int, hipStream_t);
-template void adj_pres_sweep<double, double>(const AdjPresSweep<double, double>&, int, hipStream_t);
+#define UWS_ADJ_SWEEPS(T)                                                                                  \
+    template void adj_vel_lam<T, double>(const AdjVelSweep<T, double>&, int, hipStream_t);                \
+    template void adj_vel_coef<T, double>(const AdjVelSweep<T, double>&, const SweepWindow<T, double>&, int, \
+                                          hipStream_t);                                                    \
+    template void adj_pres_lam<T, double>(const AdjPresSweep<T, double>&, int, hipStream_t);              \
+    template void adj_pres_coef<T, double>(const AdjPresSweep<T, double>&, const SweepWindow<T, double>&,   \
+                                           int, hipStream_t);
+UWS_ADJ_SWEEPS(float)
+UWS_ADJ_SWEEPS(double)
+#undef UWS_ADJ_SWEEPS
 template void dual_pack<float, double>(const float*, Dual<double>*, int, hipStream_t);
 template void dual_pack<double, double>(const double*, Dual<double>*, int, hipStream_t);
 template void dual_seed<double>(Dual<double>*, int, int, int, Colour, int, hipStream_t);
