@@ -22,8 +22,8 @@ METRIC = "MLUPS per PISO step + %HBM peak; batched vs plain cases/s at ≤1e-3 r
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per cell per launch read by rocprofv3 FETCH_SIZE (x2, calibrated on a
 # 256 MiB copy) + WRITE_SIZE on the urban case: profiles/urban_256x96x128_hbm_traffic.md
-MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 75.5, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.2,
-                                         "k_pres_sweep3": 37.9, "k_vel_sweep2": 91.1}}
+MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 76.0, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.2,
+                                         "k_pres_sweep3": 35.4, "k_vel_sweep2": 87.8}}
 
 
 def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
