@@ -27,6 +27,7 @@
 #include <array>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <new>
 #include <vector>
@@ -227,10 +228,17 @@ struct Adjoint : AdjointBase {
     int clear(E* dst, size_t n = 0) {
         return hipMemsetAsync(dst, 0, (n ? n : (size_t)N) * sizeof(E), st) == hipSuccess ? UWS_OK : UWS_ERR_DEVICE;
     }
-    int clear3(const V3a& v) {
-        for (int c = 0; c < 3; ++c)
-            if (int rc = clear(v[c])) return rc;
-        return UWS_OK;
+    // zero cell-sized adjoint arrays in one launch
+    void zero_all(std::initializer_list<A*> arrays) {
+        ZeroList<A> z{};
+        for (A* a : arrays) {
+            if (z.m == ADJ_ZERO_MAX) {
+                adj_zero(z, N, st);
+                z.m = 0;
+            }
+            z.p[z.m++] = a;
+        }
+        adj_zero(z, N, st);
     }
     void run(Kernel k, const FieldsT<T>& f, const Roles& r) { launch<T>(k, s->p, f, r, st); }
 
@@ -447,17 +455,18 @@ struct Adjoint : AdjointBase {
 
     int reverse_step(const Tape& tp) {
         const uint8_t* flag = s->f.flag;
-        for (auto* v : {&lamInvD, &lamD, &lamS, &lamG, &lamB, &lamCP, &lamU0, &lamUc})
-            if (int rc = clear3(*v)) return rc;
-        for (A* v : {lamCPE, lamInvDp, lamBp, lamIDPI, lamBPI})
-            if (int rc = clear(v)) return rc;
+        zero_all({lamInvD[0], lamInvD[1], lamInvD[2], lamD[0], lamD[1], lamD[2], lamS[0], lamS[1], lamS[2],
+              lamG[0], lamG[1], lamG[2], lamB[0], lamB[1], lamB[2], lamCP[0], lamCP[1], lamCP[2],
+              lamU0[0], lamU0[1], lamU0[2], lamUc[0], lamUc[1], lamUc[2], lamCPE, lamInvDp, lamBp});
         V3a& lu = lamU[cur];
         // ---- PISO correctors, last first ----
         for (int c = C; c >= 1; --c) {
             // u += u'(p', 1/D, aodu), p += p' (fluid cells)
             for (int a = 0; a < 3; ++a) adj_add_fluid(flag, lamUc[a], lu[a], lamUT[a], N, st);
             adj_add_fluid<A>(flag, nullptr, lamP, lamPC[0], N, st);
-            if (int rc = clear3(lamAO)) return rc;
+            // this corrector's accumulators (lamUcNew is first written by the
+            // aodu reverse below; its old contents are the previous lamUc)
+            zero_all({lamAO[0], lamAO[1], lamAO[2], lamIDPI, lamBPI, lamUcNew[0], lamUcNew[1], lamUcNew[2]});
             if (hand) {
                 AdjCorrect<T, A> a{};
                 a.piso = true;
@@ -511,10 +520,7 @@ struct Adjoint : AdjointBase {
                 Roles r{};
                 vjp([&](const ParamsT<Dual<A>>& q) { dlaunch(KN_PISO_PRECOMP, q, r); }, pr, sd, false);
             }
-            if (int rc = clear(lamIDPI)) return rc;
-            if (int rc = clear(lamBPI)) return rc;
             // aodu(u^n, u' of the previous correction)
-            if (int rc = clear3(lamUcNew)) return rc;
             if (hand) {
                 AdjAodu<T, A> a{};
                 a.flag = flag;
@@ -539,7 +545,9 @@ struct Adjoint : AdjointBase {
         }
         // ---- SIMPLE correction: u = u* + u'(p', D), p = p^n + p' ----
         for (int a = 0; a < 3; ++a) adj_add_fluid(flag, lamUc[a], lu[a], lamUT[a], N, st);
-        if (int rc = clear(lamPC[0])) return rc;
+        // p = p^n + p' (fluid cells): lamPC[0] starts as lamP there, 0 elsewhere,
+        // and the correction's reverse adds to it
+        adj_add_fluid<A>(flag, nullptr, lamP, lamPC[0], N, st);
         if (hand) {
             AdjCorrect<T, A> a{};
             a.piso = false;
@@ -561,7 +569,6 @@ struct Adjoint : AdjointBase {
             r.pin = 0; r.vout = 1; r.flags = K_NO_INPLACE;
             vjp([&](const ParamsT<Dual<A>>& q) { dlaunch(KN_VEL_CORRECT, q, r); }, pr, sd, false);
         }
-        adj_add_fluid(flag, lamPC[0], lamP, lamPC[0], N, st);
         reverse_pres(tp, 0);
         // ---- Rhie-Chow coefficients ----
         if (hand) {

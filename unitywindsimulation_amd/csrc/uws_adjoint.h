@@ -230,6 +230,14 @@ void adj_gather_bnd(const T* t, int X, int Y, int Z, int face, const Stencil& st
 // out[i] = (a ? a[i] : 0) + (b and (flag == nullptr or flag[i] is fluid) ? b[i] : 0)
 template <class T>
 void adj_add_fluid(const uint8_t* flag, const T* a, const T* b, T* out, int n, hipStream_t s);
+// zero m arrays of n elements each in one launch (m <= ADJ_ZERO_MAX)
+constexpr int ADJ_ZERO_MAX = 32;
+template <class T>
+struct ZeroList {
+    int m;
+    T* p[ADJ_ZERO_MAX];
+};
+template <class T> void adj_zero(const ZeroList<T>& z, int n, hipStream_t s);
 // acc[0] += sum_i t[i] in a fixed order: per-workgroup partials into
 // part (ADJ_SUM_PARTS elements), then one workgroup sums them
 constexpr int ADJ_SUM_PARTS = 512;

@@ -304,6 +304,12 @@ __global__ __launch_bounds__(BLK) void k_adj_add_fluid(const uint8_t* flag, cons
     out[i] = v;
 }
 
+template <class T>
+__global__ __launch_bounds__(BLK) void k_adj_zero(ZeroList<T> z, int n) {
+    const int i = gid();
+    if (i < n) z.p[blockIdx.y][i] = (T)0;
+}
+
 // fixed-order sum: workgroup b of nb sums t[b*256 + j*nb*256 + lane] into
 // part[b], then one workgroup sums part[0..nb) and adds the total to acc
 template <class T>
@@ -417,6 +423,10 @@ void adj_add_fluid(const uint8_t* flag, const T* a, const T* b, T* out, int n, h
     hipLaunchKernelGGL(k_adj_add_fluid<T>, dim3(blocks(n)), dim3(BLK), 0, s, flag, a, b, out, n);
 }
 template <class T>
+void adj_zero(const ZeroList<T>& z, int n, hipStream_t s) {
+    if (z.m > 0) hipLaunchKernelGGL(k_adj_zero<T>, dim3(blocks(n), z.m), dim3(BLK), 0, s, z, n);
+}
+template <class T>
 void adj_sum(const T* t, int n, double* acc, double* part, hipStream_t s) {
     const int nb = std::max(1, std::min(ADJ_SUM_PARTS, (n + 8 * BLK - 1) / (8 * BLK)));
     hipLaunchKernelGGL(k_adj_sum_part<T>, dim3(nb), dim3(BLK), 0, s, t, n, part);
@@ -443,5 +453,6 @@ template void adj_gather_bnd<double>(const double*, int, int, int, int, const St
                                      hipStream_t);
 template void adj_add_fluid<double>(const uint8_t*, const double*, const double*, double*, int, hipStream_t);
 template void adj_sum<double>(const double*, int, double*, double*, hipStream_t);
+template void adj_zero<double>(const ZeroList<double>&, int, hipStream_t);
 
 }  // namespace uws
