@@ -259,3 +259,28 @@ def test_adjoint_hand_reverse_matches_colouring(gpu, variant, monkeypatch):
             print(f"  {variant} prec {prec} {k:6s} max|colour| {scale:.3e}  "
                   f"max|hand - colour| / max|colour| {err:.2e}", flush=True)
             assert err < 1e-11, (prec, k, err)
+
+
+def test_adjoint_records_independent_of_sweep_fusion(gpu, monkeypatch):
+    """The recording step runs its pressure solves with the three-sweep
+    kernel, which also stores its first two sweeps' iterates; every record
+    must equal what single sweeps give, so the FP32 gradients with the
+    multi-sweep kernels and with single sweeps only (UWS_SWEEP_FUSION=0, in
+    the forward and the recording alike) are identical."""
+    w = workloads.tiny(grid=(20, 14, 17), pres_iters=8)
+    nsteps, K = 3, 2
+    wu, wp = _functional(w.ncells, seed=17)
+    out = []
+    for env in ({}, {"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"}):
+        monkeypatch.delenv("UWS_SWEEP_FUSION", raising=False)
+        monkeypatch.delenv("UWS_ZCHUNK3", raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        with _solver(w, N.PREC_FP32) as s:
+            with FvmAdjoint(s, nsteps, K) as adj:
+                adj.forward()
+                adj.backward(wu, wp)
+                out.append((adj.grad_velocity0(), adj.grad_pressure0(), adj.grad_bnd(N.X0), adj.grad_cs()))
+    for r in out[1:]:
+        for a, b in zip(r, out[0]):
+            assert np.array_equal(a, b)

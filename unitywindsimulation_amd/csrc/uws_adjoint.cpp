@@ -30,6 +30,7 @@
 #include <initializer_list>
 #include <memory>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "uws_adjoint.h"
@@ -263,15 +264,22 @@ struct Adjoint : AdjointBase {
     T* pc_final(const Tape& tp, int solve) const { return P > 0 ? tp.PS[solve][P - 1] : zero; }
 
     // one pressure solve with every iterate kept
+    // (FP32 with the step's three-sweep kernels: three sweeps per launch,
+    // the kernel storing its first two sweeps' iterates as well; the values
+    // equal single sweeps')
     void record_pres(FieldsT<T> f, const Tape& tp, int solve) {
-        for (int k = 0; k < P; ++k) {
+        for (int k = 0; k < P;) {
+            const bool three = std::is_same_v<T, float> && use_fusion3(s->fuse_mode) && k + 3 <= P;
             f.pc[0] = k > 0 ? tp.PS[solve][k - 1] : zero;
-            f.pc[1] = tp.PS[solve][k];
+            f.pc[1] = tp.PS[solve][k + (three ? 2 : 0)];
+            f.pcrec[0] = three ? tp.PS[solve][k] : nullptr;
+            f.pcrec[1] = three ? tp.PS[solve][k + 1] : nullptr;
             Roles r{};
             r.coef = solve > 0 ? 1 : 0;
             r.pin = k > 0 ? 0 : -1;
             r.pout = 1;
-            run(KN_PRES_SWEEP, f, r);
+            run(three ? KN_PRES_SWEEP3 : KN_PRES_SWEEP, f, r);
+            k += three ? 3 : 1;
         }
     }
 

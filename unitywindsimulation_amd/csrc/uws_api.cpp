@@ -207,6 +207,7 @@ size_t carve(const ParamsT<T>& p, FieldsT<T>& f, char* base, bool lowp) {
     };
     auto fl = [&](size_t n) { return (T*)take(n * sizeof(T)); };
     f.flag = (uint8_t*)take(N);
+    f.pcrec[0] = f.pcrec[1] = nullptr;
     for (int b = 0; b < 3; ++b)
         for (int c = 0; c < 3; ++c) f.vel[b][c] = fl(N);
     for (int c = 0; c < 3; ++c) f.ucorr[c] = fl(N);

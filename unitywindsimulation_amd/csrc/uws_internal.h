@@ -54,6 +54,7 @@ struct FieldsT {
     T* ucorr[3];          // u'  (velCorrectField)
     T* p;                 // p^n (presFieldLastTime)
     T* pc[2];             // p' ping-pong (presCorrectField / LastIter)
+    T* pcrec[2];          // where non-null, the three-sweep kernel also stores its first and second sweeps (adjoint records)
     T* nut;               // eddyVisField
     T* grad[9];           // cell velocity gradient du_i/dx_j (index 3i + j) of fluid cells, from the eddy-viscosity kernel
     T* nuw[3];            // faceEddyVisField slots W(.x) S(.y) D(.z)

@@ -1264,6 +1264,8 @@ struct PS2Args {
     void* eout;
     const T* xin;      // p' the round corrects (nullptr: zero)
     T s;               // scale of e
+    T* rec1;           // where non-null (LP = 0): the first and second sweeps' values are stored too
+    T* rec2;
 };
 template <class T>
 __device__ __forceinline__ PS2Args<T> ps2_args(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r) {
@@ -1272,6 +1274,8 @@ __device__ __forceinline__ PS2Args<T> ps2_args(const ParamsT<T>& p, const Fields
     a.k = pres_coef(f, r.coef);
     a.x0 = r.pin >= 0 ? f.pc[r.pin] : nullptr;
     a.out = f.pc[r.pout];
+    a.rec1 = f.pcrec[0];
+    a.rec2 = f.pcrec[1];
     return a;
 }
 
@@ -1593,6 +1597,10 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
         }
         const T s1 = rnd(comb(r0.inv, t1, r0.rhs));
         L1[S0 * PS3_THREADS + tid] = s1;
+        if constexpr (LP == 0) {
+            // the chunk's own cells of the first sweep, exact on the whole tile
+            if (p.rec1 && inner && zz >= z0 && zz < z1) st_at(p.rec1, (uint32_t)zz * planeB + colB, s1);
+        }
         // second sweep at z2 = zz - 1 (needed from z0 - 1)
         const int z2 = zz - 1;
         T s2 = 0;
@@ -1608,6 +1616,10 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
             t2 -= r1.cU * vU;
             t2 -= r2.cU * vDn;
             s2 = rnd(comb(r1.inv, t2, r1.rhs));
+            if constexpr (LP == 0) {
+                // exact inside the tile's outer ring, so on the inner cells
+                if (p.rec2 && inner && z2 >= z0 && z2 < z1) st_at(p.rec2, (uint32_t)z2 * planeB + colB, s2);
+            }
         }
         L2[S1 * PS3_THREADS + tid] = s2;
         // third sweep at z3 = zz - 2 into the output
