@@ -12,13 +12,22 @@ constexpr int BLK = 256;
 inline int blocks(int n) { return (n + BLK - 1) / BLK; }
 
 __device__ __forceinline__ int gid() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
-// the cell of a thread of a stencil kernel: workgroup ids are remapped so
-// that XCD j (ids j, j+8, ...) walks a contiguous range of cells, and the
-// neighbouring planes a workgroup reads stay in that XCD's L2
-__device__ __forceinline__ int gid_xcd() {
+// Stencil kernels run 64 x 4 tiles of one z-plane per workgroup (one wave
+// per x-row), their ids remapped so that XCD j (ids j, j+8, ...) walks a
+// contiguous range of tiles and the neighbouring rows and planes a
+// workgroup reads stay in that XCD's L2.  Returns false outside the grid.
+constexpr int TW = 64, TH = 4;
+inline int tiles2d(int X, int Y, int Z) { return ((X + TW - 1) / TW) * ((Y + TH - 1) / TH) * Z; }
+__device__ __forceinline__ bool cell2d(int X, int Y, int& x, int& y, int& z) {
     const int B = (int)gridDim.x, q = B >> 3, r = B & 7;
-    const int i = (int)blockIdx.x, xcd = i & 7;
-    return (xcd * q + min(xcd, r) + (i >> 3)) * (int)blockDim.x + (int)threadIdx.x;
+    const int b = (int)blockIdx.x, xcd = b & 7;
+    const int t = xcd * q + min(xcd, r) + (b >> 3);
+    const int gx = (X + TW - 1) / TW, gy = (Y + TH - 1) / TH;
+    const int bx = t % gx, rr = t / gx, by = rr % gy;
+    z = rr / gy;
+    x = bx * TW + (int)threadIdx.x;
+    y = by * TH + (int)threadIdx.y;
+    return x < X && y < Y;
 }
 
 // ---------------------------------------------------------------------------
@@ -34,10 +43,10 @@ __device__ __forceinline__ int gid_xcd() {
 // ---------------------------------------------------------------------------
 template <class T, class A>
 __global__ __launch_bounds__(BLK) void k_adj_vel_lam(AdjVelSweep<T, A> a, int N) {
-    const int i = gid_xcd();
-    if (i >= N) return;
     const int X = a.X, Y = a.Y, Z = a.Z;
-    const int x = i % X, t = i / X, y = t % Y, z = t / Y;
+    int x, y, z;
+    if (!cell2d(X, Y, x, y, z)) return;
+    const int i = (z * Y + y) * X + x;
     const int sy = X, sz = X * Y;
     const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
     const A q = a.q;
@@ -58,10 +67,10 @@ __global__ __launch_bounds__(BLK) void k_adj_vel_lam(AdjVelSweep<T, A> a, int N)
 
 template <class T, class A>
 __global__ __launch_bounds__(BLK) void k_adj_vel_coef(AdjVelSweep<T, A> a, SweepWindow<T, A> w, int N) {
-    const int i = gid_xcd();
-    if (i >= N) return;
     const int X = a.X, Y = a.Y, Z = a.Z;
-    const int x = i % X, t = i / X, y = t % Y, z = t / Y;
+    int x, y, z;
+    if (!cell2d(X, Y, x, y, z)) return;
+    const int i = (z * Y + y) * X + x;
     const int sy = X, sz = X * Y;
     const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
     const bool isF = a.flag[i] == UWS_CELL_FLUID;
@@ -150,10 +159,10 @@ __global__ __launch_bounds__(BLK) void k_adj_vel_coef(AdjVelSweep<T, A> a, Sweep
 // ---------------------------------------------------------------------------
 template <class T, class A>
 __global__ __launch_bounds__(BLK) void k_adj_pres_lam(AdjPresSweep<T, A> a, int N) {
-    const int i = gid_xcd();
-    if (i >= N) return;
     const int X = a.X, Y = a.Y, Z = a.Z;
-    const int x = i % X, t = i / X, y = t % Y, z = t / Y;
+    int x, y, z;
+    if (!cell2d(X, Y, x, y, z)) return;
+    const int i = (z * Y + y) * X + x;
     const int sy = X, sz = X * Y;
     const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
     const A* lo = a.lam_out;
@@ -170,10 +179,10 @@ __global__ __launch_bounds__(BLK) void k_adj_pres_lam(AdjPresSweep<T, A> a, int 
 
 template <class T, class A>
 __global__ __launch_bounds__(BLK) void k_adj_pres_coef(AdjPresSweep<T, A> a, SweepWindow<T, A> w, int N) {
-    const int i = gid_xcd();
-    if (i >= N) return;
     const int X = a.X, Y = a.Y, Z = a.Z;
-    const int x = i % X, t = i / X, y = t % Y, z = t / Y;
+    int x, y, z;
+    if (!cell2d(X, Y, x, y, z)) return;
+    const int i = (z * Y + y) * X + x;
     const int sy = X, sz = X * Y;
     const bool hW = x > 0, hE = x + 1 < X, hS = y > 0, hN = y + 1 < Y, hD = z > 0, hU = z + 1 < Z;
     const bool isF = a.flag[i] == UWS_CELL_FLUID;
@@ -381,19 +390,19 @@ Colour colour_bnd(int kind) { return kind == ST_OWN ? Colour{1, 0, 0} : Colour{5
 
 template <class T, class A>
 void adj_vel_lam(const AdjVelSweep<T, A>& a, int N, hipStream_t s) {
-    hipLaunchKernelGGL((k_adj_vel_lam<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+    hipLaunchKernelGGL((k_adj_vel_lam<T, A>), dim3(tiles2d(a.X, a.Y, a.Z)), dim3(TW, TH), 0, s, a, N);
 }
 template <class T, class A>
 void adj_vel_coef(const AdjVelSweep<T, A>& a, const SweepWindow<T, A>& w, int N, hipStream_t s) {
-    hipLaunchKernelGGL((k_adj_vel_coef<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, w, N);
+    hipLaunchKernelGGL((k_adj_vel_coef<T, A>), dim3(tiles2d(a.X, a.Y, a.Z)), dim3(TW, TH), 0, s, a, w, N);
 }
 template <class T, class A>
 void adj_pres_lam(const AdjPresSweep<T, A>& a, int N, hipStream_t s) {
-    hipLaunchKernelGGL((k_adj_pres_lam<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, N);
+    hipLaunchKernelGGL((k_adj_pres_lam<T, A>), dim3(tiles2d(a.X, a.Y, a.Z)), dim3(TW, TH), 0, s, a, N);
 }
 template <class T, class A>
 void adj_pres_coef(const AdjPresSweep<T, A>& a, const SweepWindow<T, A>& w, int N, hipStream_t s) {
-    hipLaunchKernelGGL((k_adj_pres_coef<T, A>), dim3(blocks(N)), dim3(BLK), 0, s, a, w, N);
+    hipLaunchKernelGGL((k_adj_pres_coef<T, A>), dim3(tiles2d(a.X, a.Y, a.Z)), dim3(TW, TH), 0, s, a, w, N);
 }
 template <class T, class A>
 void dual_pack(const T* src, Dual<A>* dst, int n, hipStream_t s) {
