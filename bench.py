@@ -23,7 +23,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per cell per launch read by rocprofv3 FETCH_SIZE (x2, calibrated on a
 # 256 MiB copy) + WRITE_SIZE on the urban case: profiles/urban_256x96x128_hbm_traffic.md
 MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 76.0, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.2,
-                                         "k_pres_sweep3": 35.4, "k_vel_sweep2": 87.8}}
+                                         "k_pres_sweep3": 35.4, "k_pres_sweep4": 36.2, "k_vel_sweep2": 85.7}}
 
 
 def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
@@ -34,18 +34,23 @@ def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
     return 263 - (0 if les else 83) + 77 * V + 29 * P + C * (123 + 33 * P)
 
 
-def compulsory_bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
+def compulsory_bytes_per_cell_step(V: int, P: int, C: int, les: bool = True, pres_depth: int = 3,
+                                   vel_depth: int = 2) -> int:
     """Bytes per cell one step of this build must move, each array once per
     launch: wall function + eddy viscosity with the gradient store (78), the
     momentum precompute with the LES term and the gradient read (146), V / 2
-    momentum pairs (73 each, a leftover single sweep 77), the Rhie-Chow
-    precompute (37), per pressure solve P / 3 triples (28 each; a leftover
-    pair 28, a single 29), p += p' (13), the velocity correction (53) and per
-    PISO corrector A_od u' (37), the corrector's right-hand side (4) and the
-    correction (78).  3,818 B/cell at V = P = 50, C = 2."""
-    solve = (P // 3) * 28 + {0: 0, 1: 29, 2: 28}[P % 3]
+    momentum pairs (73 each, a leftover single sweep 77; vel_depth 1: V single
+    sweeps), the Rhie-Chow precompute (37), per pressure solve P / pres_depth
+    launches of pres_depth sweeps (28 each; a leftover triple or pair 28, a
+    single 29), p += p' (13), the velocity correction (53) and per PISO
+    corrector A_od u' (37), the corrector's right-hand side (4) and the
+    correction (78).  3,818 B/cell at V = P = 50, C = 2 with triples, 3,482
+    with four-sweep launches."""
+    d = max(1, pres_depth)
+    solve = (P // d) * 28 + (0 if P % d == 0 else (29 if P % d == 1 else 28)) if d > 1 else 29 * P
+    mom = (V // 2) * 73 + (V % 2) * 77 if vel_depth == 2 else 77 * V
     one_shot = (78 + 146) if les else 69
-    return one_shot + (V // 2) * 73 + (V % 2) * 77 + 37 + solve + 13 + 53 + C * (37 + 4 + solve + 78)
+    return one_shot + mom + 37 + solve + 13 + 53 + C * (37 + 4 + solve + 78)
 
 
 def stream_ceiling(cells: int, nin: int, reps: int = 20) -> float:
@@ -495,21 +500,30 @@ def main() -> int:
     hbm_frac = b_step * ncell / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     # dominant kernel: the Jacobi sweep kernel with the larger share of step time.  The step runs the
-    # sweeps in groups (temporal blocking: k_pres_sweep3, three pressure sweeps per launch, the rest as
-    # k_pres_sweep2 pairs; FP32 momentum sweeps on the uniform grid as k_vel_sweep2 pairs).  A
-    # launch's algorithmic bytes are what it must move, each array once: pressure 28 B/cell (c_E,
-    # c_N, c_U, 1/Dp, rhs, p' in, p' out), momentum 73 B/cell (S, G, 1/D, b: 3 planes each, the flag,
-    # u in, u out); separate sweeps would move 29 and 77 B/cell each.
+    # sweeps in groups (temporal blocking: k_pres_sweep4 or k_pres_sweep3, four or three pressure sweeps
+    # per launch as the handle's launch plan says, the rest in shorter launches; FP32 momentum sweeps as
+    # k_vel_sweep2 pairs where the plan has them).  A launch's algorithmic bytes are what it must move,
+    # each array once: pressure 28 B/cell (c_E, c_N, c_U, 1/Dp, rhs, p' in, p' out), momentum 73 B/cell
+    # (S, G, 1/D, b: 3 planes each, the flag, u in, u out); separate sweeps would move 29 and 77 B/cell.
+    pres_depth, vel_depth = s.launch_plan()
     t_vel = s.time_kernel(N.KERNEL_VEL_SWEEP, 20)
     t_pres = s.time_kernel(N.KERNEL_PRES_SWEEP, 20)
     t_pres2 = s.time_kernel(N.KERNEL_PRES_SWEEP2, 20)
     t_vel2 = s.time_kernel(N.KERNEL_VEL_SWEEP2, 20)
     t_pres3 = s.time_kernel(N.KERNEL_PRES_SWEEP3, 20)
-    share_vel, share_pres = (V // 2) * t_vel2, (1 + C) * (P // 3) * t_pres3
+    t_pres4 = s.time_kernel(N.KERNEL_PRES_SWEEP4, 20)
+    t_pd = {1: t_pres, 2: t_pres2, 3: t_pres3, 4: t_pres4}[pres_depth]
+    share_vel = (V // 2) * t_vel2 if vel_depth == 2 else V * t_vel
+    share_pres = (1 + C) * (P // pres_depth) * t_pd
     if share_vel >= share_pres:
-        kname, kms, kbytes, kbytes_sweeps = "k_vel_sweep2", t_vel2, 73 * ncell, 2 * 77 * ncell
+        if vel_depth == 2:
+            kname, kms, kbytes, kbytes_sweeps = "k_vel_sweep2", t_vel2, 73 * ncell, 2 * 77 * ncell
+        else:
+            kname, kms, kbytes, kbytes_sweeps = "k_vel_sweep", t_vel, 77 * ncell, 77 * ncell
     else:
-        kname, kms, kbytes, kbytes_sweeps = "k_pres_sweep3", t_pres3, 28 * ncell, 3 * 29 * ncell
+        kname = {1: "k_pres_sweep", 2: "k_pres_sweep2", 3: "k_pres_sweep3", 4: "k_pres_sweep4"}[pres_depth]
+        kms, kbytes = t_pd, (29 if pres_depth == 1 else 28) * ncell
+        kbytes_sweeps = pres_depth * 29 * ncell
     achieved = kbytes / (kms * 1e-3) / 1e9
     tr = MEASURED_TRAFFIC_B_PER_CELL.get(args.workload, {}).get(kname)
     traffic = int(tr * ncell) if tr is not None else None
@@ -522,8 +536,8 @@ def main() -> int:
     ceil = {"pressure_like_28B_per_cell": round(stream_ceiling(ncell, 6), 1),
             "momentum_like_68B_per_cell": round(stream_ceiling(ncell, 16), 1),
             "hbm_1792MiB": round(stream_ceiling(64 << 20, 6, 10), 1)}
-    kceil = ceil["momentum_like_68B_per_cell"] if kname == "k_vel_sweep2" else ceil["pressure_like_28B_per_cell"]
-    b_comp = compulsory_bytes_per_cell_step(V, P, C, les)
+    kceil = ceil["momentum_like_68B_per_cell"] if kname.startswith("k_vel") else ceil["pressure_like_28B_per_cell"]
+    b_comp = compulsory_bytes_per_cell_step(V, P, C, les, pres_depth, vel_depth)
     comp_gbps = b_comp * ncell / (ms_step * 1e-3) / 1e9
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -557,8 +571,8 @@ def main() -> int:
                        "compulsory_frac_of_peak": round(comp_gbps / HBM_PEAK_GBS, 4),
                        "compulsory_frac_of_stream_ceiling": round(comp_gbps / ceil["momentum_like_68B_per_cell"], 4),
                        "basis": "each array once per launch of the kernels the step runs (multi-sweep launches "
-                                "counted as one pass); the stream ceiling is the plain streaming kernel on a "
-                                "working set of the momentum pair's size"},
+                                "counted as one pass, per the handle's launch plan); the stream ceiling is the "
+                                "plain streaming kernel on a working set of the momentum pair's size"},
         "stream_ceiling_GBps": ceil,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -566,7 +580,8 @@ def main() -> int:
                      "algorithmic_bytes": kbytes, "single_sweeps_bytes": kbytes_sweeps,
                      "kernel_ms": {"k_vel_sweep": round(t_vel, 5), "k_vel_sweep2": round(t_vel2, 5),
                                    "k_pres_sweep": round(t_pres, 5), "k_pres_sweep2": round(t_pres2, 5),
-                                   "k_pres_sweep3": round(t_pres3, 5)}},
+                                   "k_pres_sweep3": round(t_pres3, 5), "k_pres_sweep4": round(t_pres4, 5)}},
+        "launch_plan": {"pres_sweeps_per_launch": pres_depth, "vel_sweeps_per_launch": vel_depth},
         "finite": finite,
     }
     if not args.no_ensemble:

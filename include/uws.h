@@ -348,11 +348,18 @@ int uws_get_sweep_counts(uws_handle h, int32_t* vel_sweeps, int32_t* pres_sweeps
  * UWS_KERNEL_PRES_SWEEP2 is the two-sweep pressure kernel the step uses;
  * UWS_KERNEL_VEL_SWEEP2 the two-sweep momentum kernel (FP32, uniform or
  * stretched grid; UWS_ERR_UNSUPPORTED elsewhere), from u^n into a scratch buffer;
- * UWS_KERNEL_PRES_SWEEP3 the three-sweep pressure kernel (FP32).  The solver
- * state (u^n, p^n) is not changed.  Call after at least one uws_step. */
+ * UWS_KERNEL_PRES_SWEEP3 and UWS_KERNEL_PRES_SWEEP4 the three- and four-sweep
+ * pressure kernels (FP32).  The solver state (u^n, p^n) is not changed.  Call
+ * after at least one uws_step. */
 enum { UWS_KERNEL_VEL_SWEEP = 0, UWS_KERNEL_PRES_SWEEP = 1, UWS_KERNEL_PRES_SWEEP2 = 2, UWS_KERNEL_VEL_SWEEP2 = 3,
-       UWS_KERNEL_PRES_SWEEP3 = 4 };
+       UWS_KERNEL_PRES_SWEEP3 = 4, UWS_KERNEL_PRES_SWEEP4 = 5 };
 int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms);
+
+/* Diagnostics (no reference counterpart): the most sweeps one launch of the
+ * step runs, for the plain pressure solves (4, 3, 2 or 1; a solve's last
+ * sweeps run in shorter launches) and for the momentum predictor (2 or 1),
+ * as the multi-sweep policy picks them for this grid and precision. */
+int uws_get_launch_plan(uws_handle h, int32_t* pres_sweeps_per_launch, int32_t* vel_sweeps_per_launch);
 
 /* Diagnostics (no reference counterpart): the bandwidth ceiling a plain
  * streaming kernel reaches on the current device, in GB/s of bytes read and

@@ -145,6 +145,7 @@ struct Rec {
     bool fused() const { return f2; }
     bool fused_vel() const { return fv; }
     bool fused3() const { return f3; }
+    bool fused4() const { return false; }
 };
 
 void run_case(int V, int P, int C, bool norms, bool f2, bool fv, bool f3, bool lowp, int R, int K, bool residual,

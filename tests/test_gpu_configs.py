@@ -32,7 +32,7 @@ TOL = 1e-5  # FP32 fields after a step, relative L2
 
 
 def _no_overrides(monkeypatch):
-    for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_VEL_PAIRS", "UWS_ZCHUNK", "UWS_ZCHUNK3",
+    for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_VEL_PAIRS", "UWS_ZCHUNK", "UWS_ZCHUNK3", "UWS_ZCHUNK4",
               "UWS_TILE_ROWS2", "UWS_NO_P2", "UWS_NO_GRAPH"):
         monkeypatch.delenv(k, raising=False)
 
@@ -56,25 +56,29 @@ def test_urban_two_steps_match_oracle(gpu, monkeypatch):
 @pytest.mark.parametrize("norms", [False, True])
 def test_urban_multi_sweep_kernels_equal_single_sweeps(gpu, monkeypatch, norms):
     """At the urban grid the policy runs momentum pairs (16-plane z chunks,
-    14-row tiles) and pressure triples (19-plane chunks); two steps equal the
-    one-sweep launches bit for bit (with norms the last sweeps stay single
-    and the FP64 norms agree too)."""
+    14-row tiles) and four-sweep pressure launches (26-plane chunks); two
+    steps with them, and with pressure triples instead (19-plane chunks),
+    equal the one-sweep launches bit for bit (with norms the last sweeps stay
+    single and the FP64 norms agree too)."""
     w = workloads.urban()
     w.cf.computeNorms = norms
     w.cf.Init()
     res = []
-    for env in ({}, {"UWS_SWEEP_FUSION": "0"}):
+    runs = ({"UWS_SWEEP_FUSION": "0"}, {}, {"UWS_SWEEP_DEPTH": "3"})
+    for env in runs:
         _no_overrides(monkeypatch)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
+        print(f"  [configs] urban multi-sweep {env}", flush=True)
         with solver_for(w) as s:
             s.step(2)
             res.append((s.velocity(3), s.pressure(), s.norms() if norms else None))
-    assert np.array_equal(res[0][0].view(np.uint32), res[1][0].view(np.uint32))
-    assert np.array_equal(res[0][1].view(np.uint32), res[1][1].view(np.uint32))
-    if norms:
-        assert np.array_equal(res[0][2], res[1][2])
-        assert res[0][2][N.NORM_NONFINITE] == 0
+    for env, r in zip(runs[1:], res[1:]):
+        assert np.array_equal(r[0].view(np.uint32), res[0][0].view(np.uint32)), env
+        assert np.array_equal(r[1].view(np.uint32), res[0][1].view(np.uint32)), env
+        if norms:
+            assert np.array_equal(r[2], res[0][2]), env
+            assert r[2][N.NORM_NONFINITE] == 0
 
 
 def test_ensemble_24_cases_equal_single_cases(gpu, monkeypatch):

@@ -164,14 +164,15 @@ def test_deterministic(gpu):
                                                               ("tiny", 2, 2, False), ("isolated_building", 4, 5, False)])
 def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters, pres_iters, norms):
     """The multi-sweep kernels (temporal blocking over z-marching tiles: two
-    momentum sweeps, two or three pressure sweeps per launch) give
+    momentum sweeps, two, three or four pressure sweeps per launch) give
     bit-identical fields to the one-sweep kernels: counts that do not divide
     end with shorter launches and with norms the last sweep stays single.
     Runs: one-sweep kernels only; pairs only; triples with the grid's z chunk;
     triples with 5-, 2- and 1-plane z chunks, pairs with 1- and 3-plane ones
     (chunks that start at z = 1 and 2, where the kernels' first plane records
     reach below the grid); pairs in tiles of 15 and 5 rows (the grid's own
-    choice is 14 or 15)."""
+    choice is 14 or 15); four-sweep launches with the grid's z chunk and
+    with 5-, 2- and 1-plane ones."""
     if name == "tiny":
         w = workloads.tiny(vel_iters=vel_iters, pres_iters=pres_iters)
     else:
@@ -185,13 +186,18 @@ def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters
             {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"},
             {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "2", "UWS_ZCHUNK": "1"},
             {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "1", "UWS_ZCHUNK": "3"},
-            {"UWS_SWEEP_FUSION": "2", "UWS_TILE_ROWS2": "15"}, {"UWS_SWEEP_FUSION": "2", "UWS_TILE_ROWS2": "5"}]
+            {"UWS_SWEEP_FUSION": "2", "UWS_TILE_ROWS2": "15"}, {"UWS_SWEEP_FUSION": "2", "UWS_TILE_ROWS2": "5"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "4"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "4", "UWS_ZCHUNK4": "5"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "4", "UWS_ZCHUNK4": "2"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "4", "UWS_ZCHUNK4": "1"}]
     res = []
     for env in runs:
-        for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_ZCHUNK3", "UWS_ZCHUNK", "UWS_TILE_ROWS2"):
+        for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_ZCHUNK3", "UWS_ZCHUNK4", "UWS_ZCHUNK", "UWS_TILE_ROWS2"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
+        print(f"  [multi-sweep] {name} V={vel_iters} P={pres_iters} norms={norms} {env}", flush=True)
         with solver_for(w, p0=seeded_pressure(w)) as s:
             s.step(2)
             res.append((s.velocity(3), s.pressure(), s.norms() if norms else None))

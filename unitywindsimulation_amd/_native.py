@@ -33,7 +33,8 @@ FIELD_EDDY_VIS, FIELD_WALL_VIS, FIELD_B, FIELD_D, FIELD_DP, FIELD_BP, FIELD_AODU
 FIELD_FACE_VEL = 12
 FIELD_NCOMP = {0: 3, 1: 1, 2: 3, 3: 3, 4: 1, 5: 1, 6: 3, 7: 3, 8: 3, 9: 1, 10: 1, 11: 3, 12: 3}
 
-KERNEL_VEL_SWEEP, KERNEL_PRES_SWEEP, KERNEL_PRES_SWEEP2, KERNEL_VEL_SWEEP2, KERNEL_PRES_SWEEP3 = 0, 1, 2, 3, 4
+KERNEL_VEL_SWEEP, KERNEL_PRES_SWEEP, KERNEL_PRES_SWEEP2, KERNEL_VEL_SWEEP2, KERNEL_PRES_SWEEP3, KERNEL_PRES_SWEEP4 = \
+    0, 1, 2, 3, 4, 5
 PREC_FP32, PREC_FP64, PREC_FP32_FP16, PREC_FP32_BF16, PREC_CSTEP = 0, 1, 2, 3, 4
 CSTEP_VEL, CSTEP_PRES, CSTEP_BND, CSTEP_CS = 0, 1, 2, 3
 GRAD_VEL0, GRAD_PRES0, GRAD_BND, GRAD_CS = 0, 1, 2, 3
@@ -124,6 +125,7 @@ SIGNATURES = {
     "uws_get_field_f64": (c_int, [c_void_p, c_int32, c_void_p, c_int64]),
     "uws_get_norms": (c_int, [c_void_p, c_void_p, c_int32]),
     "uws_get_sweep_counts": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
+    "uws_get_launch_plan": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
     "uws_stream_bandwidth": (c_int, [c_int64, c_int32, c_int32, POINTER(c_double)]),
     "uws_time_kernel": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_double)]),
     "uws_batch_create": (c_int, [c_void_p, c_int32, POINTER(c_void_p)]),

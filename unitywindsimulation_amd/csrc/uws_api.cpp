@@ -122,6 +122,19 @@ int choose_zchunk3(int X, int Y, int Z) {
     return best;
 }
 
+// z planes per four-sweep workgroup (three halo planes above and below its
+// chunk), chosen as for the three-sweep kernel
+int choose_zchunk4(int X, int Y, int Z) {
+    int best = Z, best_cost = -1;
+    for (int nz = 1; nz <= Z; ++nz) {
+        const int zc = (Z + nz - 1) / nz;
+        const int blocks = pres_sweep4_grid(X, Y, Z, zc);
+        const int cost = ((blocks + 255) / 256) * (zc + 6);
+        if (best_cost < 0 || cost < best_cost) { best = zc; best_cost = cost; }
+    }
+    return best;
+}
+
 template <class T>
 void fill_params(const uws_config& c, ParamsT<T>& p) {
     p.X = c.grid_res[0]; p.Y = c.grid_res[1]; p.Z = c.grid_res[2];
@@ -165,6 +178,11 @@ void fill_params(const uws_config& c, ParamsT<T>& p) {
     if (const char* e = std::getenv("UWS_ZCHUNK3")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 1024) p.zchunk3 = v;
+    }
+    p.zchunk4 = choose_zchunk4(p.X, p.Y, p.Z);
+    if (const char* e = std::getenv("UWS_ZCHUNK4")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= 1024) p.zchunk4 = v;
     }
     p.nonuniform = c.grid_type == UWS_GRID_COLL_NONUNIFORM ? 1 : 0;
     // powers-of-two dx and den: products with exact reciprocals in the cell
@@ -868,8 +886,19 @@ int uws_get_sweep_counts(uws_handle h, int32_t* vel_sweeps, int32_t* pres_sweeps
     });
 }
 
+int uws_get_launch_plan(uws_handle h, int32_t* pres_sweeps_per_launch, int32_t* vel_sweeps_per_launch) {
+    if (!pres_sweeps_per_launch || !vel_sweeps_per_launch) return UWS_ERR_INVALID_ARG;
+    return with_solver(h, [&](SolverBase& s) {
+        int pr = 1, ve = 1;
+        s.launch_plan(&pr, &ve);
+        *pres_sweeps_per_launch = pr;
+        *vel_sweeps_per_launch = ve;
+        return (int)UWS_OK;
+    });
+}
+
 int uws_time_kernel(uws_handle h, int32_t kernel, int32_t reps, double* avg_ms) {
-    if (!avg_ms || reps <= 0 || kernel < UWS_KERNEL_VEL_SWEEP || kernel > UWS_KERNEL_PRES_SWEEP3)
+    if (!avg_ms || reps <= 0 || kernel < UWS_KERNEL_VEL_SWEEP || kernel > UWS_KERNEL_PRES_SWEEP4)
         return UWS_ERR_INVALID_ARG;
     return with_solver(h, [&](SolverBase& s) { return s.time_kernel(kernel, reps, avg_ms); });
 }

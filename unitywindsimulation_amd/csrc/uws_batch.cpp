@@ -214,6 +214,7 @@ struct uws_batch {
         bool fused() const { return use_fusion(b->fuse_mode); }
         bool fused_vel() const { return g->vel_pairs; }
         bool fused3() const { return use_fusion3(b->fuse_mode); }
+        bool fused4() const { return false; }  // single cases only
     };
 
     void sync_case_roles() {

@@ -76,6 +76,8 @@ struct SolverBase {
     virtual int run_stage(int stage) = 0;
     virtual int step_n(int n) = 0;
     virtual int time_kernel(int kernel, int reps, double* avg_ms) = 0;
+    // sweeps per launch of the plain (not low-precision) pressure solves and of the momentum predictor
+    virtual void launch_plan(int* pres, int* vel) = 0;
     virtual int set_cs(double re, double im) = 0;
     virtual int init_from_background(const float fg_pos[3], float rot_deg, const uws_background& bg,
                                      const float* vel, int ncomp, const float* pres) = 0;
@@ -119,6 +121,19 @@ inline bool use_fusion3(int mode) {
     const char* e = std::getenv("UWS_SWEEP_DEPTH");
     if (e && e[0] == '2') return false;
     return mode != 0;
+}
+// Four-sweep pressure launches (FP32 single cases) on grids of at least
+// QUAD_MIN_CELLS: 256 x 96 x 128 runs 2.0 % and 384 x 128 x 256 3.7 % faster
+// per step with them than with triples, 128 x 64 x 96, 160 x 64 x 96 and
+// 192 x 64 x 96 0.9 to 2.2 % slower (their 8-row tiles and longer halo cost
+// more there than the fewer launches save).  UWS_SWEEP_DEPTH = 4 asks for
+// them at any size, 2 or 3 never.
+constexpr int QUAD_MIN_CELLS = 2000000;
+inline bool use_fusion4(int mode, int cells) {
+    const char* e = std::getenv("UWS_SWEEP_DEPTH");
+    if (e && e[0] == '4') return mode != 0;
+    if (e && (e[0] == '2' || e[0] == '3')) return false;
+    return mode == 1 && cells >= QUAD_MIN_CELLS;
 }
 inline bool use_vel_pairs(int mode, int blocks, int zchunk) {
     if (const char* e = std::getenv("UWS_VEL_PAIRS")) return e[0] == '1';
@@ -266,6 +281,12 @@ int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest, int* s
         const int out = in == 0 ? 1 : 0;
         Roles r{};
         r.coef = coef; r.pin = in; r.pout = out;
+        if (l.fused4() && !residual_exit && k + 4 <= c.P && !(last_single && k + 4 == c.P)) {
+            l(KN_PRES_SWEEP4, r);
+            in = out;
+            k += 4;
+            continue;
+        }
         if (l.fused3() && !residual_exit && k + 3 <= c.P && !(last_single && k + 3 == c.P)) {
             l(KN_PRES_SWEEP3, r);
             in = out;

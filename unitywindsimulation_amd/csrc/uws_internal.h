@@ -30,6 +30,7 @@ struct ParamsT {
     int zchunk;           // z planes per workgroup of the two-sweep kernels
     int by2;              // tile rows of the two-sweep kernels (outputs + 2 halo rows, at most 15)
     int zchunk3;          // z planes per workgroup of the three-sweep pressure kernel
+    int zchunk4;          // z planes per workgroup of the four-sweep pressure kernel
     int nonuniform;       // 1: stretched grid (GridType.CollNonUniform), spacings from FieldsT::h
     // p2 = 1 (float, double) where dx and den are powers of two: the cell
     // kernels multiply by these exact reciprocals instead of dividing by dx,
@@ -133,7 +134,8 @@ enum Kernel {
     KN_NU_VEL_PRECOMP, KN_NU_VEL_SWEEP, KN_NU_PRES_PRECOMP, KN_NU_VEL_CORRECT, KN_NU_DIVERGENCE,
     KN_VEL_SWEEP2,  // two momentum Jacobi sweeps in one launch (single case, uniform grid, FP32)
     KN_PRES_SWEEP3,  // three pressure Jacobi sweeps in one launch (single case, FP32)
-    KN_LOWP_SWEEP3   // three low-precision correction sweeps in one launch (single case, FP32)
+    KN_LOWP_SWEEP3,  // three low-precision correction sweeps in one launch (single case, FP32)
+    KN_PRES_SWEEP4   // four pressure Jacobi sweeps in one launch (single case, FP32)
 };
 
 // Single-case launch of kernel k on stream s.  `slot` selects the norms
@@ -183,6 +185,7 @@ void mean_accumulate(T* const u[3], double* const sum[3], int n, hipStream_t s);
 // workgroups of the two-sweep pressure kernel for one grid
 int pres_sweep2_grid(int X, int Y, int Z, int zchunk, int by);
 int pres_sweep3_grid(int X, int Y, int Z, int zchunk);
+int pres_sweep4_grid(int X, int Y, int Z, int zchunk);
 
 inline int tiles_of(int X, int Y, int Z, int ty) { return ((X + 63) / 64) * ((Y + ty - 1) / ty) * Z; }
 

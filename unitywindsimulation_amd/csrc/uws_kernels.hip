@@ -1682,6 +1682,196 @@ __device__ __forceinline__ void pres_sweep3_block(const PS2Args<T>& p, int b, T*
     }
 }
 
+// ---------------------------------------------------------------------------
+// Four pressure Jacobi sweeps in one launch: the three-sweep kernel one level
+// deeper (FP32, single cases).  A workgroup of 70 x 14 threads owns a 64 x 8
+// column of cells (three halo cells on each side) and marches z from three
+// planes below its chunk to three above.  At plane zz every thread computes
+// the first sweep at zz (ring 1), the second at zz - 1 (ring 2), the third at
+// zz - 2 (ring 3) and the interior threads the fourth at zz - 3 into the
+// output; one barrier per plane.  Five plane records rotate through named
+// variables (the loop is unrolled five times, the period of the five-slot
+// rings, so LDS addresses are constant offsets).  Expressions, masks and
+// non-fluid handling are the three-sweep kernel's, so the result equals four
+// single sweeps.
+// ---------------------------------------------------------------------------
+constexpr int PS4_W = 70;                    // threads per tile row: 64 outputs + 2 x 3 halo
+constexpr int PS4_BY = 14;                   // tile rows: 8 outputs + 2 x 3 halo
+constexpr int PS4_THREADS = PS4_W * PS4_BY;  // 980
+constexpr int PS4_RING = 5;                  // planes per LDS ring (three rings)
+
+inline __host__ __device__ int pres_sweep4_blocks(int X, int Y, int Z, int zchunk) {
+    return ((X + 63) / 64) * ((Y + PS4_BY - 7) / (PS4_BY - 6)) * ((Z + zchunk - 1) / zchunk);
+}
+
+template <bool ZIN>
+__device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b, float* lds) {
+    using T = float;
+    const int tid = (int)threadIdx.x;
+    const int lx = tid % PS4_W, ly = tid / PS4_W;
+    const int ntx = (p.X + 63) / 64, nty = (p.Y + PS4_BY - 7) / (PS4_BY - 6);
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int x = bx * 64 - 3 + lx, y = by * (PS4_BY - 6) - 3 + ly;
+    const int z0 = bz * p.zchunk, z1 = min(z0 + p.zchunk, p.Z);
+    const bool cell = x >= 0 && x < p.X && y >= 0 && y < p.Y;
+    const bool inner = cell && lx >= 3 && lx <= PS4_W - 4 && ly >= 3 && ly <= PS4_BY - 4;
+    const PresCoef<T> k = p.k;
+    const T* __restrict__ x0 = ZIN ? nullptr : p.x0;
+    T* __restrict__ out = p.out;
+    const bool hW = x > 0, hE = x + 1 < p.X, hS = y > 0, hN = y + 1 < p.Y;
+    const int xc = min(max(x, 0), p.X - 1), yc = min(max(y, 0), p.Y - 1);
+    const bool gE = xc + 1 < p.X, gW = xc > 0, gN = yc + 1 < p.Y, gS = yc > 0;
+    const uint32_t colB = (uint32_t)(yc * p.X + xc) * (uint32_t)sizeof(T);
+    const uint32_t planeB = (uint32_t)(p.X * p.Y) * (uint32_t)sizeof(T);
+    const uint32_t rowB = (uint32_t)(p.X * sizeof(T));
+    // the record of plane zz: its coefficients and the input iterate at its
+    // E, W, N, S neighbours and at the plane above (clamped into the grid)
+    auto load = [&](PS2Plane<T>& q, int zz) {
+        const uint32_t oP = (uint32_t)min(max(zz, 0), p.Z - 1) * planeB + colB;
+        const uint32_t oU = (uint32_t)min(max(zz + 1, 0), p.Z - 1) * planeB + colB;
+        const uint32_t oW = gW ? oP - (uint32_t)sizeof(T) : oP, oS = gS ? oP - rowB : oP;
+        q.cE = ld_at(k.cE, oP); q.cN = ld_at(k.cN, oP); q.cU = ld_at(k.cU, oP);
+        q.cW = ld_at(k.cE, oW);
+        q.cS = ld_at(k.cN, oS);
+        q.inv = ld_at(k.invDp, oP);
+        q.rhs = ld_at(k.rhs, oP);
+        if (!ZIN) {
+            q.vE = ld_at(x0, gE ? oP + (uint32_t)sizeof(T) : oP);
+            q.vW = ld_at(x0, oW);
+            q.vN = ld_at(x0, gN ? oP + rowB : oP);
+            q.vS = ld_at(x0, oS);
+            q.vU = ld_at(x0, oU);
+        }
+    };
+    // the records below the first computed plane are read only in part:
+    // plane z0 - 5 for the input iterate above it, plane z0 - 4 for that and
+    // its U coefficient (the D coefficient of plane z0 - 3)
+    auto load_below = [&](PS2Plane<T>& q, int zz, bool with_cU) {
+        const uint32_t oP = (uint32_t)min(max(zz, 0), p.Z - 1) * planeB + colB;
+        const uint32_t oU = (uint32_t)min(max(zz + 1, 0), p.Z - 1) * planeB + colB;
+        q.cE = q.cW = q.cN = q.cS = q.inv = q.rhs = 0;
+        q.vE = q.vW = q.vN = q.vS = 0;
+        q.cU = with_cU ? ld_at(k.cU, oP) : (T)0;
+        q.vU = ZIN ? (T)0 : ld_at(x0, oU);
+    };
+    const int jE = (hE && lx + 1 < PS4_W) ? tid + 1 : tid, jW = (hW && lx > 0) ? tid - 1 : tid;
+    const int jN = (hN && ly + 1 < PS4_BY) ? tid + PS4_W : tid, jS = (hS && ly > 0) ? tid - PS4_W : tid;
+    T* const L1 = lds;
+    T* const L2 = lds + PS4_RING * PS4_THREADS;
+    T* const L3 = lds + 2 * PS4_RING * PS4_THREADS;
+    auto mask = [&](PS2Plane<T>& q, int zz) {
+        q.cW = hW ? q.cW : (T)0;
+        q.cS = hS ? q.cS : (T)0;
+        if (zz < 0) q.cU = 0;
+    };
+    // one sweep's value at a cell from the ring of the level below
+    auto sweep = [&](const PS2Plane<T>& r, const T* L, T vU, T vDn, T cD) {
+        const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
+        T t = 0;
+        t -= r.cE * vE;
+        t -= r.cW * vW;
+        t -= r.cN * vN;
+        t -= r.cS * vS;
+        t -= r.cU * vU;
+        t -= cD * vDn;
+        return r.inv * (t + r.rhs);
+    };
+    T s1m1 = 0, s1m2 = 0;  // own first-sweep values at zz - 1, zz - 2
+    T s2m1 = 0, s2m2 = 0;  // own second-sweep values at zz - 2, zz - 3
+    T s3m1 = 0, s3m2 = 0;  // own third-sweep values at zz - 3, zz - 4
+    // iteration zz: r0 = plane zz, r1 = zz - 1, r2 = zz - 2, r3 = zz - 3,
+    // nx = zz - 4 (receives zz + 1); kc: the step's position in the unrolled loop
+    auto step = [&](auto kc, PS2Plane<T>& r3, PS2Plane<T>& r2, PS2Plane<T>& r1, PS2Plane<T>& r0, PS2Plane<T>& nx,
+                    int zz) {
+        constexpr int K = decltype(kc)::value;
+        // ring slots of planes zz, zz - 1, zz - 2, zz - 3
+        constexpr int S0 = K, S1 = (K + 4) % 5, S2 = (K + 3) % 5, S3 = (K + 2) % 5;
+        const T dU4 = nx.cU;  // D coefficient of plane zz - 3
+        T vD = 0;
+        if (!ZIN) vD = r2.vU;  // the input iterate below plane zz (the record of zz - 2 read it)
+        load(nx, zz + 1 <= z1 + 2 ? zz + 1 : z1 + 2);
+        mask(r0, zz);
+        // first sweep at zz
+        T t1 = 0;
+        if (!ZIN) {
+            t1 -= r0.cE * r0.vE;
+            t1 -= r0.cW * r0.vW;
+            t1 -= r0.cN * r0.vN;
+            t1 -= r0.cS * r0.vS;
+            t1 -= r0.cU * r0.vU;
+            t1 -= r1.cU * vD;
+        }
+        const T s1 = r0.inv * (t1 + r0.rhs);
+        L1[S0 * PS4_THREADS + tid] = s1;
+        // second sweep at zz - 1 (needed from z0 - 2)
+        const int z2 = zz - 1;
+        T s2 = 0;
+        if (zz >= z0 - 1)
+            s2 = sweep(r1, L1 + S1 * PS4_THREADS, z2 + 1 < p.Z ? s1 : s1m1, z2 > 0 ? s1m2 : s1m1, r2.cU);
+        L2[S1 * PS4_THREADS + tid] = s2;
+        // third sweep at zz - 2 (needed from z0 - 1)
+        const int z3 = zz - 2;
+        T s3 = 0;
+        if (zz >= z0 + 1)
+            s3 = sweep(r2, L2 + S2 * PS4_THREADS, z3 + 1 < p.Z ? s2 : s2m1, z3 > 0 ? s2m2 : s2m1, r3.cU);
+        L3[S2 * PS4_THREADS + tid] = s3;
+        // fourth sweep at zz - 3 into the output
+        const int z4 = zz - 3;
+        if (inner && z4 >= z0 && z4 < z1) {
+            const T e4 = sweep(r3, L3 + S3 * PS4_THREADS, z4 + 1 < p.Z ? s3 : s3m1, z4 > 0 ? s3m2 : s3m1, dU4);
+            // an inner thread's column is in the grid: colB is its own
+            st_at(out, (uint32_t)z4 * planeB + colB, e4);
+        }
+        __syncthreads();
+        s1m2 = s1m1;
+        s1m1 = s1;
+        s2m2 = s2m1;
+        s2m1 = s2;
+        s3m2 = s3m1;
+        s3m1 = s3;
+    };
+    // A = plane z0 - 6 and E = z0 - 7 (not read), B = z0 - 5, C = z0 - 4
+    // (partial records), D = z0 - 3
+    PS2Plane<T> A, B, C, D, E;
+    A.cE = A.cW = A.cN = A.cS = A.cU = A.inv = A.rhs = 0;
+    A.vE = A.vW = A.vN = A.vS = A.vU = 0;
+    E = A;
+    load_below(B, z0 - 5, false);
+    load_below(C, z0 - 4, true);
+    load(D, z0 - 3);
+    // D is masked by its first step
+    mask(B, z0 - 5);
+    mask(C, z0 - 4);
+    using K0 = std::integral_constant<int, 0>;
+    using K1 = std::integral_constant<int, 1>;
+    using K2 = std::integral_constant<int, 2>;
+    using K3 = std::integral_constant<int, 3>;
+    using K4 = std::integral_constant<int, 4>;
+    for (int zz = z0 - 3;;) {
+        step(K0{}, A, B, C, D, E, zz);
+        if (++zz > z1 + 2) break;
+        step(K1{}, B, C, D, E, A, zz);
+        if (++zz > z1 + 2) break;
+        step(K2{}, C, D, E, A, B, zz);
+        if (++zz > z1 + 2) break;
+        step(K3{}, D, E, A, B, C, zz);
+        if (++zz > z1 + 2) break;
+        step(K4{}, E, A, B, C, D, zz);
+        if (++zz > z1 + 2) break;
+    }
+}
+
+template <bool ZIN>
+__global__ __launch_bounds__(1024) void k_pres_sweep4_f32(ParamsT<float> p, FieldsT<float> f, Roles r) {
+    __shared__ __align__(16) float lds_f[3 * PS4_RING * PS4_THREADS];
+    const int nb = pres_sweep4_blocks(p.X, p.Y, p.Z, p.zchunk4);
+    PS2Args<float> a = ps2_args(p, f, r);
+    a.zchunk = p.zchunk4;
+    pres_sweep4_block<ZIN>(a, xcd_tile((int)blockIdx.x, nb), lds_f);
+}
+
 template <bool ZIN>
 __global__ __launch_bounds__(1024) void k_pres_sweep3_f32(ParamsT<float> p, FieldsT<float> f, Roles r) {
     __shared__ __align__(16) float lds_f[2 * PS3_RING * PS3_THREADS];
@@ -2246,6 +2436,15 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
         }
         return;
     }
+    if (k == KN_PRES_SWEEP4) {
+        // FP32 single cases only (the step driver asks for it there)
+        if constexpr (std::is_same_v<T, float>) {
+            const int nb = pres_sweep4_blocks(p.X, p.Y, p.Z, p.zchunk4);
+            if (r.pin < 0) hipLaunchKernelGGL((k_pres_sweep4_f32<true>), dim3(nb), dim3(PS4_THREADS), 0, s, p, f, r);
+            else hipLaunchKernelGGL((k_pres_sweep4_f32<false>), dim3(nb), dim3(PS4_THREADS), 0, s, p, f, r);
+        }
+        return;
+    }
     if (k == KN_LOWP_SWEEP3) {
         if constexpr (std::is_same_v<T, float>) {
             const int nb = pres_sweep3_blocks(p.X, p.Y, p.Z, p.zchunk3);
@@ -2358,6 +2557,7 @@ __global__ __launch_bounds__(256) void k_mean_accumulate(const T* u0, const T* u
 
 int pres_sweep2_grid(int X, int Y, int Z, int zchunk, int by) { return pres_sweep2_blocks(X, Y, Z, zchunk, by); }
 int pres_sweep3_grid(int X, int Y, int Z, int zchunk) { return pres_sweep3_blocks(X, Y, Z, zchunk); }
+int pres_sweep4_grid(int X, int Y, int Z, int zchunk) { return pres_sweep4_blocks(X, Y, Z, zchunk); }
 
 // ---------------------------------------------------------------------------
 // CSInitVelPresFieldsFromBGFlowRotate (InitShaderColl.compute:184-226), the

@@ -74,6 +74,7 @@ struct Solver : SolverBase {
         }
         // the three-sweep pressure kernel: FP32
         bool fused3() const { return std::is_same<T, float>::value && use_fusion3(s->fuse_mode); }
+        bool fused4() const { return std::is_same<T, float>::value && use_fusion4(s->fuse_mode, s->p.N); }
     };
     int fuse_mode = fusion_mode();  // two-sweep pressure kernel: see fusion_mode()
 
@@ -558,6 +559,11 @@ struct Solver : SolverBase {
         }
         return UWS_OK;
     }
+    void launch_plan(int* pres, int* vel) override {
+        const L l{this};
+        *pres = l.fused4() ? 4 : l.fused3() ? 3 : l.fused() ? 2 : 1;
+        *vel = l.fused_vel() ? 2 : 1;
+    }
     int time_kernel(int kernel, int reps, double* avg_ms) override {
         if (rs.cur >= 0) return UWS_ERR_INVALID_ARG;
         hipEvent_t e0, e1;
@@ -565,7 +571,7 @@ struct Solver : SolverBase {
         if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return UWS_ERR_DEVICE; }
         const int w0 = (rs.state + 1) % 3;
         if ((kernel == UWS_KERNEL_VEL_SWEEP2 && !std::is_same<T, float>::value) ||
-            (kernel == UWS_KERNEL_PRES_SWEEP3 && !std::is_same<T, float>::value)) {
+            ((kernel == UWS_KERNEL_PRES_SWEEP3 || kernel == UWS_KERNEL_PRES_SWEEP4) && !std::is_same<T, float>::value)) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             return UWS_ERR_UNSUPPORTED;
@@ -581,6 +587,9 @@ struct Solver : SolverBase {
             } else if (kernel == UWS_KERNEL_PRES_SWEEP3) {
                 r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
                 launch<T>(KN_PRES_SWEEP3, p, f, r, stream);
+            } else if (kernel == UWS_KERNEL_PRES_SWEEP4) {
+                r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
+                launch<T>(KN_PRES_SWEEP4, p, f, r, stream);
             } else if (kernel == UWS_KERNEL_PRES_SWEEP2) {
                 r.coef = 0; r.pin = i & 1; r.pout = (i + 1) & 1;
                 launch<T>(KN_PRES_SWEEP2, p, f, r, stream);

@@ -361,6 +361,13 @@ class FvmSolverHip(IFvmSolver):
         N.check(self._lib.uws_get_sweep_counts(self._h, byref(v), byref(p)), "uws_get_sweep_counts")
         return v.value, p.value
 
+    def launch_plan(self):
+        """(pressure, momentum) sweeps per launch the step's multi-sweep
+        policy picks for this handle (uws_get_launch_plan)."""
+        pr, ve = ctypes.c_int32(), ctypes.c_int32()
+        N.check(self._lib.uws_get_launch_plan(self._h, byref(pr), byref(ve)), "uws_get_launch_plan")
+        return pr.value, ve.value
+
     def field64(self, fid: int) -> np.ndarray:
         nc = N.FIELD_NCOMP[fid]
         out = np.empty(self.n * nc, dtype=np.float64)
