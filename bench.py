@@ -23,7 +23,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per cell per launch read by rocprofv3 FETCH_SIZE (x2, calibrated on a
 # 256 MiB copy) + WRITE_SIZE on the urban case: profiles/urban_256x96x128_hbm_traffic.md
 MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 76.0, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.2,
-                                         "k_pres_sweep3": 35.4, "k_pres_sweep4": 36.2, "k_vel_sweep2": 85.7}}
+                                         "k_pres_sweep3": 35.1, "k_pres_sweep4": 36.2, "k_vel_sweep2": 85.7}}
 
 
 def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
