@@ -43,6 +43,8 @@ def test_urban_two_steps_match_oracle(gpu, monkeypatch):
     print(f"[configs] {w.name}: {w.ncells} cells, oracle on the host", flush=True)
     o = oracle_for(w)
     with solver_for(w) as s:
+        # the production launch plan at this size: four-sweep pressure launches, momentum pairs
+        assert s.launch_plan() == (4, 2)
         for k in (1, 2):
             s.Step()
             o.step(1)
