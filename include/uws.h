@@ -394,11 +394,13 @@ int uws_batch_step(uws_batch_handle b, int32_t nsteps);
 int uws_batch_synchronize(uws_batch_handle b);
 /* Launch shape of the ensemble kernels: tile rows ty in {1, 2, 4} (tiles of
  * 64 x ty cells), persistent != 0 for a grid-stride launch of a fixed number
- * of workgroups instead of one workgroup per tile, and group_bytes: consecutive
- * cases are grouped while their momentum-sweep working set (77 B per cell)
- * stays within group_bytes (<= 0: one group), each group runs its whole step
- * (one launch per kernel for the group) before the next group, so that a
- * group's Jacobi sweeps are served from the last-level cache. */
+ * of workgroups instead of one workgroup per tile, and group_bytes: > 0,
+ * consecutive cases are grouped while their momentum-sweep working set (77 B
+ * per cell) stays within group_bytes; 0, one group; < 0, the cases are packed
+ * into the fewest groups whose two-sweep launches need at most one workgroup
+ * per CU (a case larger than that is a group of its own).  Each group runs
+ * its whole step (one launch per kernel for the group) before the next group,
+ * so that a group's Jacobi sweeps are served from the last-level cache. */
 int uws_batch_set_launch(uws_batch_handle b, int32_t ty, int32_t persistent, int64_t group_bytes);
 int uws_batch_num_groups(uws_batch_handle b, int32_t* n);
 /* Autotuner: times `steps` ensemble steps for each candidate launch shape

@@ -43,9 +43,17 @@ def ensemble_run(ws, steps, ty=None, persistent=False, autotune=False, group_byt
         return [(e.velocity(c, 3), e.pressure(c)) for c in range(len(ws))]
 
 
-@pytest.mark.parametrize("ty,persistent,group_bytes", [(4, False, 0), (2, False, 0), (1, False, 0), (4, True, 0),
-                                                       (1, True, 0), (4, False, 1), (2, True, 400_000)])
-def test_ensemble_matches_single_cases_bitwise(gpu, ty, persistent, group_bytes):
+@pytest.mark.parametrize("ty,persistent,group_bytes,pack", [(4, False, 0, None), (2, False, 0, None),
+                                                            (1, False, 0, None), (4, True, 0, None),
+                                                            (1, True, 0, None), (4, False, 1, None),
+                                                            (2, True, 400_000, None), (4, False, -1, None),
+                                                            (4, False, -1, 2), (1, True, -1, 1)])
+def test_ensemble_matches_single_cases_bitwise(gpu, monkeypatch, ty, persistent, group_bytes, pack):
+    """pack: workgroups per packed group (UWS_PACK_BLOCKS), small enough
+    that the packed mode (group_bytes < 0) forms several groups of these
+    small cases"""
+    if pack is not None:
+        monkeypatch.setenv("UWS_PACK_BLOCKS", str(pack))
     ws = small_cases()
     ref = singles(ws, 4)
     got = ensemble_run(ws, 4, ty, persistent, group_bytes=group_bytes)
@@ -71,7 +79,7 @@ def test_ensemble_rejects_mixed_schedules(gpu):
 
 
 @pytest.mark.parametrize("group_bytes,vel_iters,zchunks", [(0, 6, None), (1, 6, None), (0, 4, None), (1, 5, None),
-                                                             (0, 5, (3, 1))])
+                                                             (0, 5, (3, 1)), (-1, 6, None), (-1, 5, (3, 1))])
 def test_ensemble_two_sweep_kernel_matches_single_cases(gpu, monkeypatch, group_bytes, vel_iters, zchunks):
     """With the multi-sweep kernels forced at every size (momentum pairs,
     pressure pairs and triples), the ensemble's launches over every case's

@@ -33,7 +33,7 @@ TOL = 1e-5  # FP32 fields after a step, relative L2
 
 def _no_overrides(monkeypatch):
     for k in ("UWS_SWEEP_FUSION", "UWS_SWEEP_DEPTH", "UWS_VEL_PAIRS", "UWS_ZCHUNK", "UWS_ZCHUNK3", "UWS_ZCHUNK4",
-              "UWS_TILE_ROWS2", "UWS_NO_P2", "UWS_NO_GRAPH"):
+              "UWS_TILE_ROWS2", "UWS_NO_P2", "UWS_NO_GRAPH", "UWS_PACK_BLOCKS"):
         monkeypatch.delenv(k, raising=False)
 
 
@@ -95,12 +95,16 @@ def test_ensemble_24_cases_equal_single_cases(gpu, monkeypatch):
             s.step(steps)
             ref.append((s.velocity(3), s.pressure()))
     print(f"[configs] ensemble: {len(ws)} cases, {sum(w.ncells for w in ws)} cells", flush=True)
-    for tune in (False, True):
+    for mode in ("one group", "packed", "autotuned"):
         with FvmEnsembleHip([w.cf for w in ws]) as e:
             for c, w in enumerate(ws):
                 e.init_flags(c, w.flags)
                 e.set_velocity(c, noise_velocity(w.flags, w.noise_seed, w.noise_amp))
-            if tune:
+            if mode == "packed":
+                # groups of at most one round of two-sweep workgroups each
+                e.set_launch(4, False, -1)
+                assert e.num_groups() == 6
+            if mode == "autotuned":
                 print(f"  autotuned launch: {e.autotune(1)}", flush=True)
             e.step(steps)
             for c in range(len(ws)):

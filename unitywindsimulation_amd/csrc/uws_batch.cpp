@@ -49,7 +49,7 @@ struct uws_batch {
     RoleState rs{};
     int ty = 4;
     int persistent = 0;
-    int64_t group_bytes = 0;  // <= 0: one group for the whole ensemble
+    int64_t group_bytes = 0;  // 0: one group; > 0: consecutive cases within a byte budget; < 0: packed rounds
     int fuse_mode = fusion_mode();  // two-sweep pressure kernel: see fusion_mode()
     std::vector<BatchGroup> groups;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
@@ -69,9 +69,58 @@ struct uws_batch {
     ParamsT<float>& params(int c) { return *static_cast<ParamsT<float>*>(cases[c]->device_params()); }
     FieldsT<float>& fields(int c) { return *static_cast<FieldsT<float>*>(cases[c]->device_fields()); }
 
+    // Cases packed into launch groups of at most one round of two-sweep
+    // workgroups each (one 16-wave workgroup per CU, at the 16-plane z chunk
+    // and 13-row tiles every group of this mode uses): the fewest groups that
+    // fit, found by placing the cases, largest first, each into the least
+    // loaded group it fits (a case larger than a round gets a group of its
+    // own).  A group of about one round keeps every launch of it a single
+    // wave of workgroups that start together, and a working set (3M cells:
+    // 200 MB of momentum arrays) that the 256 MB last-level cache holds
+    // across its sweeps.
+    std::vector<std::vector<int>> pack_rounds(const std::vector<ParamsT<float>>& shaped) const {
+        const int n = (int)shaped.size();
+        // workgroups per group: one round (UWS_PACK_BLOCKS overrides, for tests)
+        int cap = FUSE_MIN_BLOCKS;
+        if (const char* e = std::getenv("UWS_PACK_BLOCKS")) cap = std::max(1, std::atoi(e));
+        std::vector<int> w(n), order(n);
+        int64_t total = 0;
+        for (int c = 0; c < n; ++c) {
+            const ParamsT<float>& p = shaped[c];
+            w[c] = pres_sweep2_grid(p.X, p.Y, p.Z, PACK_ZCHUNK, p.by2);
+            total += std::min(w[c], cap);
+            order[c] = c;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+        for (int64_t nb = std::max<int64_t>(1, (total + cap - 1) / cap);; ++nb) {
+            std::vector<std::vector<int>> bins((size_t)nb);
+            std::vector<int> load((size_t)nb, 0);
+            bool ok = true;
+            for (int c : order) {
+                int best = -1;
+                for (int k = 0; k < (int)nb; ++k)
+                    if ((load[k] + w[c] <= cap || load[k] == 0) && (best < 0 || load[k] < load[best]))
+                        best = k;
+                if (best < 0) { ok = false; break; }
+                bins[best].push_back(c);
+                load[best] += w[c];
+            }
+            if (!ok) continue;
+            std::vector<std::vector<int>> out;
+            for (auto& b : bins)
+                if (!b.empty()) {
+                    std::sort(b.begin(), b.end());
+                    out.push_back(b);
+                }
+            return out;
+        }
+    }
+    static constexpr int PACK_ZCHUNK = 16;
+
     // Rebuild the launch groups and their tile tables for the current launch
-    // shape.  A group collects consecutive cases while the momentum sweep's
-    // working set (77 B per cell) stays within group_bytes.
+    // shape.  group_bytes > 0: a group collects consecutive cases while the
+    // momentum sweep's working set (77 B per cell) stays within group_bytes;
+    // 0: one group; < 0: pack_rounds.
     int build_table() {
         drop_graphs();
         if (stream) (void)hipStreamSynchronize(stream);
@@ -89,23 +138,34 @@ struct uws_batch {
             // one launch serves every case: the full two-sweep tile height
             p.by2 = 15;
         }
-        int64_t acc = 0;
-        for (int c = 0; c < n; ++c) {
-            const int64_t b = 77LL * cases[c]->ncells();
-            if (groups.empty() || (group_bytes > 0 && acc + b > group_bytes)) {
+        const bool packed = group_bytes < 0;
+        if (packed) {
+            for (auto& cs : pack_rounds(shaped)) {
                 groups.emplace_back();
-                acc = 0;
+                groups.back().cases = cs;
             }
-            groups.back().cases.push_back(c);
-            acc += b;
+        } else {
+            int64_t acc = 0;
+            for (int c = 0; c < n; ++c) {
+                const int64_t b = 77LL * cases[c]->ncells();
+                if (groups.empty() || (group_bytes > 0 && acc + b > group_bytes)) {
+                    groups.emplace_back();
+                    acc = 0;
+                }
+                groups.back().cases.push_back(c);
+                acc += b;
+            }
         }
         for (auto& g : groups) {
             const int m = (int)g.cases.size();
             // z chunk of the group's two-sweep launches, chosen over the whole
             // group as choose_zchunk does for one grid: the longest of 16, 8
             // and 4 planes that still gives a workgroup per CU, preferring one
-            // that divides every case's Z (UWS_ZCHUNK overrides, for tuning)
-            if (!std::getenv("UWS_ZCHUNK")) {
+            // that divides every case's Z (UWS_ZCHUNK overrides, for tuning);
+            // packed groups were sized at the 16-plane chunk
+            if (packed && !std::getenv("UWS_ZCHUNK")) {
+                for (int c : g.cases) shaped[c].zchunk = PACK_ZCHUNK;
+            } else if (!std::getenv("UWS_ZCHUNK")) {
                 int zbest = 8, sbest = -1;
                 for (int zc : {16, 8, 4}) {
                     int blocks = 0;
@@ -179,7 +239,9 @@ struct uws_batch {
             // does (first_ps2), with the group's z chunk
             int zmin = ps[0].zchunk;
             for (const auto& q : ps) zmin = std::min(zmin, q.zchunk);
-            g.vel_pairs = use_vel_pairs(fuse_mode, g.tab.total_ps2, zmin);
+            // a packed group is sized to one round: it takes pairs when it
+            // fills at least half of it
+            g.vel_pairs = use_vel_pairs(fuse_mode, packed ? 2 * g.tab.total_ps2 : g.tab.total_ps2, zmin);
             if (persistent) {
                 // 8 XCDs x 32 CUs, up to 8 resident 256-thread workgroups per CU
                 int gr = 256 * 8 * (4 / ty);
@@ -434,7 +496,7 @@ int uws_batch_autotune(uws_batch_handle b, int32_t steps, int64_t best[3], doubl
     }
     // candidate launch shapes: (tile rows, persistent, group budget in bytes)
     const int64_t MB = 1 << 20;
-    const int64_t budgets[4] = {0, 384 * MB, 192 * MB, 96 * MB};
+    const int64_t budgets[5] = {-1, 0, 384 * MB, 192 * MB, 96 * MB};
     const int shapes[3][2] = {{4, 0}, {2, 0}, {4, 1}};
     double bestv = 1e300;
     int64_t bb[3] = {4, 0, 0};

@@ -1245,13 +1245,26 @@ inline __host__ __device__ int pres_sweep2_blocks(int X, int Y, int Z, int zchun
 // registers than a single-case one)
 // *(base + off) for a byte offset off (uniform base: the SGPR-base form of
 // the global load, one 32-bit offset register shared by arrays)
+// Every array is device (global) memory: the accesses of scalar values say
+// so, so that they compile to global loads and stores also where the array
+// pointer itself was read from memory (an ensemble's per-case tables), not
+// only where it came in as a kernel argument; a generic (flat) access would
+// need 64-bit VGPR addresses and wait on the LDS counter too.  The two-double
+// types of the complex-step and dual-number runs keep generic accesses.
+#define UWS_GLOBAL __attribute__((address_space(1)))
 template <class T>
 __device__ __forceinline__ T ld_at(const T* base, uint32_t off) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+    if constexpr (std::is_scalar_v<T>)
+        return *(const UWS_GLOBAL T*)((const UWS_GLOBAL char*)base + off);
+    else
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
 }
 template <class T>
 __device__ __forceinline__ void st_at(T* base, uint32_t off, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+    if constexpr (std::is_scalar_v<T>)
+        *(UWS_GLOBAL T*)((UWS_GLOBAL char*)base + off) = v;
+    else
+        *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
 }
 template <class T>
 struct PS2Args {
@@ -2030,7 +2043,7 @@ __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* 
         r.GE = ld_at(p.G[0], oP); r.GN = ld_at(p.G[1], oP); r.GU = ld_at(p.G[2], oP);
         r.SW = ld_at(p.S[0], oW); r.GW = ld_at(p.G[0], oW);
         r.SS = ld_at(p.S[1], oS); r.GS = ld_at(p.G[1], oS);
-        r.fl = *(p.flag + oP / (uint32_t)sizeof(T));
+        r.fl = ld_at(p.flag, oP / (uint32_t)sizeof(T));
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             r.inv[a] = ld_at(p.invD[a], oP);
