@@ -1207,6 +1207,24 @@ __device__ __forceinline__ double block_sum(double v) {
 // expressions and operation order of body_pres_sweep, so the result equals
 // two single sweeps bit for bit.
 // ---------------------------------------------------------------------------
+// Thread layout of a z-marching tile W = 64 + 2H cells wide and BY rows
+// tall: the first 64 BY threads cover the tile's 64 interior columns, one
+// wave per row (so every load and store of a wave is one aligned 256-byte
+// row segment of the grid), the other 2 H BY threads the H halo columns on
+// each side, row by row.  (lx, ly): the thread's cell in the tile.
+__device__ __forceinline__ void tile_lane(int t, int H, int BY, int& lx, int& ly) {
+    const int nmain = 64 * BY;
+    if (t < nmain) {
+        lx = H + (t & 63);
+        ly = t >> 6;
+    } else {
+        const int h = t - nmain;
+        ly = h / (2 * H);
+        const int s = h - ly * 2 * H;
+        lx = s < H ? s : 64 + s;
+    }
+}
+
 constexpr int PS2_W = 66;                   // threads per tile row: 64 outputs + 2 halo
 constexpr int PS2_BY = 15;                  // most tile rows (13 outputs + 2 halo); a grid uses by2 <= PS2_BY
 constexpr int PS2_THREADS = PS2_W * PS2_BY;  // 990: the LDS stride of a plane
@@ -1298,8 +1316,9 @@ __device__ __forceinline__ PS2Args<T> ps2_args(const ParamsT<T>& p, const Fields
 // types of up to 8 bytes (N < 2^29 cells); wider types index.
 template <class T, bool ZIN, bool OFF = (sizeof(T) <= 8)>
 __device__ __forceinline__ void pres_sweep2_block(const PS2Args<T>& p, int b, T* lds) {
-    const int tid = (int)threadIdx.x;
-    const int lx = tid % PS2_W, ly = tid / PS2_W;
+    int lx, ly;
+    tile_lane((int)threadIdx.x, 1, p.by, lx, ly);
+    const int tid = ly * PS2_W + lx;  // the thread's entry in the LDS planes
     const int ntx = (p.X + 63) / 64, nty = (p.Y + p.by - 3) / (p.by - 2);
     const int bx = b % ntx;
     b /= ntx;
@@ -1997,8 +2016,9 @@ constexpr int VS2_LDS_FLOATS = (3 + 3) * 3 * PS2_THREADS;
 // h[1][y], h[2][z] instead of ds / 4
 template <class T, bool NU = false>
 __device__ __forceinline__ void vel_sweep2_block(const VS2Args<T>& p, int b, T* lds) {
-    const int tid = (int)threadIdx.x;
-    const int lx = tid % PS2_W, ly = tid / PS2_W;
+    int lx, ly;
+    tile_lane((int)threadIdx.x, 1, p.by, lx, ly);
+    const int tid = ly * PS2_W + lx;  // the thread's entry in the LDS planes
     const int ntx = (p.X + 63) / 64, nty = (p.Y + p.by - 3) / (p.by - 2);
     const int bx = b % ntx;
     b /= ntx;
