@@ -22,8 +22,8 @@ METRIC = "MLUPS per PISO step + %HBM peak; batched vs plain cases/s at ≤1e-3 r
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per cell per launch read by rocprofv3 FETCH_SIZE (x2, calibrated on a
 # 256 MiB copy) + WRITE_SIZE on the urban case: profiles/urban_256x96x128_hbm_traffic.md
-MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 76.0, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.2,
-                                         "k_pres_sweep3": 35.1, "k_pres_sweep4": 36.2, "k_vel_sweep2": 85.7}}
+MEASURED_TRAFFIC_B_PER_CELL = {"urban": {"k_vel_sweep": 75.9, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.2,
+                                         "k_pres_sweep3": 35.2, "k_pres_sweep4": 36.2, "k_vel_sweep2": 84.6}}
 
 
 def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
@@ -116,12 +116,15 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
             "cpu_model": cpu_model}
 
 
-def ensemble_report(steps: int = 50, check_steps: int = 500) -> dict:
+def ensemble_report(steps: int = 50, check_steps: int = 500, launch: str = "auto") -> dict:
     """The 24-case uneven ensemble: cases/s of the batched ensemble (one graph
     per step over all cases) against the same cases stepped one after another,
     each with its own graph per step ("plain") and with one host launch per
     kernel ("eager_plain", how the reference dispatches), and the FP32
-    ensemble's velocity error against FP64 runs."""
+    ensemble's velocity error against FP64 runs after check_steps steps (0:
+    not checked).  launch: "auto" (the autotuner picks the launch shape),
+    "packed" (groups of at most one round of two-sweep workgroups) or "one"
+    (one group)."""
     import numpy as np
 
     from unitywindsimulation_amd import FvmSolverHip, workloads
@@ -158,7 +161,11 @@ def ensemble_report(steps: int = 50, check_steps: int = 500) -> dict:
         for c, w in enumerate(ws):
             e.init_flags(c, w.flags)
             e.init_noise(c, w.noise_seed, w.noise_amp)
-        ty, pers, gbytes, _ = e.autotune(2)
+        if launch == "auto":
+            ty, pers, gbytes, _ = e.autotune(2)
+        else:
+            ty, pers, gbytes = 4, False, (-1 if launch == "packed" else 0)
+            e.set_launch(ty, pers, gbytes)
         ngroups = e.num_groups()
         e.step(1)
         e.synchronize()
@@ -168,21 +175,23 @@ def ensemble_report(steps: int = 50, check_steps: int = 500) -> dict:
         t_batch = (time.perf_counter() - t0) / steps
     # accuracy: FP32 ensemble vs FP64 single runs after check_steps steps
     errs = []
-    with FvmEnsembleHip([w.cf for w in ws]) as e:
-        for c, w in enumerate(ws):
-            e.init_flags(c, w.flags)
-            e.init_noise(c, w.noise_seed, w.noise_amp)
-        e.step(check_steps)
-        for c, w in enumerate(ws):
-            w.cf.precision = N.PREC_FP64
-            with FvmSolverHip(w.cf) as d:
-                d.InitFlags(w.flags)
-                d.init_noise(w.noise_seed, w.noise_amp)
-                d.step(check_steps)
-                ref = d.field64(N.FIELD_VEL)
-            w.cf.precision = N.PREC_FP32
-            got = e.velocity(c, 3).reshape(-1, 3).astype(np.float64)
-            errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    if check_steps > 0:
+        with FvmEnsembleHip([w.cf for w in ws]) as e:
+            for c, w in enumerate(ws):
+                e.init_flags(c, w.flags)
+                e.init_noise(c, w.noise_seed, w.noise_amp)
+            e.set_launch(ty, pers, gbytes)
+            e.step(check_steps)
+            for c, w in enumerate(ws):
+                w.cf.precision = N.PREC_FP64
+                with FvmSolverHip(w.cf) as d:
+                    d.InitFlags(w.flags)
+                    d.init_noise(w.noise_seed, w.noise_amp)
+                    d.step(check_steps)
+                    ref = d.field64(N.FIELD_VEL)
+                w.cf.precision = N.PREC_FP32
+                got = e.velocity(c, 3).reshape(-1, 3).astype(np.float64)
+                errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
     cells = sum(w.ncells for w in ws)
     # bound: every step must move each case's compulsory bytes; no launch shape moves them faster than
     # the plain streaming kernel does on any working set these cases form (one case's arrays, which the
@@ -209,8 +218,11 @@ def ensemble_report(steps: int = 50, check_steps: int = 500) -> dict:
             "speedup_vs_eager_plain": round(t_eager / t_batch, 3),
             "plain_ms_per_step_all_cases": round(t_plain * 1e3, 3), "batched_ms_per_step": round(t_batch * 1e3, 3),
             "eager_plain_ms_per_step_all_cases": round(t_eager * 1e3, 3),
-            "launch": {"tile_rows": ty, "persistent": pers, "group_MiB": gbytes >> 20, "groups": ngroups},
-            "max_rel_l2_u_vs_fp64": max(errs), "compared_after_steps": check_steps}
+            "launch": {"tile_rows": ty, "persistent": pers, "groups": ngroups,
+                       "grouping": ("packed: at most one round of two-sweep workgroups per group" if gbytes < 0
+                                    else "one group" if gbytes == 0 else f"consecutive cases within {gbytes >> 20} MiB"),
+                       "chosen_by": "autotuner" if launch == "auto" else "fixed"},
+            "max_rel_l2_u_vs_fp64": max(errs) if errs else None, "compared_after_steps": check_steps}
 
 
 def precision_report(steps: int = 500, mean_last: int = 250, only=None) -> dict:
@@ -451,6 +463,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 24-case ensemble leg")
+    ap.add_argument("--ensemble-steps", type=int, default=50, help="timed steps of the ensemble legs")
+    ap.add_argument("--ensemble-check-steps", type=int, default=500,
+                    help="steps after which the ensemble is compared with FP64 runs (0: no comparison)")
+    ap.add_argument("--ensemble-launch", default="auto", choices=("auto", "packed", "one"),
+                    help="launch grouping of the batched ensemble (auto: the autotuner's choice)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation leg (adjoint vs complex step and central differences)")
     ap.add_argument("--adjoint-dirs", type=int, default=16)
@@ -580,12 +597,19 @@ def main() -> int:
                      "algorithmic_bytes": kbytes, "single_sweeps_bytes": kbytes_sweeps,
                      "kernel_ms": {"k_vel_sweep": round(t_vel, 5), "k_vel_sweep2": round(t_vel2, 5),
                                    "k_pres_sweep": round(t_pres, 5), "k_pres_sweep2": round(t_pres2, 5),
-                                   "k_pres_sweep3": round(t_pres3, 5), "k_pres_sweep4": round(t_pres4, 5)}},
+                                   "k_pres_sweep3": round(t_pres3, 5), "k_pres_sweep4": round(t_pres4, 5)},
+                     # the same fraction for every sweep kernel (algorithmic bytes of one launch over its
+                     # duration), and each one's share of the step (launches per step x duration)
+                     "frac_by_kernel": {k: round(b * ncell / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k, b, t in (
+                         ("k_vel_sweep", 77, t_vel), ("k_vel_sweep2", 73, t_vel2), ("k_pres_sweep", 29, t_pres),
+                         ("k_pres_sweep2", 28, t_pres2), ("k_pres_sweep3", 28, t_pres3),
+                         ("k_pres_sweep4", 28, t_pres4))},
+                     "step_ms_share": {"momentum_sweeps": round(share_vel, 4), "pressure_sweeps_main_kernel": round(share_pres, 4)}},
         "launch_plan": {"pres_sweeps_per_launch": pres_depth, "vel_sweeps_per_launch": vel_depth},
         "finite": finite,
     }
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_report()
+        out["ensemble"] = ensemble_report(args.ensemble_steps, args.ensemble_check_steps, args.ensemble_launch)
     if not args.no_stretched:
         out["stretched"] = stretched_report()
     if not args.no_precision:
