@@ -1798,14 +1798,18 @@ __device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b
         q.cS = hS ? q.cS : (T)0;
         if (zz < 0) q.cU = 0;
     };
-    // one sweep's value at a cell from the ring of the level below
-    auto sweep = [&](const PS2Plane<T>& r, const T* L, T vU, T vDn, T cD) {
-        const T vE = L[jE], vW = L[jW], vN = L[jN], vS = L[jS];
+    // a cell's in-plane neighbours in one ring plane
+    struct Nb {
+        T e, w, n, s;
+    };
+    auto nbr = [&](const T* L) { return Nb{L[jE], L[jW], L[jN], L[jS]}; };
+    // one sweep's value at a cell from its neighbours in the level below
+    auto sweep = [&](const PS2Plane<T>& r, const Nb& v, T vU, T vDn, T cD) {
         T t = 0;
-        t -= r.cE * vE;
-        t -= r.cW * vW;
-        t -= r.cN * vN;
-        t -= r.cS * vS;
+        t -= r.cE * v.e;
+        t -= r.cW * v.w;
+        t -= r.cN * v.n;
+        t -= r.cS * v.s;
         t -= r.cU * vU;
         t -= cD * vDn;
         return r.inv * (t + r.rhs);
@@ -1823,6 +1827,13 @@ __device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b
         const T dU4 = nx.cU;  // D coefficient of plane zz - 3
         T vD = 0;
         if (!ZIN) vD = r2.vU;  // the input iterate below plane zz (the record of zz - 2 read it)
+        // the ring planes the second to fourth sweeps read were written in
+        // the previous step, before its barrier: their reads are issued
+        // first, together, so that their latency overlaps the first sweep's
+        // wait for its record instead of following one another
+        const Nb a2 = nbr(L1 + S1 * PS4_THREADS);
+        const Nb a3 = nbr(L2 + S2 * PS4_THREADS);
+        const Nb a4 = nbr(L3 + S3 * PS4_THREADS);
         load(nx, zz + 1 <= z1 + 2 ? zz + 1 : z1 + 2);
         mask(r0, zz);
         // first sweep at zz
@@ -1837,22 +1848,20 @@ __device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b
         }
         const T s1 = r0.inv * (t1 + r0.rhs);
         L1[S0 * PS4_THREADS + tid] = s1;
-        // second sweep at zz - 1 (needed from z0 - 2)
+        // second sweep at zz - 1 (needed from z0 - 2; +0 below)
         const int z2 = zz - 1;
-        T s2 = 0;
-        if (zz >= z0 - 1)
-            s2 = sweep(r1, L1 + S1 * PS4_THREADS, z2 + 1 < p.Z ? s1 : s1m1, z2 > 0 ? s1m2 : s1m1, r2.cU);
+        const T s2v = sweep(r1, a2, z2 + 1 < p.Z ? s1 : s1m1, z2 > 0 ? s1m2 : s1m1, r2.cU);
+        const T s2 = zz >= z0 - 1 ? s2v : (T)0;
         L2[S1 * PS4_THREADS + tid] = s2;
-        // third sweep at zz - 2 (needed from z0 - 1)
+        // third sweep at zz - 2 (needed from z0 - 1; +0 below)
         const int z3 = zz - 2;
-        T s3 = 0;
-        if (zz >= z0 + 1)
-            s3 = sweep(r2, L2 + S2 * PS4_THREADS, z3 + 1 < p.Z ? s2 : s2m1, z3 > 0 ? s2m2 : s2m1, r3.cU);
+        const T s3v = sweep(r2, a3, z3 + 1 < p.Z ? s2 : s2m1, z3 > 0 ? s2m2 : s2m1, r3.cU);
+        const T s3 = zz >= z0 + 1 ? s3v : (T)0;
         L3[S2 * PS4_THREADS + tid] = s3;
         // fourth sweep at zz - 3 into the output
         const int z4 = zz - 3;
+        const T e4 = sweep(r3, a4, z4 + 1 < p.Z ? s3 : s3m1, z4 > 0 ? s3m2 : s3m1, dU4);
         if (inner && z4 >= z0 && z4 < z1) {
-            const T e4 = sweep(r3, L3 + S3 * PS4_THREADS, z4 + 1 < p.Z ? s3 : s3m1, z4 > 0 ? s3m2 : s3m1, dU4);
             // an inner thread's column is in the grid: colB is its own
             st_at(out, (uint32_t)z4 * planeB + colB, e4);
         }
