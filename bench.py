@@ -134,11 +134,15 @@ def ensemble_report(steps: int = 50, check_steps: int = 500, launch: str = "auto
     ws = workloads.ensemble()
     full_steps = ws[0].steps
 
-    def sequential(eager: bool) -> float:
+    def sequential(eager: bool, single_sweeps: bool = False, nsteps: int = steps) -> float:
         """seconds per step of all cases stepped one after another; eager: one host launch per
-        kernel (the reference's dispatch pattern), else one captured graph per step and case"""
+        kernel, else one captured graph per step and case; single_sweeps: one launch per Jacobi
+        sweep too (with eager, the reference's own dispatch pattern)"""
+        saved = {k: os.environ.get(k) for k in ("UWS_NO_GRAPH", "UWS_SWEEP_FUSION")}
         if eager:
             os.environ["UWS_NO_GRAPH"] = "1"
+        if single_sweeps:
+            os.environ["UWS_SWEEP_FUSION"] = "0"
         try:
             total = 0.0
             for w in ws:
@@ -148,15 +152,20 @@ def ensemble_report(steps: int = 50, check_steps: int = 500, launch: str = "auto
                     s.step(2)
                     s.synchronize()
                     t0 = time.perf_counter()
-                    s.step(steps)
+                    s.step(nsteps)
                     s.synchronize()
-                    total += (time.perf_counter() - t0) / steps
+                    total += (time.perf_counter() - t0) / nsteps
             return total
         finally:
-            os.environ.pop("UWS_NO_GRAPH", None)
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
 
     t_plain = sequential(eager=False)
     t_eager = sequential(eager=True)
+    t_ref = sequential(eager=True, single_sweeps=True, nsteps=max(1, min(steps, 10)))
     with FvmEnsembleHip([w.cf for w in ws]) as e:
         for c, w in enumerate(ws):
             e.init_flags(c, w.flags)
@@ -218,6 +227,11 @@ def ensemble_report(steps: int = 50, check_steps: int = 500, launch: str = "auto
             "speedup_vs_eager_plain": round(t_eager / t_batch, 3),
             "plain_ms_per_step_all_cases": round(t_plain * 1e3, 3), "batched_ms_per_step": round(t_batch * 1e3, 3),
             "eager_plain_ms_per_step_all_cases": round(t_eager * 1e3, 3),
+            # the same cases dispatched as FvmSolverGpu.Step does: one launch per kernel and per
+            # Jacobi sweep, no graph (a reference point; "speedup" above is against the plain
+            # solver as this library runs it)
+            "reference_dispatch_plain_ms_per_step_all_cases": round(t_ref * 1e3, 3),
+            "speedup_vs_reference_dispatch_plain": round(t_ref / t_batch, 3),
             "launch": {"tile_rows": ty, "persistent": pers, "groups": ngroups,
                        "grouping": ("packed: at most one round of two-sweep workgroups per group" if gbytes < 0
                                     else "one group" if gbytes == 0 else f"consecutive cases within {gbytes >> 20} MiB"),
