@@ -1,0 +1,27 @@
+"""The ensemble's launch-group packing (pack_groups in uws_host.h): a host
+program built from the header (tests/cpp/pack_groups_test.cpp, hipcc for the
+host only; nothing runs on a GPU) checks the partition, the capacity, the
+lower bound and, on small random instances, the exhaustive optimum."""
+from __future__ import annotations
+
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+SRC = ROOT / "tests" / "cpp" / "pack_groups_test.cpp"
+EXE = ROOT / "tests" / "cpp" / "pack_groups_test"
+CSRC = ROOT / "unitywindsimulation_amd" / "csrc"
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+@pytest.mark.skipif(not Path(HIPCC).exists(), reason="hipcc is needed to build the host program")
+def test_pack_groups():
+    subprocess.run([HIPCC, "-std=c++17", "-O1", f"-I{CSRC}", str(SRC), "-o", str(EXE)], check=True,
+                   capture_output=True, text=True, timeout=600)
+    res = subprocess.run([str(EXE)], capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0, res.stdout
+    assert "cases passed" in res.stdout

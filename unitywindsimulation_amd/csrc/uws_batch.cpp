@@ -79,41 +79,15 @@ struct uws_batch {
     // 200 MB of momentum arrays) that the 256 MB last-level cache holds
     // across its sweeps.
     std::vector<std::vector<int>> pack_rounds(const std::vector<ParamsT<float>>& shaped) const {
-        const int n = (int)shaped.size();
         // workgroups per group: one round (UWS_PACK_BLOCKS overrides, for tests)
         int cap = FUSE_MIN_BLOCKS;
         if (const char* e = std::getenv("UWS_PACK_BLOCKS")) cap = std::max(1, std::atoi(e));
-        std::vector<int> w(n), order(n);
-        int64_t total = 0;
-        for (int c = 0; c < n; ++c) {
+        std::vector<int> w(shaped.size());
+        for (size_t c = 0; c < shaped.size(); ++c) {
             const ParamsT<float>& p = shaped[c];
             w[c] = pres_sweep2_grid(p.X, p.Y, p.Z, PACK_ZCHUNK, p.by2);
-            total += std::min(w[c], cap);
-            order[c] = c;
         }
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
-        for (int64_t nb = std::max<int64_t>(1, (total + cap - 1) / cap);; ++nb) {
-            std::vector<std::vector<int>> bins((size_t)nb);
-            std::vector<int> load((size_t)nb, 0);
-            bool ok = true;
-            for (int c : order) {
-                int best = -1;
-                for (int k = 0; k < (int)nb; ++k)
-                    if ((load[k] + w[c] <= cap || load[k] == 0) && (best < 0 || load[k] < load[best]))
-                        best = k;
-                if (best < 0) { ok = false; break; }
-                bins[best].push_back(c);
-                load[best] += w[c];
-            }
-            if (!ok) continue;
-            std::vector<std::vector<int>> out;
-            for (auto& b : bins)
-                if (!b.empty()) {
-                    std::sort(b.begin(), b.end());
-                    out.push_back(b);
-                }
-            return out;
-        }
+        return pack_groups(w, cap);
     }
     static constexpr int PACK_ZCHUNK = 16;
 

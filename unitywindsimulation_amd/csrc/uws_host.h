@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdint>
@@ -135,6 +136,49 @@ inline bool use_fusion4(int mode, int cells) {
     if (e && (e[0] == '2' || e[0] == '3')) return false;
     return mode == 1 && cells >= QUAD_MIN_CELLS;
 }
+// Items of weights w (an ensemble's cases, weighed in workgroups) into the
+// fewest groups whose weights sum to at most cap; an item above cap is a
+// group of its own.  For a group count from the lower bound upwards, each
+// item, largest first, goes to the least loaded group it fits; the first
+// count that places every item is taken.  Each group lists its items in
+// increasing order; empty groups are dropped.
+inline std::vector<std::vector<int>> pack_groups(const std::vector<int>& w, int cap) {
+    const int n = (int)w.size();
+    if (n == 0) return {};
+    if (cap < 1) cap = 1;
+    std::vector<int> order(n);
+    int64_t total = 0;
+    for (int c = 0; c < n; ++c) {
+        total += std::min(w[c], cap);
+        order[c] = c;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+    for (int64_t nb = std::max<int64_t>(1, (total + cap - 1) / cap);; ++nb) {
+        std::vector<std::vector<int>> bins((size_t)nb);
+        std::vector<int64_t> load((size_t)nb, 0);
+        bool ok = true;
+        for (int c : order) {
+            int best = -1;
+            for (int k = 0; k < (int)nb; ++k)
+                if ((load[k] + w[c] <= cap || load[k] == 0) && (best < 0 || load[k] < load[best])) best = k;
+            if (best < 0) {
+                ok = false;
+                break;
+            }
+            bins[best].push_back(c);
+            load[best] += w[c];
+        }
+        if (!ok) continue;  // with nb = n every item has a group of its own
+        std::vector<std::vector<int>> out;
+        for (auto& b : bins)
+            if (!b.empty()) {
+                std::sort(b.begin(), b.end());
+                out.push_back(b);
+            }
+        return out;
+    }
+}
+
 inline bool use_vel_pairs(int mode, int blocks, int zchunk) {
     if (const char* e = std::getenv("UWS_VEL_PAIRS")) return e[0] == '1';
     return mode == 2 || (mode == 1 && blocks >= FUSE_MIN_BLOCKS && zchunk >= VEL_PAIR_MIN_ZCHUNK);
