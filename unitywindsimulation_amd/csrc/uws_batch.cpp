@@ -196,7 +196,9 @@ struct uws_batch {
                 hipMemcpyAsync(g.d_params, ps.data(), sizeof(ParamsT<float>) * m, hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
                 hipMemcpyAsync(g.d_fields, fs.data(), sizeof(FieldsT<float>) * m, hipMemcpyHostToDevice, stream) !=
-                    hipSuccess)
+                    hipSuccess ||
+                // the host tables above go out of scope with this iteration
+                hipStreamSynchronize(stream) != hipSuccess)
                 return UWS_ERR_DEVICE;
             g.tab.ncases = m;
             g.tab.total_tiles = first[m];
