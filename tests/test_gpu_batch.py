@@ -103,3 +103,22 @@ def test_ensemble_two_sweep_kernel_matches_single_cases(gpu, monkeypatch, group_
     for (ug, pg), (ur, pr) in zip(singles(ws, 3), ref):
         np.testing.assert_array_equal(ug, ur)
         np.testing.assert_array_equal(pg, pr)
+
+
+@pytest.mark.parametrize("group_bytes,zchunk4", [(0, None), (-1, None), (0, 3), (1, 1)])
+def test_ensemble_four_sweep_kernel_matches_single_sweeps(gpu, monkeypatch, group_bytes, zchunk4):
+    """With four-sweep pressure launches forced (UWS_SWEEP_DEPTH=4: every
+    solve of P = 8 sweeps runs two of them), the ensemble's launches over
+    every case's z-marching blocks equal the same cases stepped with one-sweep
+    kernels only, bit for bit.  zchunk4: z planes per four-sweep workgroup."""
+    ws = small_cases()
+    monkeypatch.setenv("UWS_SWEEP_FUSION", "0")
+    ref = singles(ws, 3)
+    monkeypatch.delenv("UWS_SWEEP_FUSION")
+    monkeypatch.setenv("UWS_SWEEP_DEPTH", "4")
+    if zchunk4 is not None:
+        monkeypatch.setenv("UWS_ZCHUNK4", str(zchunk4))
+    got = ensemble_run(ws, 3, 4, False, group_bytes=group_bytes)
+    for (ug, pg), (ur, pr) in zip(got, ref):
+        np.testing.assert_array_equal(ug, ur)
+        np.testing.assert_array_equal(pg, pr)

@@ -25,6 +25,8 @@ struct BatchGroup {
     int* d_first = nullptr;
     int* d_first2 = nullptr;
     int* d_first3 = nullptr;
+    int* d_first4 = nullptr;
+    bool quads = false;  // four-sweep pressure launches
     ParamsT<float>* d_params = nullptr;
     FieldsT<float>* d_fields = nullptr;
     void release() {
@@ -33,6 +35,8 @@ struct BatchGroup {
         d_first2 = nullptr;
         if (d_first3) (void)hipFree(d_first3);
         d_first3 = nullptr;
+        if (d_first4) (void)hipFree(d_first4);
+        d_first4 = nullptr;
         if (d_params) (void)hipFree(d_params);
         if (d_fields) (void)hipFree(d_fields);
         d_first = nullptr;
@@ -171,7 +175,23 @@ struct uws_batch {
                 }
                 for (int c : g.cases) shaped[c].zchunk3 = zbest;
             }
-            std::vector<int> first(m + 1, 0), first2(m + 1, 0), first3(m + 1, 0);
+            // z chunk of the group's four-sweep launches, the same way (three
+            // halo planes above and below a chunk)
+            if (!std::getenv("UWS_ZCHUNK4")) {
+                int zbest = 4, cbest = -1;
+                for (int zc = 4; zc <= 64; ++zc) {
+                    int blocks = 0;
+                    for (int c : g.cases) {
+                        const ParamsT<float>& p = shaped[c];
+                        blocks += pres_sweep4_grid(p.X, p.Y, p.Z, zc);
+                    }
+                    const int cost = ((blocks + 255) / 256) * (zc + 6);
+                    if (cbest < 0 || cost < cbest) { zbest = zc; cbest = cost; }
+                }
+                for (int c : g.cases) shaped[c].zchunk4 = zbest;
+            }
+            std::vector<int> first(m + 1, 0), first2(m + 1, 0), first3(m + 1, 0), first4(m + 1, 0);
+            int64_t min_cells = INT64_MAX;
             std::vector<ParamsT<float>> ps(m);
             std::vector<FieldsT<float>> fs(m);
             for (int i = 0; i < m; ++i) {
@@ -180,10 +200,13 @@ struct uws_batch {
                 first[i + 1] = first[i] + ps[i].tiles;
                 first2[i + 1] = first2[i] + pres_sweep2_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk, ps[i].by2);
                 first3[i + 1] = first3[i] + pres_sweep3_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk3);
+                first4[i + 1] = first4[i] + pres_sweep4_grid(ps[i].X, ps[i].Y, ps[i].Z, ps[i].zchunk4);
+                min_cells = std::min(min_cells, (int64_t)ps[i].X * ps[i].Y * ps[i].Z);
             }
             if (hipMalloc(&g.d_first, sizeof(int) * (m + 1)) != hipSuccess ||
                 hipMalloc(&g.d_first2, sizeof(int) * (m + 1)) != hipSuccess ||
                 hipMalloc(&g.d_first3, sizeof(int) * (m + 1)) != hipSuccess ||
+                hipMalloc(&g.d_first4, sizeof(int) * (m + 1)) != hipSuccess ||
                 hipMalloc(&g.d_params, sizeof(ParamsT<float>) * m) != hipSuccess ||
                 hipMalloc(&g.d_fields, sizeof(FieldsT<float>) * m) != hipSuccess)
                 return UWS_ERR_ALLOC;
@@ -192,6 +215,8 @@ struct uws_batch {
                 hipMemcpyAsync(g.d_first2, first2.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
                 hipMemcpyAsync(g.d_first3, first3.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
+                    hipSuccess ||
+                hipMemcpyAsync(g.d_first4, first4.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
                 hipMemcpyAsync(g.d_params, ps.data(), sizeof(ParamsT<float>) * m, hipMemcpyHostToDevice, stream) !=
                     hipSuccess ||
@@ -209,6 +234,12 @@ struct uws_batch {
             g.tab.first_ps2 = g.d_first2;
             g.tab.total_ps3 = first3[m];
             g.tab.first_ps3 = g.d_first3;
+            g.tab.total_ps4 = first4[m];
+            g.tab.first_ps4 = g.d_first4;
+            // four-sweep launches where every case's grid would take them alone
+            // (the 24-case ensemble, of grids of 0.26M to 1.2M cells, ran 6 %
+            // slower with them than with triples in its packed groups)
+            g.quads = use_fusion4(fuse_mode, min_cells > INT32_MAX ? INT32_MAX : (int)min_cells);
             g.tab.params = g.d_params;
             g.tab.fields = g.d_fields;
             // the momentum pair tiles the grid as the two-sweep pressure kernel
@@ -252,7 +283,7 @@ struct uws_batch {
         bool fused() const { return use_fusion(b->fuse_mode); }
         bool fused_vel() const { return g->vel_pairs; }
         bool fused3() const { return use_fusion3(b->fuse_mode); }
-        bool fused4() const { return false; }  // single cases only
+        bool fused4() const { return g->quads; }
     };
 
     void sync_case_roles() {

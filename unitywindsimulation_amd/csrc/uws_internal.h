@@ -98,6 +98,8 @@ struct BatchTable {
     const int* first_ps2;       // [ncases + 1]: first two-sweep workgroup of each case
     int total_ps3;              // workgroups of the three-sweep pressure kernel over all cases
     const int* first_ps3;       // [ncases + 1]: first three-sweep workgroup of each case
+    int total_ps4;              // workgroups of the four-sweep pressure kernel over all cases
+    const int* first_ps4;       // [ncases + 1]: first four-sweep workgroup of each case
     const ParamsT<T>* params;   // [ncases]
     const FieldsT<T>* fields;   // [ncases]
 };

@@ -2264,6 +2264,22 @@ __global__ __launch_bounds__(1024) void k_batch_pres_sweep3(BatchTable<T> tab, R
     pres_sweep3_block<T, ZIN>(a, t - tab.first_ps3[lo], reinterpret_cast<T*>(lds_raw));
 }
 
+// four-sweep pressure kernel over every case of an ensemble: workgroup t
+// belongs to the case c with first_ps4[c] <= t < first_ps4[c + 1]
+template <bool ZIN>
+__global__ __launch_bounds__(1024) void k_batch_pres_sweep4(BatchTable<float> tab, Roles r) {
+    __shared__ __align__(16) float lds_f[3 * PS4_RING * PS4_THREADS];
+    const int t = xcd_tile((int)blockIdx.x, tab.total_ps4);
+    int lo = 0, hi = tab.ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab.first_ps4[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    PS2Args<float> a = ps2_args(tab.params[lo], tab.fields[lo], r);
+    a.zchunk = tab.params[lo].zchunk4;
+    pres_sweep4_block<ZIN>(a, t - tab.first_ps4[lo], lds_f);
+}
+
 // two-sweep pressure kernel over every case of an ensemble: workgroup t
 // belongs to the case c with first_ps2[c] <= t < first_ps2[c + 1]
 template <class T, bool ZIN>
@@ -2558,6 +2574,14 @@ void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, 
             hipLaunchKernelGGL((k_batch_pres_sweep3<T, true>), dim3(tab.total_ps3), dim3(PS3_THREADS), 0, s, tab, r);
         else
             hipLaunchKernelGGL((k_batch_pres_sweep3<T, false>), dim3(tab.total_ps3), dim3(PS3_THREADS), 0, s, tab, r);
+        return;
+    }
+    if (k == KN_PRES_SWEEP4) {
+        static_assert(std::is_same_v<T, float>, "ensembles run FP32");
+        if (r.pin < 0)
+            hipLaunchKernelGGL(k_batch_pres_sweep4<true>, dim3(tab.total_ps4), dim3(PS4_THREADS), 0, s, tab, r);
+        else
+            hipLaunchKernelGGL(k_batch_pres_sweep4<false>, dim3(tab.total_ps4), dim3(PS4_THREADS), 0, s, tab, r);
         return;
     }
     if (k == KN_VEL_SWEEP2) {
