@@ -87,9 +87,20 @@ def test_stage_parity(gpu, name, w):
                 assert e <= TOL, (name, stage, fid, e)
 
 
-@pytest.mark.parametrize("wname", ["tiny", "isolated_building"])
+def tiny_or_named(name, **kw):
+    """``tiny:XxYxZ`` is the tiny case on that grid (its two buildings
+    clipped to it); other names are the named workloads."""
+    if name.startswith("tiny"):
+        grid = tuple(int(v) for v in name.split(":")[1].split("x")) if ":" in name else (16, 8, 12)
+        return workloads.tiny(grid=grid, **kw)
+    return workloads.WORKLOADS[name]()
+
+
+# tiny:100x21x37: two x tiles of the tiled kernels, the second 36 columns
+# wide, and Y and Z that no tile height or z chunk divides
+@pytest.mark.parametrize("wname", ["tiny", "isolated_building", "tiny:100x21x37"])
 def test_one_step_parity(gpu, wname):
-    w = workloads.WORKLOADS[wname]()
+    w = tiny_or_named(wname)
     o = oracle_for(w)
     with solver_for(w) as s:
         s.Step()
@@ -159,9 +170,13 @@ def test_deterministic(gpu):
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+# tiny:100x21x37 as above; tiny:67x5x7: a 64-column tile and a 3-column
+# one, fewer rows than the four-sweep kernel's two halo bands
 @pytest.mark.parametrize("name,vel_iters,pres_iters,norms", [("tiny", 6, 8, False), ("tiny", 7, 7, False),
                                                               ("tiny", 6, 8, True), ("tiny", 5, 8, True),
-                                                              ("tiny", 2, 2, False), ("isolated_building", 4, 5, False)])
+                                                              ("tiny", 2, 2, False), ("isolated_building", 4, 5, False),
+                                                              ("tiny:100x21x37", 6, 9, False),
+                                                              ("tiny:67x5x7", 5, 8, True)])
 def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters, pres_iters, norms):
     """The multi-sweep kernels (temporal blocking over z-marching tiles: two
     momentum sweeps, two, three or four pressure sweeps per launch) give
@@ -173,8 +188,8 @@ def test_two_sweep_kernel_equals_single_sweeps(gpu, monkeypatch, name, vel_iters
     reach below the grid); pairs in tiles of 15 and 5 rows (the grid's own
     choice is 14 or 15); four-sweep launches with the grid's z chunk and
     with 5-, 2- and 1-plane ones."""
-    if name == "tiny":
-        w = workloads.tiny(vel_iters=vel_iters, pres_iters=pres_iters)
+    if name.startswith("tiny"):
+        w = tiny_or_named(name, vel_iters=vel_iters, pres_iters=pres_iters)
     else:
         w = workloads.isolated_building()
         w.cf.presMaxNumIter = pres_iters
