@@ -1,7 +1,8 @@
-"""The byte accounting the bench line reports, against the kernel table it is
-built from: SURVEY's per-step algorithmic bytes (every sweep a pass) and the
-compulsory bytes of the launches this build runs (each array once per
-launch).  CPU only: the functions are plain arithmetic."""
+"""The byte accounting the bench line reports, against the per-kernel byte
+table it is built from: the per-step algorithmic bytes of the reference's
+dispatch sequence (every sweep a pass over its arrays) and the compulsory
+bytes of the launches the solver runs (each array once per launch).  CPU
+only: the functions are plain arithmetic."""
 from __future__ import annotations
 
 import importlib.util
@@ -22,7 +23,7 @@ def bench():
     return mod
 
 
-def test_step_bytes_per_survey_table(bench):
+def test_step_bytes_per_kernel_table(bench):
     # wall function 25, eddy viscosity 17, LES term 41, momentum precompute 69, Rhie-Chow precompute 37,
     # p += p' 13, velocity correction 53, zero p' / copy u 8: 263 fixed; 77 per momentum sweep, 29 per
     # pressure sweep, per PISO corrector 37 + 13 + 65 + 8 and 33 per corrector sweep
