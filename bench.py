@@ -36,7 +36,7 @@ def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
 
 def compulsory_bytes_per_cell_step(V: int, P: int, C: int, les: bool = True, pres_depth: int = 3,
                                    vel_depth: int = 2) -> int:
-    """Bytes per cell one step of this build must move, each array once per
+    """Bytes per cell one step of the solver's launch plan must move, each array once per
     launch: wall function + eddy viscosity with the gradient store (78), the
     momentum precompute with the LES term and the gradient read (146), V / 2
     momentum pairs (73 each, a leftover single sweep 77; vel_depth 1: V single
@@ -108,8 +108,8 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
     return {"value": round(X * Y * Z * n / dt / 1e6, 4), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"{n} PISO steps of {w.name} (V=P=50, C=2), FP32 oracle on {threads} OpenMP threads "
                       f"({os.cpu_count()} logical CPUs visible)",
-            "stand_in_for": "the C# Parallel.For transcription of the reference kernels, which this build does not "
-                            "have (the image it is built in has no .NET toolchain): the C++ OpenMP oracle, the same "
+            "stand_in_for": "the C# Parallel.For transcription of the reference kernels, which cannot be built "
+                            "without a .NET toolchain (none is installed): the C++ OpenMP oracle, the same "
                             "kernels, order and FP32 arithmetic, runs instead",
             "dotnet_toolchain_on_host": dotnet,
             "ms_per_step": round(dt / n * 1e3, 2), "algorithmic_GBps": round(b_step * X * Y * Z * n / dt / 1e9, 2),
@@ -497,7 +497,7 @@ def main() -> int:
                     help="also run a plain 256 MiB device copy (reference point for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the solver step runs on a single MI355X and this build has no "
+        print(f"bench.py measures one GPU: the solver step runs on a single MI355X and the library has no "
               f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         return 2
 
