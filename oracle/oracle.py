@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (``oracle/liboracle_uws.so``).
 
 Test infrastructure: only the tests, the smoke check and the bench's
-CPU-baseline leg use it, always as the checker.
+CPU-baseline leg use it, always as the reference the HIP results are
+compared with (or, in the CPU-baseline leg, timed beside).
 """
 from __future__ import annotations
 

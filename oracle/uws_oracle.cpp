@@ -1,8 +1,9 @@
 /*
  * uws_oracle.cpp -- CPU restatement of UnityWindSimulation's collocated
- * finite-volume solver step.  TEST INFRASTRUCTURE ONLY: it is the checker the
- * parity tests, the smoke check and the bench's CPU-baseline leg compare the
- * HIP library against.  Nothing in the product library links or calls it.
+ * finite-volume solver step.  TEST INFRASTRUCTURE ONLY: the parity tests and
+ * the smoke check compare the HIP library against it, and the bench's
+ * CPU-baseline leg times it.  Nothing in the product library links or calls
+ * it.
  *
  * Every kernel is restated from the reference shader text in the same
  * arithmetic order (left-to-right float expressions, one accumulator per
