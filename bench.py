@@ -605,7 +605,11 @@ def main() -> int:
                                 "counted as one pass, per the handle's launch plan); the stream ceiling is the "
                                 "plain streaming kernel on a working set of the momentum pair's size"},
         "stream_ceiling_GBps": ceil,
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kname,
+                     "selected_by": "the Jacobi sweep kernel with the larger share of the step (step_ms_share; "
+                                    "where the two shares are close either can be named); frac_by_kernel gives "
+                                    "every sweep kernel's fraction",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "frac_of_stream_ceiling": round(achieved / kceil, 4),
                      "algorithmic_bytes": kbytes, "single_sweeps_bytes": kbytes_sweeps,
