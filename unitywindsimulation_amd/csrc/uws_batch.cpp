@@ -29,6 +29,7 @@ struct BatchGroup {
     bool quads = false;  // four-sweep pressure launches
     ParamsT<float>* d_params = nullptr;
     FieldsT<float>* d_fields = nullptr;
+    std::vector<FieldsT<float>> h_fields;  // d_fields' contents, for launches that take them as arguments
     void release() {
         if (d_first) (void)hipFree(d_first);
         if (d_first2) (void)hipFree(d_first2);
@@ -242,6 +243,7 @@ struct uws_batch {
             g.quads = use_fusion4(fuse_mode, min_cells > INT32_MAX ? INT32_MAX : (int)min_cells);
             g.tab.params = g.d_params;
             g.tab.fields = g.d_fields;
+            g.h_fields = fs;
             // the momentum pair tiles the grid as the two-sweep pressure kernel
             // does (first_ps2), with the group's z chunk
             int zmin = ps[0].zchunk;
@@ -265,7 +267,7 @@ struct uws_batch {
         uws_batch* b;
         BatchGroup* g;
         void operator()(Kernel k, const Roles& r, int slot = 0) {
-            launch_batch<float>(k, g->tab, g->grid, r, b->stream, slot);
+            launch_batch<float>(k, g->tab, g->grid, r, b->stream, slot, g->h_fields.data());
         }
         int copy_vel(int dst, int src) {
             for (int c : g->cases) {

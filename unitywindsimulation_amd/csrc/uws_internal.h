@@ -145,9 +145,24 @@ enum Kernel {
 template <class T>
 void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s, int slot = 0);
 
-// Ensemble launch: one grid over every case's tiles.
+// Up to this many cases' field tables travel in the kernel arguments of an
+// ensemble's cell-wise launches (4 x 728 B, within the 4 KB argument limit):
+// array pointers read from the kernel arguments are addressed as global
+// memory, pointers read from a table in device memory as generic (flat)
+// memory, whose loads also wait on the LDS counter.
+constexpr int BATCH_ARG_CASES = 4;
 template <class T>
-void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot = 0);
+struct BatchArgFields {
+    FieldsT<T> f[BATCH_ARG_CASES];
+};
+
+// Ensemble launch: one grid over every case's tiles.  host_fields: the
+// group's field tables on the host (tab.fields' contents), or null; with
+// at most BATCH_ARG_CASES cases the cell-wise kernels take them as kernel
+// arguments.
+template <class T>
+void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot = 0,
+                  const FieldsT<T>* host_fields = nullptr);
 
 // A background flow field resident on the device, for the warm start of
 // InitVelPresFieldsFromBackGroundFlow (geometry in R, values in T).  The

@@ -2322,24 +2322,35 @@ __global__ __launch_bounds__(256) void k_single(ParamsT<T> p, FieldsT<T> f, Role
     const Cell c = cell_of<SINGLE_TY>(p, xcd_tile((int)blockIdx.x, p.tiles));
     run_body<T, K>(p, f, c, r);
 }
+// `fields`: the cases' field tables, tab.fields (device memory) or the
+// kernel-argument copy of BatchArgFields
 template <class T, int K>
-__device__ __forceinline__ void batch_tile(const BatchTable<T>& tab, const Roles& r, int t) {
+__device__ __forceinline__ void batch_tile(const BatchTable<T>& tab, const FieldsT<T>* fields, const Roles& r,
+                                           int t) {
     const int cid = case_of(tab, t);
     const ParamsT<T>& p = tab.params[cid];
-    const FieldsT<T>& f = tab.fields[cid];
+    const FieldsT<T>& f = fields[cid];
     const Cell c = cell_of(p, t - tab.first_tile[cid]);
     run_body<T, K>(p, f, c, r);
 }
 template <class T, int K>
-__global__ __launch_bounds__(256) void k_batch(BatchTable<T> tab, Roles r) {
+__device__ __forceinline__ void batch_tiles(const BatchTable<T>& tab, const FieldsT<T>* fields, const Roles& r) {
     if (!tab.persistent) {
-        batch_tile<T, K>(tab, r, xcd_tile((int)blockIdx.x, tab.total_tiles));
+        batch_tile<T, K>(tab, fields, r, xcd_tile((int)blockIdx.x, tab.total_tiles));
         return;
     }
     int lo, hi;
     xcd_range(tab.total_tiles, (int)blockIdx.x & 7, &lo, &hi);
     const int per = (int)gridDim.x >> 3;
-    for (int t = lo + ((int)blockIdx.x >> 3); t < hi; t += per) batch_tile<T, K>(tab, r, t);
+    for (int t = lo + ((int)blockIdx.x >> 3); t < hi; t += per) batch_tile<T, K>(tab, fields, r, t);
+}
+template <class T, int K>
+__global__ __launch_bounds__(256) void k_batch(BatchTable<T> tab, Roles r) {
+    batch_tiles<T, K>(tab, tab.fields, r);
+}
+template <class T, int K>
+__global__ __launch_bounds__(256) void k_batch_arg(BatchTable<T> tab, Roles r, BatchArgFields<T> af) {
+    batch_tiles<T, K>(tab, af.f, r);
 }
 template <class T, int K>
 __global__ __launch_bounds__(256) void k_single_reduce(ParamsT<T> p, FieldsT<T> f, Roles r) {
@@ -2348,24 +2359,34 @@ __global__ __launch_bounds__(256) void k_single_reduce(ParamsT<T> p, FieldsT<T> 
     if (threadIdx.x == 0 && threadIdx.y == 0) f.partial[c.tile] = s;
 }
 template <class T, int K>
-__device__ __forceinline__ void batch_reduce_tile(const BatchTable<T>& tab, const Roles& r, int t) {
+__device__ __forceinline__ void batch_reduce_tile(const BatchTable<T>& tab, const FieldsT<T>* fields, const Roles& r,
+                                                  int t) {
     const int cid = case_of(tab, t);
     const ParamsT<T>& p = tab.params[cid];
-    const FieldsT<T>& f = tab.fields[cid];
+    const FieldsT<T>& f = fields[cid];
     const Cell c = cell_of(p, t - tab.first_tile[cid]);
     const double s = block_sum(run_reduce<T, K>(p, f, c, r));
     if (threadIdx.x == 0 && threadIdx.y == 0) f.partial[c.tile] = s;
 }
 template <class T, int K>
-__global__ __launch_bounds__(256) void k_batch_reduce(BatchTable<T> tab, Roles r) {
+__device__ __forceinline__ void batch_reduce_tiles(const BatchTable<T>& tab, const FieldsT<T>* fields,
+                                                   const Roles& r) {
     if (!tab.persistent) {
-        batch_reduce_tile<T, K>(tab, r, xcd_tile((int)blockIdx.x, tab.total_tiles));
+        batch_reduce_tile<T, K>(tab, fields, r, xcd_tile((int)blockIdx.x, tab.total_tiles));
         return;
     }
     int lo, hi;
     xcd_range(tab.total_tiles, (int)blockIdx.x & 7, &lo, &hi);
     const int per = (int)gridDim.x >> 3;
-    for (int t = lo + ((int)blockIdx.x >> 3); t < hi; t += per) batch_reduce_tile<T, K>(tab, r, t);
+    for (int t = lo + ((int)blockIdx.x >> 3); t < hi; t += per) batch_reduce_tile<T, K>(tab, fields, r, t);
+}
+template <class T, int K>
+__global__ __launch_bounds__(256) void k_batch_reduce(BatchTable<T> tab, Roles r) {
+    batch_reduce_tiles<T, K>(tab, tab.fields, r);
+}
+template <class T, int K>
+__global__ __launch_bounds__(256) void k_batch_reduce_arg(BatchTable<T> tab, Roles r, BatchArgFields<T> af) {
+    batch_reduce_tiles<T, K>(tab, af.f, r);
 }
 
 // sum one case's partials into norms[slot]; with `scale` also store the
@@ -2402,13 +2423,30 @@ template <class T, int K>
 void go_single_reduce(const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, hipStream_t s) {
     hipLaunchKernelGGL((k_single_reduce<T, K>), dim3(p.tiles), dim3(TX, SINGLE_TY, 1), 0, s, p, f, r);
 }
-template <class T, int K>
-void go_batch(const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s) {
-    hipLaunchKernelGGL((k_batch<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r);
+// with the group's field tables on the host (hf) and at most BATCH_ARG_CASES
+// cases, the tables go in the kernel arguments
+template <class T>
+bool batch_arg_fields(const BatchTable<T>& tab, const FieldsT<T>* hf, BatchArgFields<T>& af) {
+    if (!hf || tab.ncases > BATCH_ARG_CASES) return false;
+    af = BatchArgFields<T>{};
+    for (int i = 0; i < tab.ncases; ++i) af.f[i] = hf[i];
+    return true;
 }
 template <class T, int K>
-void go_batch_reduce(const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s) {
-    hipLaunchKernelGGL((k_batch_reduce<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r);
+void go_batch(const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, const FieldsT<T>* hf) {
+    BatchArgFields<T> af;
+    if (batch_arg_fields(tab, hf, af))
+        hipLaunchKernelGGL((k_batch_arg<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r, af);
+    else
+        hipLaunchKernelGGL((k_batch<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r);
+}
+template <class T, int K>
+void go_batch_reduce(const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, const FieldsT<T>* hf) {
+    BatchArgFields<T> af;
+    if (batch_arg_fields(tab, hf, af))
+        hipLaunchKernelGGL((k_batch_reduce_arg<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r, af);
+    else
+        hipLaunchKernelGGL((k_batch_reduce<T, K>), dim3(grid), dim3(TX, tab.ty, 1), 0, s, tab, r);
 }
 
 #include "uws_adjoint_op1.h"
@@ -2568,7 +2606,8 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
 }
 
 template <class T>
-void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot) {
+void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, hipStream_t s, int slot,
+                  const FieldsT<T>* host_fields) {
     if (k == KN_PRES_SWEEP3) {
         if (r.pin < 0)
             hipLaunchKernelGGL((k_batch_pres_sweep3<T, true>), dim3(tab.total_ps3), dim3(PS3_THREADS), 0, s, tab, r);
@@ -2602,8 +2641,8 @@ void launch_batch(Kernel k, const BatchTable<T>& tab, int grid, const Roles& r, 
                            k == KN_DEFECT_SCALE ? 1 : 0);
         return;
     }
-#define GOB(K) go_batch<T, K>(tab, grid, r, s)
-#define GORB(K) go_batch_reduce<T, K>(tab, grid, r, s)
+#define GOB(K) go_batch<T, K>(tab, grid, r, s, host_fields)
+#define GORB(K) go_batch_reduce<T, K>(tab, grid, r, s, host_fields)
     UWS_KERNEL_CASES(GOB, GORB)
 #undef GOB
 #undef GORB
@@ -2838,6 +2877,7 @@ template void launch<Dual<float>>(Kernel, const ParamsT<Dual<float>>&, const Fie
                                   hipStream_t, int);
 template void launch<Dual<double>>(Kernel, const ParamsT<Dual<double>>&, const FieldsT<Dual<double>>&,
                                    const Roles&, hipStream_t, int);
-template void launch_batch<float>(Kernel, const BatchTable<float>&, int, const Roles&, hipStream_t, int);
+template void launch_batch<float>(Kernel, const BatchTable<float>&, int, const Roles&, hipStream_t, int,
+                                   const FieldsT<float>*);
 
 }  // namespace uws
