@@ -62,6 +62,32 @@ def test_ensemble_matches_single_cases_bitwise(gpu, monkeypatch, ty, persistent,
         np.testing.assert_array_equal(pg, pr)
 
 
+@pytest.mark.parametrize("ty,persistent,group_bytes,pack", [(4, False, 0, None), (2, True, 0, None),
+                                                            (4, False, -1, 3)])
+def test_six_case_groups_match_single_cases_bitwise(gpu, monkeypatch, ty, persistent, group_bytes, pack):
+    """Six cases: one group of six reads the cases' field tables from device
+    memory; packed groups of at most four cases pass them as kernel
+    arguments.  Both equal the single cases bit for bit."""
+    if pack is not None:
+        monkeypatch.setenv("UWS_PACK_BLOCKS", str(pack))
+    ws = small_cases() + [workloads.tiny(grid=(24, 9, 14), seed=9), workloads.tiny(grid=(65, 8, 10), seed=4),
+                          workloads.tiny(grid=(16, 8, 12), seed=12)]
+    ref = singles(ws, 3)
+    with FvmEnsembleHip([w.cf for w in ws]) as e:
+        for c, w in enumerate(ws):
+            e.init_flags(c, w.flags)
+            e.set_velocity(c, noise_velocity(w.flags, w.noise_seed, w.noise_amp))
+        e.set_launch(ty, persistent, group_bytes)
+        groups = e.num_groups()
+        e.step(3)
+        got = [(e.velocity(c, 3), e.pressure(c)) for c in range(len(ws))]
+    print(f"  six cases, launch {ty}/{persistent}/{group_bytes}/{pack}: {groups} groups", flush=True)
+    assert groups == 1 if group_bytes == 0 else groups >= 2
+    for (ug, pg), (ur, pr) in zip(got, ref):
+        np.testing.assert_array_equal(ug, ur)
+        np.testing.assert_array_equal(pg, pr)
+
+
 def test_autotune_leaves_state_unchanged(gpu):
     ws = small_cases()
     ref = singles(ws, 3)
