@@ -17,11 +17,18 @@ CSRC = ROOT / "unitywindsimulation_amd" / "csrc"
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 
+# the host program as built, and under AddressSanitizer and UBSan (host code
+# only: no GPU code is built)
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-Xarch_host", "-fno-sanitize-recover=all"]
+
+
 @pytest.mark.skipif(not Path(HIPCC).exists(), reason="hipcc is needed to build the host program")
-def test_step_driver_launch_sequence():
-    subprocess.run([HIPCC, "-std=c++17", "-O1", f"-I{CSRC}", str(SRC), "-o", str(EXE)], check=True,
+@pytest.mark.parametrize("flags", [[], SANITIZE], ids=["plain", "asan_ubsan"])
+def test_step_driver_launch_sequence(flags):
+    exe = EXE.with_name(EXE.name + ("_asan" if flags else ""))
+    subprocess.run([HIPCC, "-std=c++17", "-O1", f"-I{CSRC}", *flags, str(SRC), "-o", str(exe)], check=True,
                    capture_output=True, text=True, timeout=600)
-    res = subprocess.run([str(EXE)], capture_output=True, text=True, timeout=120)
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stdout
     assert "cases passed" in res.stdout
