@@ -97,8 +97,10 @@ def tiny_or_named(name, **kw):
 
 
 # tiny:100x21x37: two x tiles of the tiled kernels, the second 36 columns
-# wide, and Y and Z that no tile height or z chunk divides
-@pytest.mark.parametrize("wname", ["tiny", "isolated_building", "tiny:100x21x37"])
+# wide, and Y and Z that no tile height or z chunk divides; 2x2x2, 5x1x3 and
+# 1x4x6: one or two cells in a direction (every neighbour index clamped)
+@pytest.mark.parametrize("wname", ["tiny", "isolated_building", "tiny:100x21x37", "tiny:2x2x2", "tiny:5x1x3",
+                                   "tiny:1x4x6"])
 def test_one_step_parity(gpu, wname):
     w = tiny_or_named(wname)
     o = oracle_for(w)
