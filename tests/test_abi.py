@@ -76,6 +76,13 @@ def test_invalid_bnd_and_grid_rejected():
     c = workloads.tiny().cf.to_c()
     c.reserved[0] = 1
     assert N.lib().uws_create(ctypes.byref(c), ctypes.byref(h)) == -2
+    # the cell count must stay below 2^29 (32-bit byte offsets of 8-byte
+    # values): exactly 2^29 cells, and extents whose product overflows 32 bits
+    for grid in ((1024, 1024, 512), (1 << 20, 1 << 20, 1 << 20), (65536, 65536, 1)):
+        c = workloads.tiny().cf.to_c()
+        c.grid_res[0], c.grid_res[1], c.grid_res[2] = grid
+        assert N.lib().uws_create(ctypes.byref(c), ctypes.byref(h)) == -2, grid
+        assert not h.value
 
 
 def test_null_handle_calls_fail_cleanly():
