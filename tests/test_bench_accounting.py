@@ -53,3 +53,14 @@ def test_compulsory_bytes_of_the_launch_plans(bench):
     # an odd momentum count: the leftover sweep is a single launch (77)
     assert (bench.compulsory_bytes_per_cell_step(51, 50, 2, pres_depth=4)
             - bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=4)) == 77
+
+
+def test_more_than_one_gpu_is_refused():
+    # the solver step runs on one GPU; asking for more exits with status 2
+    # and a message, before any device work
+    import subprocess
+
+    res = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                         timeout=300)
+    assert res.returncode == 2
+    assert "one GPU" in res.stderr and res.stdout == ""
