@@ -53,6 +53,38 @@ def compulsory_bytes_per_cell_step(V: int, P: int, C: int, les: bool = True, pre
     return one_shot + mom + 37 + solve + 13 + 53 + C * (37 + 4 + solve + 78)
 
 
+# HBM bytes per cell each kernel moved per launch on the urban case
+# (256 x 96 x 128), read from the FETCH_SIZE (scaled by 2.000) and WRITE_SIZE
+# counters (profiles/urban_256x96x128_hbm_traffic.md); "_zin": the first
+# launch of a pressure solve, which starts from p' = 0
+MEASURED_LAUNCH_B_PER_CELL = {"urban": {
+    "k_wall_eddy": 66.9, "k_vel_precompute": 165.2, "k_pres_precompute": 59.3, "k_apply_pres": 13.0,
+    "k_vel_correct": 53.5, "k_piso_aodu": 38.9, "k_piso_precompute": 46.2, "k_piso_correct": 73.5,
+    "k_vel_sweep": 75.9, "k_vel_sweep2": 84.6, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.2,
+    "k_pres_sweep3": 35.2, "k_pres_sweep4": 36.2, "k_pres_sweep4_zin": 30.5}}
+
+
+def measured_bytes_per_cell_step(workload: str, V: int, P: int, C: int, les: bool = True, pres_depth: int = 4,
+                                 vel_depth: int = 2):
+    """Counter-measured HBM bytes per cell of one step: the per-launch
+    figures above summed over the launch plan (the launches
+    compulsory_bytes_per_cell_step counts), or None where no profile of the
+    workload holds them."""
+    m = MEASURED_LAUNCH_B_PER_CELL.get(workload)
+    if m is None or not les:
+        return None
+    names = {1: "k_pres_sweep", 2: "k_pres_sweep2", 3: "k_pres_sweep3", 4: "k_pres_sweep4"}
+    d = max(1, pres_depth)
+    n_main, rem = (P // d, P % d) if d > 1 else (P, 0)
+    solve = n_main * m[names[d]] + (0.0 if rem == 0 else m[names[rem]])
+    if n_main > 0 and names[d] + "_zin" in m:
+        solve += m[names[d] + "_zin"] - m[names[d]]
+    mom = (V // 2) * m["k_vel_sweep2"] + (V % 2) * m["k_vel_sweep"] if vel_depth == 2 else V * m["k_vel_sweep"]
+    total = (m["k_wall_eddy"] + m["k_vel_precompute"] + mom + m["k_pres_precompute"] + solve + m["k_apply_pres"]
+             + m["k_vel_correct"] + C * (m["k_piso_aodu"] + m["k_piso_precompute"] + solve + m["k_piso_correct"]))
+    return round(total, 1)
+
+
 def stream_ceiling(cells: int, nin: int, reps: int = 20) -> float:
     """GB/s of the library's plain streaming kernel over nin + 1 float arrays
     of `cells` values (uws_stream_bandwidth), launched back to back on the
@@ -570,6 +602,7 @@ def main() -> int:
     kceil = ceil["momentum_like_68B_per_cell"] if kname.startswith("k_vel") else ceil["pressure_like_28B_per_cell"]
     b_comp = compulsory_bytes_per_cell_step(V, P, C, les, pres_depth, vel_depth)
     comp_gbps = b_comp * ncell / (ms_step * 1e-3) / 1e9
+    b_meas = measured_bytes_per_cell_step(args.workload, V, P, C, les, pres_depth, vel_depth)
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -604,6 +637,11 @@ def main() -> int:
                        "basis": "each array once per launch of the kernels the step runs (multi-sweep launches "
                                 "counted as one pass, per the handle's launch plan); the stream ceiling is the "
                                 "plain streaming kernel on a working set of the momentum pair's size"},
+        "step_bytes_measured": None if b_meas is None else {
+            "B_per_cell": b_meas, "GBps": round(b_meas * ncell / (ms_step * 1e-3) / 1e9, 1),
+            "frac_of_peak": round(b_meas * ncell / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "basis": "per-launch HBM bytes from the FETCH_SIZE and WRITE_SIZE counters of a profiled urban run, "
+                     "summed over the step's launch plan"},
         "stream_ceiling_GBps": ceil,
         "roofline": {"bound": "hbm", "kernel": kname,
                      "selected_by": "the Jacobi sweep kernel with the larger share of the step (step_ms_share; "

@@ -64,3 +64,14 @@ def test_more_than_one_gpu_is_refused():
                          timeout=300)
     assert res.returncode == 2
     assert "one GPU" in res.stderr and res.stdout == ""
+
+
+def test_counter_measured_step_bytes(bench):
+    # urban plan (V = P = 50, C = 2, four-sweep pressure launches, momentum pairs), summed by hand from the
+    # per-launch counter table: 66.9 + 165.2 (wall function, momentum precompute) + 25 x 84.6 (pairs)
+    # + 59.3 + 13.0 + 53.5 (Rhie-Chow precompute, p += p', velocity correction) + 3 solves of
+    # (30.5 + 11 x 36.2 + 32.2) + 2 x (38.9 + 46.2 + 73.5) = 4172.8
+    assert bench.measured_bytes_per_cell_step("urban", 50, 50, 2, True, 4, 2) == 4172.8
+    # no profile of other workloads or of the laminar plan
+    assert bench.measured_bytes_per_cell_step("isolated_building", 50, 50, 2) is None
+    assert bench.measured_bytes_per_cell_step("urban", 50, 50, 2, les=False) is None
