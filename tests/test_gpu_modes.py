@@ -12,8 +12,11 @@ from .helpers import oracle_for, rel_l2, solver_for
 pytestmark = pytest.mark.gpu
 
 
-def test_fp64_gpu_run_matches_fp64_oracle(gpu):
-    w = workloads.tiny()
+# (100, 21, 37): two x tiles, the second partial, Y and Z divided by no tile
+# height or z chunk
+@pytest.mark.parametrize("grid", [(16, 8, 12), (100, 21, 37)])
+def test_fp64_gpu_run_matches_fp64_oracle(gpu, grid):
+    w = workloads.tiny(grid=grid)
     w.cf.precision = N.PREC_FP64
     o = oracle_for(w, precision=1)
     with solver_for(w) as s:
@@ -74,13 +77,13 @@ def test_low_precision_defect_correction(gpu, prec, tol):
 
 
 @pytest.mark.parametrize("prec", [N.PREC_FP32_FP16, N.PREC_FP32_BF16])
-@pytest.mark.parametrize("sweeps", [20, 7])
-def test_low_precision_triples_equal_single_sweeps(gpu, monkeypatch, prec, sweeps):
+@pytest.mark.parametrize("sweeps,grid", [(20, (20, 14, 17)), (7, (20, 14, 17)), (7, (100, 21, 37))])
+def test_low_precision_triples_equal_single_sweeps(gpu, monkeypatch, prec, sweeps, grid):
     """The three-sweep low-precision kernel (its first two sweeps rounded to 2
     bytes as the stored iterate would be, the last of a round writing p')
     gives the fields and defect norms of single low-precision sweeps bit for
     bit, with the grid's z chunk and with 5- and 1-plane chunks."""
-    w = workloads.tiny(grid=(20, 14, 17))
+    w = workloads.tiny(grid=grid)
     w.cf.precision, w.cf.refineRounds, w.cf.refineSweeps, w.cf.computeNorms = prec, 3, sweeps, True
     runs = [{"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"},
             {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "1"}]
