@@ -31,6 +31,8 @@ VARIANTS = {
     "simple_les": (lambda: workloads.tiny(piso=False), 3, 3),
     "piso_laminar": (lambda: workloads.tiny(les=False), 3, 2),
     "no_momentum_sweeps": (lambda: workloads.tiny(vel_iters=0, pres_iters=3), 2, 1),
+    # two x tiles, the second 6 columns wide; Y and Z divided by no tile height or z chunk
+    "piso_les_ragged": (lambda: workloads.tiny(grid=(70, 9, 11)), 2, 1),
 }
 
 
@@ -212,6 +214,7 @@ def _variant(w, slots=None, model=None, **bnd):
 
 REVERSE_VARIANTS = {
     "piso_les_slots3_rough": lambda: workloads.tiny(),
+    "piso_les_ragged": lambda: workloads.tiny(grid=(70, 9, 11)),
     "simple_les_slots1_smooth": lambda: _variant(workloads.tiny(piso=False), 1, 0),
     "piso_les_slots1_rough": lambda: _variant(workloads.tiny(), 1, 1),
     "piso_laminar_slots3_smooth": lambda: _variant(workloads.tiny(les=False), 3, 0),
