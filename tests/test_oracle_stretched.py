@@ -369,3 +369,72 @@ def test_log_profile_at_face_midpoints():
     np.testing.assert_allclose(u[..., 0], np.broadcast_to(prof[None, :, None], (Z, Y, X)), rtol=1e-6)
     b = o.bnd_field(0).reshape(Z, Y, 3)
     np.testing.assert_allclose(b[..., 0], np.broadcast_to(prof[None, :], (Z, Y)), rtol=1e-6)
+
+
+def _mirror(a, grid, axis):
+    """A cell field [z][y][x](...) flattened, reversed along one grid axis."""
+    X, Y, Z = grid
+    b = a.reshape(Z, Y, X, *a.shape[1:])
+    return np.flip(b, axis=2 - axis).reshape(a.shape).copy()
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_pressure_precompute_reflection_symmetry(axis):
+    """Mirroring one axis (its uneven cell sizes reversed, and u, p, D and a
+    solid block with them, the velocity component along the axis negated)
+    mirrors Dp and bp, and mirrors each stored face velocity's Rhie-Chow
+    term up to the size of the cell that computes it (the reference scales
+    the term by that cell's volume dv).  So the + and - side branches of the
+    pressure-gradient averages, central and one-sided next to the domain
+    boundary and next to solids, are each other's mirror images: a spacing
+    or neighbour slipped in one side's branch fails it.  The sizes along the
+    mirrored axis are multiples of 1/8, so both face sets are exact."""
+    grid = (10, 8, 9)
+    n_a = grid[axis]
+    sizes = (np.arange(n_a) * 5 % 11 + 4) / 8.0  # 0.5 .. 1.75, uneven
+    faces = faces_for(grid, seed=41)
+    mfaces = list(faces)
+    faces[axis] = np.concatenate([[0.0], np.cumsum(sizes)]).astype(np.float32)
+    mfaces[axis] = np.concatenate([[0.0], np.cumsum(sizes[::-1])]).astype(np.float32)
+    flags = rasterise(grid, [Box(3, 5, 2, 4, 3)])
+    rng = np.random.default_rng(7 + axis)
+    n = int(np.prod(grid))
+    fluid = (flags == 0)
+    u = rng.standard_normal((n, 3)) * fluid[:, None]
+    p = rng.standard_normal(n)
+    D = rng.uniform(0.5, 2.0, (n, 3))
+    mu = _mirror(u, grid, axis)
+    mu[:, axis] *= -1
+
+    def run(fc, fl, uu, pp, DD):
+        cf = stretched_config(grid)
+        set_bc(cf, [1] * 6, [1] * 6)
+        o = make(cf, fc, flags=fl)
+        o.set_field(N.FIELD_VEL, uu)
+        o.set_pressure(pp)
+        o.set_field(N.FIELD_D, DD)
+        o.run_stage(N.STAGE_PRES_PRECOMP)
+        return o
+
+    a = run(faces, flags, u, p, D)
+    b = run(mfaces, _mirror(flags, grid, axis), mu, _mirror(p, grid, axis), _mirror(D, grid, axis))
+    np.testing.assert_allclose(_mirror(b.field(N.FIELD_DP), grid, axis), a.field(N.FIELD_DP), rtol=1e-13, atol=0)
+    np.testing.assert_allclose(_mirror(b.field(N.FIELD_BP), grid, axis), a.field(N.FIELD_BP), rtol=1e-12,
+                               atol=1e-12)
+    # the stored face velocities with u = 0: only the pressure terms, each scaled by its cell's size
+    zero = np.zeros((n, 3))
+    fa = run(faces, flags, zero, p, D).face_flux()[..., axis]
+    fb = run(mfaces, _mirror(flags, grid, axis), zero, _mirror(p, grid, axis), _mirror(D, grid, axis)).face_flux()[
+        ..., axis]
+    ax = 2 - axis  # the face array is [Z+1][Y+1][X+1]
+    fa, fb = np.moveaxis(fa, ax, 0), np.moveaxis(fb, ax, 0)
+    X, Y, Z = grid
+    other = [slice(0, m) for m in (Z, Y, X)]
+    del other[ax]
+    checked = 0
+    for i in range(1, n_a):
+        ta = fa[i][tuple(other)] / sizes[i]
+        tb = -fb[n_a - i][tuple(other)] / sizes[::-1][n_a - i]
+        np.testing.assert_allclose(tb, ta, rtol=1e-12, atol=1e-12 * max(1.0, float(np.max(np.abs(ta)))))
+        checked += int(np.count_nonzero(ta))
+    assert checked > 0
