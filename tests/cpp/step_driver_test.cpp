@@ -2,7 +2,8 @@
 // run against a recording launcher on the host: every buffer carries the
 // number of Jacobi sweeps its contents have seen, and each launch checks and
 // updates those counts.  Checked for every combination of multi-sweep
-// kernels, sweep counts, norms, PISO correctors and low-precision rounds:
+// kernels (pressure pairs, triples and four-sweep launches, momentum pairs),
+// sweep counts, norms, PISO correctors and low-precision rounds:
 //  - no launch reads and writes the same buffer, and u^n is never written
 //    by the momentum sweeps;
 //  - the momentum result holds V sweeps of u^n, every pressure solve P
@@ -37,7 +38,7 @@ std::string g_case;
 constexpr int GARBAGE = -1000;
 
 struct Rec {
-    bool f2 = false, fv = false, f3 = false;  // pressure pairs, momentum pairs, triples
+    bool f2 = false, fv = false, f3 = false, f4 = false;  // pressure pairs, momentum pairs, triples, fours
     int V = 0, P = 0, K = 0;
     int state = 0;
     int vel[3] = {GARBAGE, GARBAGE, GARBAGE};  // sweeps seen by each velocity buffer
@@ -71,10 +72,12 @@ struct Rec {
             break;
         case KN_PRES_SWEEP:
         case KN_PRES_SWEEP2:
-        case KN_PRES_SWEEP3: {
-            const int n = k == KN_PRES_SWEEP3 ? 3 : (k == KN_PRES_SWEEP2 ? 2 : 1);
+        case KN_PRES_SWEEP3:
+        case KN_PRES_SWEEP4: {
+            const int n = k == KN_PRES_SWEEP4 ? 4 : (k == KN_PRES_SWEEP3 ? 3 : (k == KN_PRES_SWEEP2 ? 2 : 1));
             if (n == 2) CHECK(f2);
             if (n == 3) CHECK(f3);
+            if (n == 4) CHECK(f4);
             CHECK(r.pin != r.pout);
             if (r.pin < 0) CHECK(pres_in_solve == 0);
             else CHECK(pc[r.pin] == pres_in_solve);
@@ -93,9 +96,11 @@ struct Rec {
             e[0] = e[1] = GARBAGE;
             break;
         case KN_LOWP_SWEEP:
-        case KN_LOWP_SWEEP3: {
-            const int n = k == KN_LOWP_SWEEP3 ? 3 : 1;
+        case KN_LOWP_SWEEP3:
+        case KN_LOWP_SWEEP4: {
+            const int n = k == KN_LOWP_SWEEP4 ? 4 : (k == KN_LOWP_SWEEP3 ? 3 : 1);
             if (n == 3) CHECK(f3);
+            if (n == 4) CHECK(f4);
             const bool first = (r.flags & K_LOWP_FIRST) != 0, last = (r.flags & K_LOWP_LAST) != 0;
             int have = 0;
             if (!first) {
@@ -145,21 +150,22 @@ struct Rec {
     bool fused() const { return f2; }
     bool fused_vel() const { return fv; }
     bool fused3() const { return f3; }
-    bool fused4() const { return false; }
+    bool fused4() const { return f4; }
 };
 
-void run_case(int V, int P, int C, bool norms, bool f2, bool fv, bool f3, bool lowp, int R, int K, bool residual,
-              int state0) {
-    char name[200];
-    std::snprintf(name, sizeof name, "V=%d P=%d C=%d norms=%d pairs=%d vel_pairs=%d triples=%d lowp=%d R=%d K=%d "
-                                     "residual=%d state=%d",
-                  V, P, C, (int)norms, (int)f2, (int)fv, (int)f3, (int)lowp, R, K, (int)residual, state0);
+void run_case(int V, int P, int C, bool norms, int fl, bool lowp, int R, int K, bool residual, int state0) {
+    // fl: bit 0 pressure pairs, 1 momentum pairs, 2 triples, 3 four-sweep launches
+    const bool f2 = fl & 1, fv = (fl >> 1) & 1, f3 = (fl >> 2) & 1, f4 = (fl >> 3) & 1;
+    char name[220];
+    std::snprintf(name, sizeof name, "V=%d P=%d C=%d norms=%d pairs=%d vel_pairs=%d triples=%d fours=%d lowp=%d R=%d "
+                                     "K=%d residual=%d state=%d",
+                  V, P, C, (int)norms, (int)f2, (int)fv, (int)f3, (int)f4, (int)lowp, R, K, (int)residual, state0);
     g_case = name;
     StepCfg c;
     c.V = V; c.P = lowp ? R * K : P; c.C = C; c.piso = C > 0; c.norms = norms; c.lowp = lowp; c.R = R; c.K = K;
     c.cal_residual = residual; c.vci = 1; c.pci = 1; c.vtol = 1e-30f; c.ptol = 1e-30f;
     Rec l;
-    l.f2 = f2; l.fv = fv; l.f3 = f3; l.V = V; l.P = c.P; l.K = K;
+    l.f2 = f2; l.fv = fv; l.f3 = f3; l.f4 = f4; l.V = V; l.P = c.P; l.K = K;
     l.state = state0;
     l.vel[state0] = 0;
     RoleState rs;
@@ -191,20 +197,20 @@ int main() {
             for (int P : {0, 1, 2, 3, 4, 5, 7, 50})
                 for (int C : {0, 2})
                     for (int norms = 0; norms < 2; ++norms)
-                        for (int fl = 0; fl < 8; ++fl) {
-                            run_case(V, P, C, norms, fl & 1, (fl >> 1) & 1, (fl >> 2) & 1, false, 0, 0, false, state0);
+                        for (int fl = 0; fl < 16; ++fl) {
+                            run_case(V, P, C, norms, fl, false, 0, 0, false, state0);
                             ++ncases;
                         }
     // low-precision rounds (FP32 momentum, 2-byte corrections)
     for (int R : {1, 3})
-        for (int K : {1, 2, 3, 4, 7, 20})
-            for (int fl = 0; fl < 8; ++fl) {
-                run_case(6, 0, 2, true, fl & 1, (fl >> 1) & 1, (fl >> 2) & 1, true, R, K, false, 0);
+        for (int K : {1, 2, 3, 4, 5, 7, 8, 20})
+            for (int fl = 0; fl < 16; ++fl) {
+                run_case(6, 0, 2, true, fl, true, R, K, false, 0);
                 ++ncases;
             }
     // residual-based exit: single sweeps only, whatever the launcher offers
-    for (int fl = 0; fl < 8; ++fl) {
-        run_case(7, 9, 0, true, fl & 1, (fl >> 1) & 1, (fl >> 2) & 1, false, 0, 0, true, 1);
+    for (int fl = 0; fl < 16; ++fl) {
+        run_case(7, 9, 0, true, fl, false, 0, 0, true, 1);
         ++ncases;
     }
     if (g_fail) return 1;

@@ -79,17 +79,20 @@ def test_low_precision_defect_correction(gpu, prec, tol):
 @pytest.mark.parametrize("prec", [N.PREC_FP32_FP16, N.PREC_FP32_BF16])
 @pytest.mark.parametrize("sweeps,grid", [(20, (20, 14, 17)), (7, (20, 14, 17)), (7, (100, 21, 37))])
 def test_low_precision_triples_equal_single_sweeps(gpu, monkeypatch, prec, sweeps, grid):
-    """The three-sweep low-precision kernel (its first two sweeps rounded to 2
-    bytes as the stored iterate would be, the last of a round writing p')
-    gives the fields and defect norms of single low-precision sweeps bit for
-    bit, with the grid's z chunk and with 5- and 1-plane chunks."""
+    """The three- and four-sweep low-precision kernels (their intermediate
+    sweeps rounded to 2 bytes as the stored iterate would be, the last of a
+    round writing p') give the fields and defect norms of single
+    low-precision sweeps bit for bit, with the grid's z chunk and with 5-
+    and 1-plane chunks."""
     w = workloads.tiny(grid=grid)
     w.cf.precision, w.cf.refineRounds, w.cf.refineSweeps, w.cf.computeNorms = prec, 3, sweeps, True
     runs = [{"UWS_SWEEP_FUSION": "0"}, {"UWS_SWEEP_FUSION": "2"}, {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "5"},
-            {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "1"}]
+            {"UWS_SWEEP_FUSION": "2", "UWS_ZCHUNK3": "1"}, {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "4"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "4", "UWS_ZCHUNK4": "5"},
+            {"UWS_SWEEP_FUSION": "2", "UWS_SWEEP_DEPTH": "4", "UWS_ZCHUNK4": "1"}]
     res = []
     for env in runs:
-        for k in ("UWS_SWEEP_FUSION", "UWS_ZCHUNK3"):
+        for k in ("UWS_SWEEP_FUSION", "UWS_ZCHUNK3", "UWS_SWEEP_DEPTH", "UWS_ZCHUNK4"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)

@@ -297,15 +297,15 @@ int pres_solve(Lr& l, const StepCfg& c, int coef, int solve, int* latest, int* s
             r.coef = coef; r.pin = x;
             l(KN_DEFECT_RESIDUAL, r);
             l(KN_DEFECT_SCALE, r, UWS_NORM_REFINE0 + 8 * solve + rd);
-            // triples of sweeps as one three-sweep launch where the launcher
-            // has it; each launch writes the e buffer it does not read
+            // fours or triples of sweeps as one multi-sweep launch where the
+            // launcher has them; each launch writes the e buffer it does not read
             int ein = -1;
             for (int k = 0; k < c.K;) {
                 Roles s{};
                 s.coef = coef; s.pin = x; s.pout = 0; s.ein = ein; s.eout = ein == 0 ? 1 : 0;
-                const int n = (l.fused3() && k + 3 <= c.K) ? 3 : 1;
+                const int n = (l.fused4() && k + 4 <= c.K) ? 4 : ((l.fused3() && k + 3 <= c.K) ? 3 : 1);
                 s.flags = (k == 0 ? K_LOWP_FIRST : 0) | (k + n == c.K ? K_LOWP_LAST : 0) | (c.bf16 ? K_BF16 : 0);
-                l(n == 3 ? KN_LOWP_SWEEP3 : KN_LOWP_SWEEP, s);
+                l(n == 4 ? KN_LOWP_SWEEP4 : (n == 3 ? KN_LOWP_SWEEP3 : KN_LOWP_SWEEP), s);
                 ein = s.eout;
                 k += n;
             }

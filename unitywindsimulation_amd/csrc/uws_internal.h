@@ -137,7 +137,8 @@ enum Kernel {
     KN_VEL_SWEEP2,  // two momentum Jacobi sweeps in one launch (single case, uniform grid, FP32)
     KN_PRES_SWEEP3,  // three pressure Jacobi sweeps in one launch (single case, FP32)
     KN_LOWP_SWEEP3,  // three low-precision correction sweeps in one launch (single case, FP32)
-    KN_PRES_SWEEP4   // four pressure Jacobi sweeps in one launch (single case, FP32)
+    KN_PRES_SWEEP4,  // four pressure Jacobi sweeps in one launch (single case, FP32)
+    KN_LOWP_SWEEP4   // four low-precision correction sweeps in one launch (single case, FP32)
 };
 
 // Single-case launch of kernel k on stream s.  `slot` selects the norms

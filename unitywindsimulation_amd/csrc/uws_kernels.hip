@@ -1736,7 +1736,11 @@ inline __host__ __device__ int pres_sweep4_blocks(int X, int Y, int Z, int zchun
     return ((X + 63) / 64) * ((Y + PS4_BY - 7) / (PS4_BY - 6)) * ((Z + zchunk - 1) / zchunk);
 }
 
-template <bool ZIN>
+// LP, LAST: the low-precision correction equation as in the three-sweep
+// kernel (e read in 2 bytes, the first three sweeps' values rounded to 2
+// bytes as the stored iterate would be, the fourth to e_out or, as the
+// round's last sweep, to p' = x + e / s)
+template <bool ZIN, int LP = 0, bool LAST = false>
 __device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b, float* lds) {
     using T = float;
     const int tid = (int)threadIdx.x;
@@ -1770,11 +1774,20 @@ __device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b
         q.inv = ld_at(k.invDp, oP);
         q.rhs = ld_at(k.rhs, oP);
         if (!ZIN) {
-            q.vE = ld_at(x0, gE ? oP + (uint32_t)sizeof(T) : oP);
-            q.vW = ld_at(x0, oW);
-            q.vN = ld_at(x0, gN ? oP + rowB : oP);
-            q.vS = ld_at(x0, oS);
-            q.vU = ld_at(x0, oU);
+            if constexpr (LP == 0) {
+                q.vE = ld_at(x0, gE ? oP + (uint32_t)sizeof(T) : oP);
+                q.vW = ld_at(x0, oW);
+                q.vN = ld_at(x0, gN ? oP + rowB : oP);
+                q.vS = ld_at(x0, oS);
+                q.vU = ld_at(x0, oU);
+            } else {
+                const uint32_t es = (uint32_t)sizeof(T);
+                q.vE = ld_lp<LP == 2>(p.ein, (int)((gE ? oP + es : oP) / es));
+                q.vW = ld_lp<LP == 2>(p.ein, (int)(oW / es));
+                q.vN = ld_lp<LP == 2>(p.ein, (int)((gN ? oP + rowB : oP) / es));
+                q.vS = ld_lp<LP == 2>(p.ein, (int)(oS / es));
+                q.vU = ld_lp<LP == 2>(p.ein, (int)(oU / es));
+            }
         }
     };
     // the records below the first computed plane are read only in part:
@@ -1786,7 +1799,22 @@ __device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b
         q.cE = q.cW = q.cN = q.cS = q.inv = q.rhs = 0;
         q.vE = q.vW = q.vN = q.vS = 0;
         q.cU = with_cU ? ld_at(k.cU, oP) : (T)0;
-        q.vU = ZIN ? (T)0 : ld_at(x0, oU);
+        q.vU = 0;
+        if (!ZIN) {
+            if constexpr (LP == 0) q.vU = ld_at(x0, oU);
+            else q.vU = ld_lp<LP == 2>(p.ein, (int)(oU / (uint32_t)sizeof(T)));
+        }
+    };
+    const T sc = LP == 0 ? (T)0 : p.s;
+    // one sweep's value from its term, and the value the next sweep reads
+    auto comb = [&](T inv, T t, T rhs) {
+        if constexpr (LP == 0) return inv * (t + rhs);
+        else return inv * t + sc * rhs;
+    };
+    auto rnd = [&](T v) {
+        if constexpr (LP == 0) return v;
+        else if constexpr (LP == 2) return (T)__bfloat162float(__float2bfloat16((float)v));
+        else return (T)(float)(_Float16)(float)v;
     };
     const int jE = (hE && lx + 1 < PS4_W) ? tid + 1 : tid, jW = (hW && lx > 0) ? tid - 1 : tid;
     const int jN = (hN && ly + 1 < PS4_BY) ? tid + PS4_W : tid, jS = (hS && ly > 0) ? tid - PS4_W : tid;
@@ -1812,7 +1840,7 @@ __device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b
         t -= r.cS * v.s;
         t -= r.cU * vU;
         t -= cD * vDn;
-        return r.inv * (t + r.rhs);
+        return comb(r.inv, t, r.rhs);
     };
     T s1m1 = 0, s1m2 = 0;  // own first-sweep values at zz - 1, zz - 2
     T s2m1 = 0, s2m2 = 0;  // own second-sweep values at zz - 2, zz - 3
@@ -1846,24 +1874,34 @@ __device__ __forceinline__ void pres_sweep4_block(const PS2Args<float>& p, int b
             t1 -= r0.cU * r0.vU;
             t1 -= r1.cU * vD;
         }
-        const T s1 = r0.inv * (t1 + r0.rhs);
+        const T s1 = rnd(comb(r0.inv, t1, r0.rhs));
         L1[S0 * PS4_THREADS + tid] = s1;
         // second sweep at zz - 1 (needed from z0 - 2; +0 below)
         const int z2 = zz - 1;
-        const T s2v = sweep(r1, a2, z2 + 1 < p.Z ? s1 : s1m1, z2 > 0 ? s1m2 : s1m1, r2.cU);
+        const T s2v = rnd(sweep(r1, a2, z2 + 1 < p.Z ? s1 : s1m1, z2 > 0 ? s1m2 : s1m1, r2.cU));
         const T s2 = zz >= z0 - 1 ? s2v : (T)0;
         L2[S1 * PS4_THREADS + tid] = s2;
         // third sweep at zz - 2 (needed from z0 - 1; +0 below)
         const int z3 = zz - 2;
-        const T s3v = sweep(r2, a3, z3 + 1 < p.Z ? s2 : s2m1, z3 > 0 ? s2m2 : s2m1, r3.cU);
+        const T s3v = rnd(sweep(r2, a3, z3 + 1 < p.Z ? s2 : s2m1, z3 > 0 ? s2m2 : s2m1, r3.cU));
         const T s3 = zz >= z0 + 1 ? s3v : (T)0;
         L3[S2 * PS4_THREADS + tid] = s3;
         // fourth sweep at zz - 3 into the output
         const int z4 = zz - 3;
         const T e4 = sweep(r3, a4, z4 + 1 < p.Z ? s3 : s3m1, z4 > 0 ? s3m2 : s3m1, dU4);
         if (inner && z4 >= z0 && z4 < z1) {
-            // an inner thread's column is in the grid: colB is its own
-            st_at(out, (uint32_t)z4 * planeB + colB, e4);
+            if constexpr (LP == 0) {
+                // an inner thread's column is in the grid: colB is its own
+                st_at(out, (uint32_t)z4 * planeB + colB, e4);
+            } else {
+                const int P4 = (z4 * p.Y + y) * p.X + x;
+                if constexpr (LAST) {
+                    const T xv = p.xin ? p.xin[P4] : (T)0;
+                    out[P4] = xv + e4 / sc;
+                } else {
+                    st_lp<LP == 2>(p.eout, P4, (float)e4);
+                }
+            }
         }
         __syncthreads();
         s1m2 = s1m1;
@@ -1937,6 +1975,23 @@ __global__ __launch_bounds__(1024) void k_lowp_sweep3(ParamsT<float> p, FieldsT<
     a.out = f.pc[r.pout];
     a.s = f.scalar[1];
     pres_sweep3_block<float, ZIN, LP, LAST>(a, xcd_tile((int)blockIdx.x, nb), lds_f);
+}
+
+// four low-precision correction sweeps in one launch (the four-sweep
+// kernel's tiling; grids that take four-sweep pressure launches)
+template <bool ZIN, int LP, bool LAST>
+__global__ __launch_bounds__(1024) void k_lowp_sweep4(ParamsT<float> p, FieldsT<float> f, Roles r) {
+    __shared__ __align__(16) float lds_f[3 * PS4_RING * PS4_THREADS];
+    const int nb = pres_sweep4_blocks(p.X, p.Y, p.Z, p.zchunk4);
+    PS2Args<float> a = ps2_args(p, f, r);
+    a.zchunk = p.zchunk4;
+    a.k.rhs = f.rho;
+    a.ein = ZIN ? nullptr : f.e16[r.ein];
+    a.eout = f.e16[r.eout];
+    a.xin = r.pin >= 0 ? f.pc[r.pin] : nullptr;
+    a.out = f.pc[r.pout];
+    a.s = f.scalar[1];
+    pres_sweep4_block<ZIN, LP, LAST>(a, xcd_tile((int)blockIdx.x, nb), lds_f);
 }
 
 // ---------------------------------------------------------------------------
@@ -2556,6 +2611,24 @@ void launch(Kernel k, const ParamsT<T>& p, const FieldsT<T>& f, const Roles& r, 
                 else { if (last) UWS_LP3(false, 1, true); else UWS_LP3(false, 1, false); }
             }
 #undef UWS_LP3
+        }
+        return;
+    }
+    if (k == KN_LOWP_SWEEP4) {
+        if constexpr (std::is_same_v<T, float>) {
+            const int nb = pres_sweep4_blocks(p.X, p.Y, p.Z, p.zchunk4);
+            const bool first = (r.flags & K_LOWP_FIRST) != 0, last = (r.flags & K_LOWP_LAST) != 0;
+            const bool bf = (r.flags & K_BF16) != 0;
+#define UWS_LP4(Z, L, LAST)                                                                                \
+    hipLaunchKernelGGL((k_lowp_sweep4<Z, L, LAST>), dim3(nb), dim3(PS4_THREADS), 0, s, p, f, r)
+            if (bf) {
+                if (first) { if (last) UWS_LP4(true, 2, true); else UWS_LP4(true, 2, false); }
+                else { if (last) UWS_LP4(false, 2, true); else UWS_LP4(false, 2, false); }
+            } else {
+                if (first) { if (last) UWS_LP4(true, 1, true); else UWS_LP4(true, 1, false); }
+                else { if (last) UWS_LP4(false, 1, true); else UWS_LP4(false, 1, false); }
+            }
+#undef UWS_LP4
         }
         return;
     }
