@@ -465,8 +465,25 @@ int enqueue_step(Lr& l, const StepCfg& c, RoleState& rs) {
         r.flags = K_FUSE_LES;
     }
     l(KN_VEL_PRECOMP, r);
-    for (int s = UWS_STAGE_VEL_SWEEPS; s <= UWS_STAGE_VEL_CORRECT; ++s)
+    for (int s = UWS_STAGE_VEL_SWEEPS; s <= UWS_STAGE_PRES_SWEEPS; ++s)
         if (int rc = run_stage(l, c, rs, s)) return rc;
+    if (c.normalise) {
+        for (int s = UWS_STAGE_APPLY_PRES; s <= UWS_STAGE_VEL_CORRECT; ++s)
+            if (int rc = run_stage(l, c, rs, s)) return rc;
+    } else {
+        // p += p' (UWS_STAGE_APPLY_PRES) in the velocity correction's launch:
+        // no normalisation reads p in between, and the correction reads p'
+        // only
+        if (rs.cur < 0) return UWS_ERR_INVALID_ARG;
+        ensure_pc(l, rs);
+        Roles v{};
+        v.state = rs.state;
+        v.pin = rs.pc_latest;
+        v.vout = rs.cur;
+        v.flags = K_APPLY_P;
+        l(KN_VEL_CORRECT, v);
+        rs.pc_latest = -1;  // SetPresCorrectFieldtoZero (FvmSolverGpu.cs:1088)
+    }
     for (int n = 0; n < c.C; ++n)
         if (int rc = run_stage(l, c, rs, UWS_STAGE_PISO)) return rc;
     return run_stage(l, c, rs, UWS_STAGE_FINISH);

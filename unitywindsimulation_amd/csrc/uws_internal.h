@@ -121,7 +121,8 @@ enum {
     K_WALL = 1, K_EDDY = 2, K_FUSE_LES = 4, K_LOWP_FIRST = 16, K_LOWP_LAST = 32,
     K_BF16 = 64,
     K_NO_INPLACE = 128,  // corrections write u' only, not u += u' / p += p'
-    K_NO_FLUX = 256      // stretched-grid pressure precompute: leave velFaceFluxField as it is
+    K_NO_FLUX = 256,     // stretched-grid pressure precompute: leave velFaceFluxField as it is
+    K_APPLY_P = 512      // SIMPLE velocity correction: also p += p' in its fluid cells (CSApplyPresCorrection)
 };
 
 // kernel identifiers understood by launch()

@@ -833,6 +833,10 @@ __device__ void body_vel_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, con
 #pragma unroll
     for (int d = 0; d < 6; ++d) pQ[d] = pc[Qs[d]];
     const T DP[3] = {f.D[0][P], f.D[1][P], f.D[2][P]};
+    // with K_APPLY_P the launch also does CSApplyPresCorrection's p += p'
+    // (the step fuses the two where no normalisation follows)
+    const bool applyp = (r.flags & K_APPLY_P) != 0;
+    const T pP = applyp ? f.p[P] : (T)0;
     const bool inplace = (r.flags & K_NO_INPLACE) == 0;
     T u0[3] = {0, 0, 0};
     if (inplace) {
@@ -840,6 +844,7 @@ __device__ void body_vel_correct_t(const ParamsT<T>& p, const FieldsT<T>& f, con
         u0[0] = u[0][P]; u0[1] = u[1][P]; u0[2] = u[2][P];
     }
     if (!isF) return;
+    if (applyp) f.p[P] = pP + pcP;
     const T iv[3] = {(T)1.0f / DP[0], (T)1.0f / DP[1], (T)1.0f / DP[2]};
     T pf[6];
 #pragma unroll

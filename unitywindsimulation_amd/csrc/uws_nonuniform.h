@@ -360,6 +360,8 @@ __device__ void body_nu_vel_correct(const ParamsT<T>& p, const FieldsT<T>& f, co
     const NuGeo<T> g = nu_geo(p, f, x, y, z);
     const T* __restrict__ pc = f.pc[r.pin];
     const T pcP = pc[P];
+    // K_APPLY_P: CSApplyPresCorrection's p += p' in the same launch
+    if (r.flags & K_APPLY_P) f.p[P] = f.p[P] + pcP;
     T pf[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
