@@ -35,31 +35,34 @@ def bytes_per_cell_step(V: int, P: int, C: int, les: bool = True) -> int:
 
 
 def compulsory_bytes_per_cell_step(V: int, P: int, C: int, les: bool = True, pres_depth: int = 3,
-                                   vel_depth: int = 2) -> int:
+                                   vel_depth: int = 2, normalise: bool = False) -> int:
     """Bytes per cell one step of the solver's launch plan must move, each array once per
     launch: wall function + eddy viscosity with the gradient store (78), the
     momentum precompute with the LES term and the gradient read (146), V / 2
     momentum pairs (73 each, a leftover single sweep 77; vel_depth 1: V single
     sweeps), the Rhie-Chow precompute (37), per pressure solve P / pres_depth
     launches of pres_depth sweeps (28 each; a leftover triple or pair 28, a
-    single 29), p += p' (13), the velocity correction (53) and per PISO
-    corrector A_od u' (37), the corrector's right-hand side (4) and the
-    correction (78).  3,818 B/cell at V = P = 50, C = 2 with triples, 3,482
-    with four-sweep launches."""
+    single 29), the velocity correction with p += p' in the same launch (53 +
+    8; with the pressure normalisation they are separate launches, 53 + 13)
+    and per PISO corrector A_od u' (37), the corrector's right-hand side (4)
+    and the correction (78).  3,813 B/cell at V = P = 50, C = 2 with triples,
+    3,477 with four-sweep launches."""
     d = max(1, pres_depth)
     solve = (P // d) * 28 + (0 if P % d == 0 else (29 if P % d == 1 else 28)) if d > 1 else 29 * P
     mom = (V // 2) * 73 + (V % 2) * 77 if vel_depth == 2 else 77 * V
     one_shot = (78 + 146) if les else 69
-    return one_shot + mom + 37 + solve + 13 + 53 + C * (37 + 4 + solve + 78)
+    correct = 53 + 13 if normalise else 53 + 8
+    return one_shot + mom + 37 + solve + correct + C * (37 + 4 + solve + 78)
 
 
 # HBM bytes per cell each kernel moved per launch on the urban case
 # (256 x 96 x 128), read from the FETCH_SIZE (scaled by 2.000) and WRITE_SIZE
 # counters (profiles/urban_256x96x128_hbm_traffic.md); "_zin": the first
-# launch of a pressure solve, which starts from p' = 0
+# launch of a pressure solve, which starts from p' = 0; k_vel_correct
+# includes the p += p' it applies in the same launch
 MEASURED_LAUNCH_B_PER_CELL = {"urban": {
-    "k_wall_eddy": 66.9, "k_vel_precompute": 165.2, "k_pres_precompute": 59.3, "k_apply_pres": 13.0,
-    "k_vel_correct": 53.5, "k_piso_aodu": 38.9, "k_piso_precompute": 46.2, "k_piso_correct": 73.5,
+    "k_wall_eddy": 66.9, "k_vel_precompute": 165.2, "k_pres_precompute": 59.3,
+    "k_vel_correct": 61.5, "k_piso_aodu": 38.9, "k_piso_precompute": 46.2, "k_piso_correct": 73.5,
     "k_vel_sweep": 75.9, "k_vel_sweep2": 84.6, "k_pres_sweep": 29.7, "k_pres_sweep2": 32.2,
     "k_pres_sweep3": 35.2, "k_pres_sweep4": 36.2, "k_pres_sweep4_zin": 30.5}}
 
@@ -80,8 +83,8 @@ def measured_bytes_per_cell_step(workload: str, V: int, P: int, C: int, les: boo
     if n_main > 0 and names[d] + "_zin" in m:
         solve += m[names[d] + "_zin"] - m[names[d]]
     mom = (V // 2) * m["k_vel_sweep2"] + (V % 2) * m["k_vel_sweep"] if vel_depth == 2 else V * m["k_vel_sweep"]
-    total = (m["k_wall_eddy"] + m["k_vel_precompute"] + mom + m["k_pres_precompute"] + solve + m["k_apply_pres"]
-             + m["k_vel_correct"] + C * (m["k_piso_aodu"] + m["k_piso_precompute"] + solve + m["k_piso_correct"]))
+    total = (m["k_wall_eddy"] + m["k_vel_precompute"] + mom + m["k_pres_precompute"] + solve + m["k_vel_correct"]
+             + C * (m["k_piso_aodu"] + m["k_piso_precompute"] + solve + m["k_piso_correct"]))
     return round(total, 1)
 
 

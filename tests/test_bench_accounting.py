@@ -37,19 +37,23 @@ def test_step_bytes_per_kernel_table(bench):
 
 
 def test_compulsory_bytes_of_the_launch_plans(bench):
-    # one-shot kernels 78 + 146, 25 momentum pairs of 73, Rhie-Chow 37, p += p' 13, velocity correction 53,
-    # per corrector A_od u' 37, right-hand side 4 and correction 78; a solve of 50 sweeps is 16 triples and
-    # one pair (17 launches of 28) or 12 four-sweep launches and one pair (13 of 28)
-    fixed = 78 + 146 + 25 * 73 + 37 + 13 + 53 + 2 * (37 + 4 + 78)
-    assert bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=3) == fixed + 3 * 17 * 28 == 3818
-    assert bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=4) == fixed + 3 * 13 * 28 == 3482
+    # one-shot kernels 78 + 146, 25 momentum pairs of 73, Rhie-Chow 37, the velocity correction 53 with
+    # p += p' in its launch (+ 8: p read and written), per corrector A_od u' 37, right-hand side 4 and
+    # correction 78; a solve of 50 sweeps is 16 triples and one pair (17 launches of 28) or 12 four-sweep
+    # launches and one pair (13 of 28)
+    fixed = 78 + 146 + 25 * 73 + 37 + 53 + 8 + 2 * (37 + 4 + 78)
+    assert bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=3) == fixed + 3 * 17 * 28 == 3813
+    assert bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=4) == fixed + 3 * 13 * 28 == 3477
+    # with the pressure normalisation p += p' is a launch of its own (13)
+    assert (bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=4, normalise=True)
+            - bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=4)) == 13 - 8
     # single sweeps everywhere: every sweep one pass of its arrays
     single = bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=1, vel_depth=1)
-    assert single == 78 + 146 + 50 * 77 + 37 + 13 + 53 + 2 * (37 + 4 + 78) + 3 * 50 * 29
+    assert single == 78 + 146 + 50 * 77 + 37 + 53 + 8 + 2 * (37 + 4 + 78) + 3 * 50 * 29
     # a leftover sweep of a solve: a single launch (29), two or three: one multi-sweep launch (28)
     for P, depth, launches28, singles in ((49, 4, 12, 1), (51, 4, 13, 0), (50, 3, 17, 0), (49, 3, 16, 1)):
         got = bench.compulsory_bytes_per_cell_step(50, P, 0, pres_depth=depth)
-        assert got == 78 + 146 + 25 * 73 + 37 + 13 + 53 + launches28 * 28 + singles * 29, (P, depth)
+        assert got == 78 + 146 + 25 * 73 + 37 + 53 + 8 + launches28 * 28 + singles * 29, (P, depth)
     # an odd momentum count: the leftover sweep is a single launch (77)
     assert (bench.compulsory_bytes_per_cell_step(51, 50, 2, pres_depth=4)
             - bench.compulsory_bytes_per_cell_step(50, 50, 2, pres_depth=4)) == 77
@@ -69,9 +73,9 @@ def test_more_than_one_gpu_is_refused():
 def test_counter_measured_step_bytes(bench):
     # urban plan (V = P = 50, C = 2, four-sweep pressure launches, momentum pairs), summed by hand from the
     # per-launch counter table: 66.9 + 165.2 (wall function, momentum precompute) + 25 x 84.6 (pairs)
-    # + 59.3 + 13.0 + 53.5 (Rhie-Chow precompute, p += p', velocity correction) + 3 solves of
-    # (30.5 + 11 x 36.2 + 32.2) + 2 x (38.9 + 46.2 + 73.5) = 4172.8
-    assert bench.measured_bytes_per_cell_step("urban", 50, 50, 2, True, 4, 2) == 4172.8
+    # + 59.3 + 61.5 (Rhie-Chow precompute, velocity correction with p += p') + 3 solves of
+    # (30.5 + 11 x 36.2 + 32.2) + 2 x (38.9 + 46.2 + 73.5) = 4167.8
+    assert bench.measured_bytes_per_cell_step("urban", 50, 50, 2, True, 4, 2) == 4167.8
     # no profile of other workloads or of the laminar plan
     assert bench.measured_bytes_per_cell_step("isolated_building", 50, 50, 2) is None
     assert bench.measured_bytes_per_cell_step("urban", 50, 50, 2, les=False) is None
